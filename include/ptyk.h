@@ -1,0 +1,145 @@
+/*
+ * ptyk.h -- C ABI of the gfx950 ptychography engine.
+ *
+ * One handle owns the device state of one reconstruction: the FP16 frame store, the
+ * scan positions, the object canvas, the probe modes and the per-method solver state.
+ * Handles replace the reference's process-wide singletons (CXParams, FFTPlanner,
+ * PhaserUtil; Parameters.h:203, CudaFFTPlan.h:125, utilities.h:639), so several
+ * reconstructions can live in one process.
+ *
+ * All pointers are HOST pointers unless stated otherwise; sizes are element counts.
+ * Complex arrays are interleaved float32 (re, im), row-major, no pitch padding.
+ * Every function returns PTYK_OK (0) or a negative PTYK_E* code and never exits the
+ * process (the reference calls exit() on errors: utilities.h:73-92, ePIE.cpp:100-105).
+ */
+#ifndef PTYK_H
+#define PTYK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ptyk_ctx ptyk_ctx;
+
+enum {
+  PTYK_OK = 0,
+  PTYK_EINVAL = -1,  /* bad argument (shape, size, null pointer, unsupported M) */
+  PTYK_ENOMEM = -2,  /* device allocation failed */
+  PTYK_EDEVICE = -3, /* HIP runtime error */
+  PTYK_ESTATE = -4,  /* call order: frames/positions/probe not set yet */
+  PTYK_ERANGE = -5   /* a scan position puts a ROI outside the object canvas */
+};
+
+/* Iteration flags, the booleans IPhaser::phaseStep passes to IPhasingMethod::iteration
+ * (IPhaser.cpp:391-398; IPhasingMethod.h:53-69). */
+enum {
+  PTYK_FLAG_PHASE_CONSTRAINT = 1u, /* i <= phaseConstraint */
+  PTYK_FLAG_UPDATE_PROBE = 2u,     /* i >= updateProbe */
+  PTYK_FLAG_UPDATE_MODES = 4u,     /* i >= updateProbeModes */
+  PTYK_FLAG_RMS = 8u               /* accumulate the Fourier error (ePIE) */
+};
+
+/* Scalar state slots readable with ptyk_get_scalars. */
+enum {
+  PTYK_SCAL_OBJ_MAX = 0,   /* Sample::m_maxObjectIntensity (max |O|^2, or sum for DM) */
+  PTYK_SCAL_PROBE_MAX = 1, /* Probe::m_maxIntensity (max |P|^2, or sum for DM) */
+  PTYK_SCAL_ERROR = 2,     /* last iteration's error accumulator (ePIE: sum over positions) */
+  PTYK_SCAL_DM_FACTOR_OBJ = 4,   /* DM cached 1/objMax (probe-update factor; DM.cu:46-73) */
+  PTYK_SCAL_DM_FACTOR_PROBE = 5, /* DM cached 1/probeMax (object-update factor) */
+  PTYK_SCAL_COUNT = 8
+};
+
+typedef struct {
+  int32_t n_frames;    /* N diffraction patterns */
+  int32_t frame_size;  /* M: square frames M x M, power of two in [16, 512] */
+  int32_t n_modes;     /* K probe modes (Probe(size, modes), Probe.cpp:54-68) */
+  int32_t obj_rows;    /* object canvas (Sample(arrayShapeX, arrayShapeY), Sample.cpp:55-62) */
+  int32_t obj_cols;
+  int32_t device;      /* HIP device ordinal */
+  int32_t tex_quantize;/* 1: bilinear weights rounded to 1/256 as a texture unit does; 0: exact */
+  int32_t reserved;
+} ptyk_config;
+
+/* Create / destroy.  Replaces the device allocations of Sample, Probe, Diffractions
+ * (Sample.cpp:55-62, Probe.cpp:54-68, Diffractions.cpp:159-186). */
+int ptyk_create(const ptyk_config* cfg, ptyk_ctx** out);
+void ptyk_destroy(ptyk_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed ptyk_create when ctx is NULL). */
+const char* ptyk_last_error(const ptyk_ctx* ctx);
+
+/* Frame store.  Amplitudes (not intensities), DC at [0,0], N x M x M
+ * (Diffractions::loadarr, Diffractions.cpp:159-186).  Stored as FP16 with a per-frame
+ * power-of-two FP32 scale; ptyk_get_frames_f32 returns the dequantised values the
+ * solvers actually use. */
+int ptyk_set_frames_f32(ptyk_ctx* ctx, const float* amplitudes);
+int ptyk_set_frames_f16(ptyk_ctx* ctx, const uint16_t* halves, const float* scales);
+int ptyk_get_frames_f32(ptyk_ctx* ctx, float* out);
+int ptyk_get_frame_scales(ptyk_ctx* ctx, float* out);
+
+/* Scan positions in pixels, N x (row, col) float32, already shifted to the canvas
+ * (IPtychoScanMesh::addScanPosition, IPtychoScanMesh.cpp:102-109).  ROI origins are
+ * roundf(pos) (ePIE.cpp:114-115, SampleKernels.cu:426); every ROI must lie inside
+ * the canvas, else PTYK_ERANGE. */
+int ptyk_set_positions(ptyk_ctx* ctx, const float* pos_rc);
+/* The rounded ROI origins the engine uses, N x (row, col) int32. */
+int ptyk_get_roi(ptyk_ctx* ctx, int32_t* out);
+
+/* Object canvas rows x cols complex64 and probe modes K x M x M complex64
+ * (Sample::loadarr / Probe::init with probearr; Sample.cpp:184-188, Probe.cpp:342-359). */
+int ptyk_set_object(ptyk_ctx* ctx, const float* obj);
+int ptyk_get_object(ptyk_ctx* ctx, float* obj);
+int ptyk_clear_object(ptyk_ctx* ctx); /* Sample::clearObjectArray (Sample.cpp:216-220) */
+int ptyk_set_probe(ptyk_ctx* ctx, const float* probe);
+int ptyk_get_probe(ptyk_ctx* ctx, float* probe);
+/* DM exit-wave state, N x M x M complex64 (DM::m_psi, DM.cu:95-100). */
+int ptyk_set_psi(ptyk_ctx* ctx, const float* psi);
+int ptyk_get_psi(ptyk_ctx* ctx, float* psi);
+
+/* Scalar state; setting the DM factor slots also marks the DM cache as initialised. */
+int ptyk_get_scalars(ptyk_ctx* ctx, double* out /* PTYK_SCAL_COUNT */);
+int ptyk_set_scalars(ptyk_ctx* ctx, const double* in /* PTYK_SCAL_COUNT */);
+/* Recompute objMax = max |O|^2 and probeMax = max |P|^2 from the current arrays
+ * (Sample/Probe::updateIntensities with useSum = false). */
+int ptyk_refresh_maxima(ptyk_ctx* ctx);
+
+/* Initial probe for ePIE/DM (Probe::init without a guess, Probe.cpp:360-372):
+ * P0 = |sinc(r dx_s/beamSize) exp(-r^2/w^2)|, w = M/(2*2.35); P0 <- IFFT(mean_j(A_j)
+ * exp(i arg FFT(P0)))/M^2; all modes normalised to max|P| = 1 (Probe::normalize). */
+int ptyk_init_probe(ptyk_ctx* ctx, double beam_size, double dx_s);
+
+/* ePIE modal reconstruction start / end (Probe::beginModalReconstruction and
+ * endModalReconstruction, Probe.cpp:1768-1782; initProbeModes, Probe.cpp:320-330). */
+int ptyk_begin_modal(ptyk_ctx* ctx);
+int ptyk_end_modal(ptyk_ctx* ctx);
+
+/* One ePIE iteration over all positions in the given order (ePIE::iteration,
+ * ePIE.cpp:65-129).  order: N frame indices (a permutation).  err (nullable)
+ * receives the averaged Fourier error when PTYK_FLAG_RMS is set, else 0. */
+int ptyk_epie_iteration(ptyk_ctx* ctx, const int32_t* order, uint32_t flags, double* err);
+
+/* One DM iteration (DM::iteration, DM.cu:80-128).  The first call takes the cached
+ * normalisation factors from the current objMax/probeMax (DM.cu:88-92). */
+int ptyk_dm_iteration(ptyk_ctx* ctx, uint32_t flags);
+/* Same, timing each of the 7 kernel launches with HIP events on the engine stream:
+ * kernel_ms[7] = {rows_fwd, cols, rows_inv, probe_partial, object, probe_apply, finish}. */
+int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
+
+/* Batched in-place 2D FFT of n frames of M x M complex64 (host buffers), forward
+ * exp(-2 pi i kn/M), inverse unnormalised (CudaFFTPlan::getC2CPlan, CudaFFTPlan.cpp:63-101). */
+int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);
+
+/* Simulator (Diffractions::simulate, Diffractions.cpp:80-116): for each of n positions,
+ * |FFT2(bilinear(obj, pos) * probe)| (squared = 0) or its square (squared = 1).
+ * obj is rows x cols, probe M x M, pos n x (row, col); out n x M x M float32. */
+int ptyk_simulate(ptyk_ctx* ctx, const float* obj, int32_t rows, int32_t cols, const float* probe,
+                  const float* pos_rc, int64_t n, int32_t squared, float* out);
+
+int ptyk_synchronize(ptyk_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTYK_H */

@@ -1,0 +1,361 @@
+// FP64 CPU oracle of the reference's hot-path equations -- TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load this library,
+// and only as the checker / the timed CPU baseline.  The engine never calls it.
+//
+// Each function restates the reference algorithm in double precision, in the order the
+// reference applies it (not in the fused form the GPU engine uses), citing the reference
+// file:line it follows.  Complex arrays are interleaved (re, im) doubles, row-major.
+//
+// Pins (see tests/test_oracle_*.py): orc_fft2 against numpy.fft and closed forms;
+// the modulus projection against its closed form and idempotence; ePIE/DM updates against
+// fixed points (exact data => zero update) and hand-computed 1-frame cases; bilinear
+// extraction against integer gathers and separable linear interpolation; the mesh against
+// float32 closed forms.  Functions without such a check are marked "parity unpinned".
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cd;
+
+namespace {
+
+// ------------------------------------------------------------------ FFT
+// Iterative radix-2 decimation-in-time with bit reversal; forward exp(-2 pi i kn/N),
+// inverse unnormalised (cuFFT conventions, CudaFFTPlan.cpp:91-101).
+struct Fft1 {
+  int n = 0, logn = 0;
+  std::vector<int> rev;
+  std::vector<cd> w;  // exp(-2 pi i k / n), k < n/2
+  explicit Fft1(int n_) : n(n_) {
+    while ((1 << logn) < n) ++logn;
+    rev.resize(n);
+    for (int i = 0; i < n; ++i) {
+      int r = 0;
+      for (int b = 0; b < logn; ++b)
+        if (i & (1 << b)) r |= 1 << (logn - 1 - b);
+      rev[i] = r;
+    }
+    w.resize(n / 2 > 0 ? n / 2 : 1);
+    for (int k = 0; k < n / 2; ++k) w[k] = std::polar(1.0, -2.0 * M_PI * k / n);
+  }
+  void run(cd* x, long stride, bool inv) const {
+    std::vector<cd> a(n);
+    for (int i = 0; i < n; ++i) a[rev[i]] = x[i * stride];
+    for (int len = 2; len <= n; len <<= 1) {
+      const int half = len / 2, step = n / len;
+      for (int s = 0; s < n; s += len)
+        for (int k = 0; k < half; ++k) {
+          cd t = w[k * step];
+          if (inv) t = std::conj(t);
+          const cd u = a[s + k], v = a[s + k + half] * t;
+          a[s + k] = u + v;
+          a[s + k + half] = u - v;
+        }
+    }
+    for (int i = 0; i < n; ++i) x[i * stride] = a[i];
+  }
+};
+
+void fft2(cd* f, int m, bool inv, const Fft1& p) {
+  for (int r = 0; r < m; ++r) p.run(f + (long)r * m, 1, inv);
+  for (int c = 0; c < m; ++c) p.run(f + c, m, inv);
+}
+
+// exp(i arg f) with arg 0 for f == 0 whatever the signs of the zeros.  The reference computes
+// atan2f(f.y, f.x) on cuFFT output (utilitiesKernels.cu:626-631, SampleKernels.cu:441-444);
+// for an all-zero input (first DM/ePIE iteration from O = 0) the signs of cuFFT's zeros, and
+// so atan2's +-pi versus 0, are not specified -- this value is the one for +0 inputs.
+inline cd unit_phase(cd f) {
+  if (f.real() == 0.0 && f.imag() == 0.0) return cd(1.0, 0.0);
+  return std::polar(1.0, std::atan2(f.imag(), f.real()));
+}
+
+inline cd clamp_phase(cd u) {  // SampleKernels.cu:209-215
+  double arg = std::atan2(u.imag(), u.real());
+  arg = arg <= 0 ? arg : 0;
+  return std::polar(std::abs(u), arg);
+}
+
+// Linear-filtered, clamp-addressed, unnormalised texture fetch at (u, v)
+// (SampleKernels.cu:167-178, 493-508).  quantize: weights rounded to 1/256.
+inline cd tex_bilinear(const cd* o, int rows, int cols, float u, float v, int quantize) {
+  const float xb = u - 0.5f, yb = v - 0.5f;
+  const float fx = std::floor(xb), fy = std::floor(yb);
+  double al = (double)(xb - fx), be = (double)(yb - fy);
+  if (quantize) {
+    al = std::nearbyint(al * 256.0) / 256.0;
+    be = std::nearbyint(be * 256.0) / 256.0;
+  }
+  auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  const int i0 = cl((int)fx, cols - 1), i1 = cl((int)fx + 1, cols - 1);
+  const int j0 = cl((int)fy, rows - 1), j1 = cl((int)fy + 1, rows - 1);
+  return (1 - al) * (1 - be) * o[(long)j0 * cols + i0] + al * (1 - be) * o[(long)j0 * cols + i1] +
+         (1 - al) * be * o[(long)j1 * cols + i0] + al * be * o[(long)j1 * cols + i1];
+}
+
+inline cd* C(double* p) { return reinterpret_cast<cd*>(p); }
+inline const cd* C(const double* p) { return reinterpret_cast<const cd*>(p); }
+
+}  // namespace
+
+extern "C" {
+
+int orc_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Batched 2D FFT, n frames of m x m.
+void orc_fft2(double* data, int64_t n, int m, int inverse) {
+  Fft1 p(m);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t i = 0; i < n; ++i) fft2(C(data) + i * (long)m * m, m, inverse != 0, p);
+}
+
+// ROI extraction: bilinear fetch at (row + pos.x, col + pos.y) (d_extractObjectArray).
+void orc_extract_bilinear(const double* obj, int rows, int cols, float px, float py, int m, int quantize,
+                          double* out) {
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j)
+      C(out)[(long)i * m + j] =
+          tex_bilinear(C(obj), rows, cols, (float)j + py + 0.5f, (float)i + px + 0.5f, quantize);
+}
+
+// Single-mode Fourier modulus projection (PhaserUtil::applyModulusConstraint with
+// d_adjustFFT, utilities.cpp:138-154, utilitiesKernels.cu:633-653):
+// psi <- IFFT_u( A exp(i atan2(Phi)) / M^2 ),  Phi = FFT(psi).
+void orc_modulus_projection(double* psi, const double* amp, int m) {
+  Fft1 p(m);
+  cd* f = C(psi);
+  fft2(f, m, false, p);
+  const double nrm = 1.0 / ((double)m * m);
+  for (long q = 0; q < (long)m * m; ++q) f[q] = amp[q] * unit_phase(f[q]) * nrm;
+  fft2(f, m, true, p);
+}
+
+// Initial probe (Probe::init without a guess, Probe.cpp:360-372; d_initProbe
+// ProbeKernels.cu:207-218; d_normalizeDiffractionIntensities ProbeKernels.cu:388-407;
+// Probe::normalize Probe.cpp:303-318).  probe: k_total x m x m (modes >= 1 zeroed).
+// Returns max |P|^2 after normalisation.
+double orc_probe_init(const double* amp, int64_t n, int m, int k_total, double beam_size, double dx_s,
+                      double* probe) {
+  const long mm = (long)m * m;
+  std::vector<double> avg(mm, 0.0);
+  for (int64_t j = 0; j < n; ++j)
+    for (long q = 0; q < mm; ++q) avg[q] += amp[j * mm + q];
+  for (long q = 0; q < mm; ++q) avg[q] /= (double)n;
+  const double window = (double)m / (2 * 2.35), factor = dx_s / beam_size;
+  std::fill(probe, probe + 2 * mm * k_total, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      const double a = i - m * 0.5, b = j - m * 0.5;
+      const double r = std::sqrt(a * a + b * b);
+      double v = r * factor;
+      v = v != 0 ? std::sin(M_PI * v) / (M_PI * v) : 1.0;
+      const double g = std::exp(-r * r / (window * window));
+      C(probe)[(long)i * m + j] = std::abs(v * g);
+    }
+  orc_modulus_projection(probe, avg.data(), m);
+  double mx = 0;
+  for (long q = 0; q < mm * k_total; ++q) mx = std::max(mx, std::abs(C(probe)[q]));
+  double m2 = 0;
+  for (long q = 0; q < mm * k_total; ++q) {
+    C(probe)[q] /= mx;
+    m2 = std::max(m2, std::norm(C(probe)[q]));
+  }
+  return m2;
+}
+
+// Simulator (Diffractions::simulate, Diffractions.cpp:80-116; d_simulatePSI
+// SampleKernels.cu:113-122): out_j = |FFT2(bilinear(O, pos_j) * P)| (or squared).
+void orc_simulate(const double* obj, int rows, int cols, const double* probe, const float* pos, int64_t n, int m,
+                  int squared, int quantize, double* out) {
+  Fft1 p(m);
+  const long mm = (long)m * m;
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t j = 0; j < n; ++j) {
+    std::vector<cd> f(mm);
+    for (int i = 0; i < m; ++i)
+      for (int c = 0; c < m; ++c)
+        f[(long)i * m + c] = tex_bilinear(C(obj), rows, cols, (float)c + pos[2 * j + 1] + 0.5f,
+                                          (float)i + pos[2 * j] + 0.5f, quantize) *
+                             C(probe)[(long)i * m + c];
+    fft2(f.data(), m, false, p);
+    for (long q = 0; q < mm; ++q) out[j * mm + q] = squared ? std::norm(f[q]) : std::abs(f[q]);
+  }
+}
+
+// One ePIE iteration (ePIE::iteration, ePIE.cpp:65-129).
+//  obj rows x cols, obj_int rows x cols (|O|^2 cache), probe / probe_int k_total x m x m,
+//  amp n x m x m amplitudes, amp_sum n (sum of amplitudes per frame), pos n x (row, col)
+//  float32, order: n indices.  scal: [0] objMax, [1] probeMax (in/out), [2] error sum (out).
+//  k_use modes take part (K when modal reconstruction has begun, else 1); modal selects
+//  d_adjustModalFFT and the modal probe maximum.
+void orc_epie_iteration(double* obj, double* obj_int, int rows, int cols, double* probe, double* probe_int,
+                        int k_total, int k_use, int modal, const double* amp, const double* amp_sum,
+                        const float* pos, const int32_t* order, int64_t n, int m, int phase_constraint,
+                        int update_probe, int rms, int quantize, double* scal) {
+  Fft1 p(m);
+  const long mm = (long)m * m;
+  cd* O = C(obj);
+  cd* P = C(probe);
+  std::vector<cd> psi_old((size_t)k_use * mm), psi((size_t)k_use * mm);
+  double obj_max = scal[0], probe_max = scal[1], err = 0.0;
+  const double nrm = 1.0 / (double)mm;
+  for (int64_t s = 0; s < n; ++s) {
+    const int fi = order[s];
+    const float px = pos[2 * fi], py = pos[2 * fi + 1];
+    const double* A = amp + (long)fi * mm;
+    // psi_old_k = ROI * P_k (CXMath::multiply with firstAElementOnly, CXMath.h:49-58)
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        const cd roi = tex_bilinear(O, rows, cols, (float)j + py + 0.5f, (float)i + px + 0.5f, quantize);
+        for (int k = 0; k < k_use; ++k) psi_old[(size_t)k * mm + i * m + j] = roi * P[(size_t)k * mm + i * m + j];
+      }
+    psi = psi_old;
+    for (int k = 0; k < k_use; ++k) fft2(psi.data() + (size_t)k * mm, m, false, p);
+    for (long q = 0; q < mm; ++q) {
+      if (k_use == 1) {
+        psi[q] = A[q] * unit_phase(psi[q]) * nrm;
+      } else {
+        double s2 = 0;
+        for (int k = 0; k < k_use; ++k) s2 += std::norm(psi[(size_t)k * mm + q]);
+        const double f = A[q] / std::sqrt(s2) * nrm;
+        for (int k = 0; k < k_use; ++k) psi[(size_t)k * mm + q] *= f;
+      }
+    }
+    for (int k = 0; k < k_use; ++k) fft2(psi.data() + (size_t)k * mm, m, true, p);
+    // object update at the rounded position (d_updateObjectArray(M), SampleKernels.cu:244-294)
+    const int rx = (int)std::round(px), ry = (int)std::round(py);  // roundf: half away from zero
+    const double co = 1.0 / probe_max;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        cd u = 0;
+        for (int k = 0; k < k_use; ++k) {
+          const size_t q = (size_t)k * mm + i * m + j;
+          u += std::conj(P[q]) * (psi[q] - psi_old[q]);
+        }
+        const long oi = (long)(rx + i) * cols + (ry + j);
+        cd o = O[oi] + u * co;
+        const double mag = std::abs(o);
+        if (phase_constraint) o = clamp_phase(o);
+        O[oi] = o;
+        obj_int[oi] = mag * mag;
+      }
+    obj_max = 0;
+    for (long q = 0; q < (long)rows * cols; ++q) obj_max = std::max(obj_max, obj_int[q]);
+    // probe update with the updated object (d_updateProbe(M), ProbeKernels.cu:425-458)
+    if (update_probe) {
+      const double cp = 1.0 / obj_max;
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+          const cd oc = std::conj(O[(long)(rx + i) * cols + (ry + j)]);
+          for (int k = 0; k < k_use; ++k) {
+            const size_t q = (size_t)k * mm + i * m + j;
+            P[q] += oc * (psi[q] - psi_old[q]) * cp;
+            probe_int[q] = std::norm(P[q]);
+          }
+        }
+      probe_max = 0;
+      if (modal) {
+        for (long q = 0; q < mm; ++q) {
+          double sm = 0;
+          for (int k = 0; k < k_total; ++k) sm += probe_int[(size_t)k * mm + q];
+          probe_max = std::max(probe_max, sm);
+        }
+      } else {
+        for (long q = 0; q < mm * k_total; ++q) probe_max = std::max(probe_max, probe_int[q]);
+      }
+    }
+    // Fourier error (PhaserUtil::calculateER + d_calculateER; utilities.cpp:188-195)
+    if (rms) {
+      for (int k = 0; k < k_use; ++k) fft2(psi_old.data() + (size_t)k * mm, m, false, p);
+      double acc = 0;
+      for (long q = 0; q < mm; ++q) {
+        cd sm = 0;
+        for (int k = 0; k < k_use; ++k) sm += psi_old[(size_t)k * mm + q];
+        acc += std::fabs(A[q] - std::abs(sm));
+      }
+      err += acc / k_use / amp_sum[fi];
+    }
+  }
+  scal[0] = obj_max;
+  scal[1] = probe_max;
+  scal[2] = err;
+}
+
+// One DM iteration (DM::iteration, DM.cu:80-128, with the callbacks d_calculatePsi,
+// d_modulusConstraint and d_updateObject, SampleKernels.cu:418-488).  Object and probe
+// reads use the values at the start of the iteration (the reference reads them while
+// other threads add to them); all increments are summed.
+//  psi n x m x m (in/out), probe m x m (mode 0), factors[2] = cached {1/objMax, 1/probeMax}
+//  (in/out, DM.cu:46-73), sums[2] = {sum |O|^2, sum |P|^2} after the update (out).
+void orc_dm_iteration(double* obj, int rows, int cols, double* probe, double* psi, const double* amp,
+                      const float* pos, int64_t n, int m, int phase_constraint, int update_probe, double* factors,
+                      double* sums) {
+  Fft1 p(m);
+  const long mm = (long)m * m;
+  cd* O = C(obj);
+  cd* P = C(probe);
+  const std::vector<cd> O0(O, O + (long)rows * cols), P0(P, P + mm);
+  const double s_o = factors[1], s_p = factors[0];
+  const double nrm = 1.0 / (double)mm;
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t j = 0; j < n; ++j) {
+    cd* ps = C(psi) + j * mm;
+    const int rx = (int)std::round(pos[2 * j]), ry = (int)std::round(pos[2 * j + 1]);
+    std::vector<cd> f(mm), po(mm);
+    for (int i = 0; i < m; ++i)
+      for (int c = 0; c < m; ++c) {
+        po[i * m + c] = P0[i * m + c] * O0[(long)(rx + i) * cols + ry + c];
+        f[i * m + c] = 2.0 * po[i * m + c] - ps[i * m + c];
+      }
+    fft2(f.data(), m, false, p);
+    for (long q = 0; q < mm; ++q)
+      f[q] = amp[j * mm + q] * unit_phase(f[q]) * nrm;
+    fft2(f.data(), m, true, p);
+    for (long q = 0; q < mm; ++q) ps[q] = ps[q] + f[q] - 2.0 * po[q];
+  }
+  // increments, summed per object / probe pixel in frame order
+  std::vector<cd> dO((long)rows * cols, 0.0), dP(mm, 0.0);
+  for (int64_t j = 0; j < n; ++j) {
+    const cd* ps = C(psi) + j * mm;
+    const int rx = (int)std::round(pos[2 * j]), ry = (int)std::round(pos[2 * j + 1]);
+    for (int i = 0; i < m; ++i)
+      for (int c = 0; c < m; ++c) {
+        const long oi = (long)(rx + i) * cols + ry + c;
+        cd u = std::conj(P0[i * m + c]) * ps[i * m + c] * s_o;
+        if (phase_constraint) u = clamp_phase(u);
+        dO[oi] += u;
+        if (update_probe) dP[i * m + c] += std::conj(O0[oi]) * ps[i * m + c] * s_p;
+      }
+  }
+  double so = 0, sp = 0;
+  for (long q = 0; q < (long)rows * cols; ++q) {
+    O[q] += dO[q];
+    so += std::norm(O[q]);
+  }
+  for (long q = 0; q < mm; ++q) {
+    P[q] += dP[q];
+    sp += std::norm(P[q]);
+  }
+  sums[0] = so;
+  sums[1] = sp;
+  // DM::checkCache at the start of the next iteration (DM.cu:54-73)
+  const double passed[2] = {1.0 / so, 1.0 / sp};
+  for (int i = 0; i < 2; ++i)
+    if (std::fabs(factors[i] - passed[i]) > 1e-3) factors[i] = passed[i];
+}
+
+}  // extern "C"

@@ -1,0 +1,646 @@
+// C ABI of the engine (include/ptyk.h): handle, device memory, validation, dispatch.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ptyk.h"
+#include "ptyk_device.h"
+#include "ptyk_internal.h"
+
+using namespace ptyk;
+
+struct ptyk_ctx {
+  ptyk_config cfg{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  long mm = 0;  // M*M
+  // device buffers
+  float2* tw = nullptr;
+  float2* obj = nullptr;
+  float* obj_int = nullptr;
+  float2* probe = nullptr;
+  float* probe_int = nullptr;
+  float2* probe_tmp = nullptr;
+  __half* amp = nullptr;
+  float* amp_scale = nullptr;
+  float* amp_sum = nullptr;
+  float* avg = nullptr;
+  float2* pos_dev = nullptr;
+  int2* roi_dev = nullptr;
+  int* order_dev = nullptr;
+  double* scal = nullptr;
+  float* factors = nullptr;
+  // ePIE scratch
+  float2* e_psi_old = nullptr;
+  float2* e_psi = nullptr;
+  // DM state
+  float2* psi = nullptr;
+  int* row_ptr = nullptr;
+  int* row_frames = nullptr;
+  double2* probe_partial = nullptr;
+  double* obj_partial = nullptr;
+  double* probe_partial_sum = nullptr;
+  int probe_chunks = 1, frames_per_chunk = 1;
+  hipEvent_t ev[8] = {};
+  // host state
+  std::vector<float2> pos;
+  std::vector<int2> roi;
+  bool have_frames = false, have_pos = false, have_probe = false;
+  bool dm_started = false, modal = false, modes_initialized = false;
+};
+
+static thread_local std::string g_create_err;
+
+static int fail(ptyk_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_create_err = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((c), e_ == hipErrorOutOfMemory ? PTYK_ENOMEM : PTYK_EDEVICE,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+// Zero-initialised device allocation (Cuda2DArray::allocateDeviceMemory zero-fills,
+// Cuda2DArray.hpp:154-160).  The engine stream is non-blocking, so the fill must have
+// completed before any later launch on that stream may read the buffer.
+template <class T>
+static hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e == hipSuccess) e = hipMemset(*p, 0, count * sizeof(T));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return e;
+}
+
+static void free_all(ptyk_ctx* c) {
+  void* ptrs[] = {c->tw, c->obj, c->obj_int, c->probe, c->probe_int, c->probe_tmp, c->amp, c->amp_scale,
+                  c->amp_sum, c->avg, c->pos_dev, c->roi_dev, c->order_dev, c->scal, c->factors, c->e_psi_old,
+                  c->e_psi, c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial,
+                  c->probe_partial_sum};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+int ptyk_create(const ptyk_config* cfg, ptyk_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, PTYK_EINVAL, "null argument");
+  *out = nullptr;
+  if (!frame_size_supported(cfg->frame_size))
+    return fail(nullptr, PTYK_EINVAL, "frame_size must be one of 16, 32, 64, 128, 256, 512");
+  if (cfg->n_frames <= 0 || cfg->n_modes <= 0 || cfg->n_modes > 16 || cfg->obj_rows < cfg->frame_size ||
+      cfg->obj_cols < cfg->frame_size)
+    return fail(nullptr, PTYK_EINVAL, "bad n_frames / n_modes / object shape");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, PTYK_EDEVICE, "no HIP device");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, PTYK_EINVAL, "bad device ordinal");
+  if (hipSetDevice(cfg->device) != hipSuccess) return fail(nullptr, PTYK_EDEVICE, "hipSetDevice failed");
+
+  ptyk_ctx* c = new ptyk_ctx();
+  c->cfg = *cfg;
+  const long n = cfg->n_frames, m = cfg->frame_size, k = cfg->n_modes;
+  c->mm = m * m;
+  const long canvas = (long)cfg->obj_rows * cfg->obj_cols;
+  auto bail = [&](hipError_t e, const char* what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    free_all(c);
+    delete c;
+    return fail(nullptr, e == hipErrorOutOfMemory ? PTYK_ENOMEM : PTYK_EDEVICE, msg);
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "stream");
+  if ((e = dalloc(&c->tw, PTYK_TWN)) != hipSuccess) return bail(e, "twiddles");
+  {
+    std::vector<float2> tw(PTYK_TWN);
+    for (int i = 0; i < PTYK_TWN; ++i) {
+      const double a = -2.0 * M_PI * (double)i / PTYK_TWN;
+      tw[i] = make_float2((float)cos(a), (float)sin(a));
+    }
+    if ((e = hipMemcpy(c->tw, tw.data(), PTYK_TWN * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(e, "twiddle upload");
+  }
+  if ((e = dalloc(&c->obj, canvas)) != hipSuccess) return bail(e, "object");
+  if ((e = dalloc(&c->obj_int, canvas)) != hipSuccess) return bail(e, "object intensities");
+  if ((e = dalloc(&c->probe, k * c->mm)) != hipSuccess) return bail(e, "probe");
+  if ((e = dalloc(&c->probe_int, k * c->mm)) != hipSuccess) return bail(e, "probe intensities");
+  if ((e = dalloc(&c->probe_tmp, k * c->mm)) != hipSuccess) return bail(e, "probe scratch");
+  if ((e = dalloc(&c->amp, n * c->mm)) != hipSuccess) return bail(e, "frames");
+  if ((e = dalloc(&c->amp_scale, n)) != hipSuccess) return bail(e, "frame scales");
+  if ((e = dalloc(&c->amp_sum, n)) != hipSuccess) return bail(e, "frame sums");
+  if ((e = dalloc(&c->avg, c->mm)) != hipSuccess) return bail(e, "mean frame");
+  if ((e = dalloc(&c->pos_dev, n)) != hipSuccess) return bail(e, "positions");
+  if ((e = dalloc(&c->roi_dev, n)) != hipSuccess) return bail(e, "roi");
+  if ((e = dalloc(&c->order_dev, n)) != hipSuccess) return bail(e, "order");
+  if ((e = dalloc(&c->scal, PTYK_SCAL_COUNT)) != hipSuccess) return bail(e, "scalars");
+  if ((e = dalloc(&c->factors, 2)) != hipSuccess) return bail(e, "factors");
+  if ((e = dalloc(&c->e_psi_old, k * c->mm)) != hipSuccess) return bail(e, "epie scratch");
+  if ((e = dalloc(&c->e_psi, k * c->mm)) != hipSuccess) return bail(e, "epie scratch");
+  for (auto& ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "events");
+  // Probe::m_maxIntensity starts at 1 (Probe.cpp:54-58); objMax at 0.
+  {
+    double s0[PTYK_SCAL_COUNT] = {0.0, 1.0};
+    if ((e = hipMemcpy(c->scal, s0, sizeof(s0), hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "scalars");
+  }
+  *out = c;
+  return PTYK_OK;
+}
+
+void ptyk_destroy(ptyk_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->cfg.device);
+  hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+}
+
+const char* ptyk_last_error(const ptyk_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int ptyk_synchronize(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- frames
+int ptyk_set_frames_f32(ptyk_ctx* c, const float* a) {
+  if (!c || !a) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  const long n = c->cfg.n_frames, total = n * c->mm;
+  for (long i = 0; i < total; ++i)
+    if (!std::isfinite(a[i]) || a[i] < 0.f) return fail(c, PTYK_EINVAL, "amplitudes must be finite and >= 0");
+  float* tmp = nullptr;
+  HIPCHK(c, hipMalloc((void**)&tmp, total * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(tmp, a, total * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = quantize_frames(tmp, c->amp, c->amp_scale, n, c->cfg.frame_size, c->stream);
+  if (e == hipSuccess) e = frame_sums(c->amp, c->amp_scale, c->amp_sum, n, c->cfg.frame_size, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  HIPCHK(c, e);
+  c->have_frames = true;
+  return PTYK_OK;
+}
+
+int ptyk_set_frames_f16(ptyk_ctx* c, const uint16_t* h, const float* sc) {
+  if (!c || !h || !sc) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  const long n = c->cfg.n_frames;
+  for (long i = 0; i < n; ++i)
+    if (!(sc[i] > 0.f) || !std::isfinite(sc[i])) return fail(c, PTYK_EINVAL, "scales must be finite and > 0");
+  HIPCHK(c, hipMemcpyAsync(c->amp, h, n * c->mm * sizeof(__half), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->amp_scale, sc, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, frame_sums(c->amp, c->amp_scale, c->amp_sum, n, c->cfg.frame_size, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_frames = true;
+  return PTYK_OK;
+}
+
+int ptyk_get_frames_f32(ptyk_ctx* c, float* out) {
+  if (!c || !out) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->have_frames) return fail(c, PTYK_ESTATE, "frames not set");
+  hipSetDevice(c->cfg.device);
+  const long n = c->cfg.n_frames, total = n * c->mm;
+  float* tmp = nullptr;
+  HIPCHK(c, hipMalloc((void**)&tmp, total * sizeof(float)));
+  hipError_t e = dequantize_frames(c->amp, c->amp_scale, tmp, n, c->cfg.frame_size, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  HIPCHK(c, e);
+  return PTYK_OK;
+}
+
+int ptyk_get_frame_scales(ptyk_ctx* c, float* out) {
+  if (!c || !out) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  HIPCHK(c, hipMemcpyAsync(out, c->amp_scale, c->cfg.n_frames * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- positions
+static void free_dm(ptyk_ctx* c) {
+  void* ptrs[] = {c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial, c->probe_partial_sum};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  c->psi = nullptr;
+  c->row_ptr = c->row_frames = nullptr;
+  c->probe_partial = nullptr;
+  c->obj_partial = c->probe_partial_sum = nullptr;
+  c->dm_started = false;
+}
+
+int ptyk_set_positions(ptyk_ctx* c, const float* p) {
+  if (!c || !p) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  const int n = c->cfg.n_frames, m = c->cfg.frame_size;
+  std::vector<float2> pos(n);
+  std::vector<int2> roi(n);
+  for (int i = 0; i < n; ++i) {
+    const float x = p[2 * i], y = p[2 * i + 1];
+    if (!std::isfinite(x) || !std::isfinite(y)) return fail(c, PTYK_EINVAL, "non-finite position");
+    // ePIE.cpp:100-105 rejects negative positions; ROIs must also fit in the canvas.
+    if (x < 0.f || y < 0.f)
+      return fail(c, PTYK_ERANGE, "position " + std::to_string(i) + " is negative: object array too small");
+    const int rx = (int)roundf(x), ry = (int)roundf(y);
+    if (rx + m > c->cfg.obj_rows || ry + m > c->cfg.obj_cols)
+      return fail(c, PTYK_ERANGE, "ROI of position " + std::to_string(i) + " exceeds the object canvas");
+    pos[i] = make_float2(x, y);
+    roi[i] = make_int2(rx, ry);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->pos_dev, pos.data(), n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->roi_dev, roi.data(), n * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->pos = pos;
+  c->roi = roi;
+  c->have_pos = true;
+  // DM's frame lists depend on the positions; rebuild them on the next DM call.
+  if (c->row_ptr) {
+    hipFree(c->row_ptr);
+    hipFree(c->row_frames);
+    c->row_ptr = c->row_frames = nullptr;
+  }
+  return PTYK_OK;
+}
+
+int ptyk_get_roi(ptyk_ctx* c, int32_t* out) {
+  if (!c || !out) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->have_pos) return fail(c, PTYK_ESTATE, "positions not set");
+  for (size_t i = 0; i < c->roi.size(); ++i) {
+    out[2 * i] = c->roi[i].x;
+    out[2 * i + 1] = c->roi[i].y;
+  }
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- arrays
+static int copy_in(ptyk_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || !src) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+static int copy_out(ptyk_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || !dst) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+int ptyk_set_object(ptyk_ctx* c, const float* o) {
+  if (!c) return PTYK_EINVAL;
+  const long canvas = (long)c->cfg.obj_rows * c->cfg.obj_cols;
+  int r = copy_in(c, c->obj, o, canvas * sizeof(float2));
+  if (r) return r;
+  HIPCHK(c, intensities(c->obj, c->obj_int, canvas, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+int ptyk_get_object(ptyk_ctx* c, float* o) {
+  if (!c) return PTYK_EINVAL;
+  return copy_out(c, o, c->obj, (size_t)c->cfg.obj_rows * c->cfg.obj_cols * sizeof(float2));
+}
+int ptyk_clear_object(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  hipSetDevice(c->cfg.device);
+  const long canvas = (long)c->cfg.obj_rows * c->cfg.obj_cols;
+  HIPCHK(c, hipMemsetAsync(c->obj, 0, canvas * sizeof(float2), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->obj_int, 0, canvas * sizeof(float), c->stream));
+  double z = 0.0;
+  HIPCHK(c, hipMemcpyAsync(c->scal + PTYK_SCAL_OBJ_MAX, &z, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+int ptyk_set_probe(ptyk_ctx* c, const float* p) {
+  if (!c) return PTYK_EINVAL;
+  const long cnt = (long)c->cfg.n_modes * c->mm;
+  int r = copy_in(c, c->probe, p, cnt * sizeof(float2));
+  if (r) return r;
+  HIPCHK(c, intensities(c->probe, c->probe_int, cnt, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_probe = true;
+  return PTYK_OK;
+}
+int ptyk_get_probe(ptyk_ctx* c, float* p) {
+  if (!c) return PTYK_EINVAL;
+  return copy_out(c, p, c->probe, (size_t)c->cfg.n_modes * c->mm * sizeof(float2));
+}
+
+static int ensure_psi(ptyk_ctx* c) {
+  if (c->psi) return PTYK_OK;
+  hipError_t e = dalloc(&c->psi, (size_t)c->cfg.n_frames * c->mm);
+  if (e != hipSuccess) return fail(c, PTYK_ENOMEM, std::string("psi: ") + hipGetErrorString(e));
+  return PTYK_OK;
+}
+int ptyk_set_psi(ptyk_ctx* c, const float* p) {
+  if (!c) return PTYK_EINVAL;
+  hipSetDevice(c->cfg.device);
+  int r = ensure_psi(c);
+  if (r) return r;
+  return copy_in(c, c->psi, p, (size_t)c->cfg.n_frames * c->mm * sizeof(float2));
+}
+int ptyk_get_psi(ptyk_ctx* c, float* p) {
+  if (!c) return PTYK_EINVAL;
+  if (!c->psi) return fail(c, PTYK_ESTATE, "no DM state yet");
+  return copy_out(c, p, c->psi, (size_t)c->cfg.n_frames * c->mm * sizeof(float2));
+}
+
+int ptyk_get_scalars(ptyk_ctx* c, double* out) {
+  if (!c || !out) return fail(c, PTYK_EINVAL, "null argument");
+  int r = copy_out(c, out, c->scal, PTYK_SCAL_COUNT * sizeof(double));
+  if (r) return r;
+  float f[2];
+  r = copy_out(c, f, c->factors, sizeof(f));
+  if (r) return r;
+  out[PTYK_SCAL_DM_FACTOR_OBJ] = f[0];
+  out[PTYK_SCAL_DM_FACTOR_PROBE] = f[1];
+  return PTYK_OK;
+}
+int ptyk_set_scalars(ptyk_ctx* c, const double* in) {
+  if (!c || !in) return fail(c, PTYK_EINVAL, "null argument");
+  int r = copy_in(c, c->scal, in, PTYK_SCAL_COUNT * sizeof(double));
+  if (r) return r;
+  const float f[2] = {(float)in[PTYK_SCAL_DM_FACTOR_OBJ], (float)in[PTYK_SCAL_DM_FACTOR_PROBE]};
+  r = copy_in(c, c->factors, f, sizeof(f));
+  if (r) return r;
+  c->dm_started = true;
+  return PTYK_OK;
+}
+
+int ptyk_refresh_maxima(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  hipSetDevice(c->cfg.device);
+  const long canvas = (long)c->cfg.obj_rows * c->cfg.obj_cols;
+  HIPCHK(c, intensities(c->obj, c->obj_int, canvas, c->stream));
+  HIPCHK(c, max_abs2(c->obj, canvas, c->scal + PTYK_SCAL_OBJ_MAX, c->stream));
+  HIPCHK(c, intensities(c->probe, c->probe_int, (long)c->cfg.n_modes * c->mm, c->stream));
+  HIPCHK(c, max_abs2(c->probe, (long)c->cfg.n_modes * c->mm, c->scal + PTYK_SCAL_PROBE_MAX, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- probe init
+int ptyk_init_probe(ptyk_ctx* c, double beam_size, double dx_s) {
+  if (!c) return PTYK_EINVAL;
+  if (!c->have_frames) return fail(c, PTYK_ESTATE, "frames not set");
+  if (!(beam_size > 0) || !(dx_s > 0)) return fail(c, PTYK_EINVAL, "beam_size and dx_s must be > 0");
+  hipSetDevice(c->cfg.device);
+  const int m = c->cfg.frame_size;
+  const long cnt = (long)c->cfg.n_modes * c->mm;
+  // h_initProbe(..., gaussWindow = M/(2*2.35), beamSize, dx_s) with real_t (float) arguments
+  const float window = (float)((double)m / (2 * 2.35));
+  const float factor = (float)dx_s / (float)beam_size;
+  HIPCHK(c, hipMemsetAsync(c->probe, 0, cnt * sizeof(float2), c->stream));
+  HIPCHK(c, probe_sinc_gauss(c->probe, m, window, factor, c->stream));
+  HIPCHK(c, mean_amplitude(c->amp, c->amp_scale, c->avg, c->cfg.n_frames, m, c->stream));
+  HIPCHK(c, fft2_batch(c->probe, 1, m, false, c->tw, c->stream));
+  HIPCHK(c, modulus_project_single(c->probe, c->avg, m, c->stream));
+  HIPCHK(c, fft2_batch(c->probe, 1, m, true, c->tw, c->stream));
+  HIPCHK(c, probe_normalize(c->probe, c->probe_int, c->cfg.n_modes, m, c->scal + PTYK_SCAL_PROBE_MAX, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_probe = true;
+  c->modes_initialized = false;
+  return PTYK_OK;
+}
+
+int ptyk_begin_modal(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  hipSetDevice(c->cfg.device);
+  c->modal = true;
+  if (c->modes_initialized) return PTYK_OK;
+  const int k = c->cfg.n_modes;
+  if (k > 1) {
+    HIPCHK(c, modes_from_base(c->probe, c->probe_int, k, c->cfg.frame_size, 0.05f, c->stream));
+    HIPCHK(c, orthogonalize_modes(c->probe, c->probe_tmp, k, c->cfg.frame_size, c->stream));
+    c->modes_initialized = true;  // Probe::initProbeModes sets it only when K > 1
+  }
+  // initProbeModes ends with updateMaxIntensity(): the modal maximum of the stored
+  // intensities, which orthogonalize() leaves untouched (Probe.cpp:320-330).
+  const long mm = c->mm;
+  std::vector<float> pint((size_t)k * mm);
+  HIPCHK(c, hipMemcpyAsync(pint.data(), c->probe_int, pint.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float mx = 0.f;
+  for (long q = 0; q < mm; ++q) {
+    float s = 0.f;
+    for (int i = 0; i < k; ++i) s += pint[(size_t)i * mm + q];
+    mx = std::max(mx, s);
+  }
+  const double v = mx;
+  HIPCHK(c, hipMemcpyAsync(c->scal + PTYK_SCAL_PROBE_MAX, &v, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+int ptyk_end_modal(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  c->modal = false;
+  c->modes_initialized = false;
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- ePIE
+int ptyk_epie_iteration(ptyk_ctx* c, const int32_t* order, uint32_t flags, double* err) {
+  if (!c || !order) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->have_frames || !c->have_pos || !c->have_probe)
+    return fail(c, PTYK_ESTATE, "frames, positions and probe must be set");
+  hipSetDevice(c->cfg.device);
+  const int n = c->cfg.n_frames;
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (order[i] < 0 || order[i] >= n || seen[order[i]]) return fail(c, PTYK_EINVAL, "order must be a permutation");
+    seen[order[i]] = 1;
+  }
+  if (flags & PTYK_FLAG_UPDATE_MODES) {
+    int r = ptyk_begin_modal(c);
+    if (r) return r;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->order_dev, order, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  EpieArgs a{};
+  a.obj = c->obj;
+  a.obj_int = c->obj_int;
+  a.probe = c->probe;
+  a.probe_int = c->probe_int;
+  a.psi_old = c->e_psi_old;
+  a.psi = c->e_psi;
+  a.amp = c->amp;
+  a.amp_scale = c->amp_scale;
+  a.amp_sum = c->amp_sum;
+  a.pos = c->pos_dev;
+  a.order = c->order_dev;
+  a.scal = c->scal;
+  a.tw = c->tw;
+  a.n_pos = n;
+  a.m = c->cfg.frame_size;
+  a.modal = c->modal && c->cfg.n_modes > 1;
+  a.k_use = a.modal ? c->cfg.n_modes : 1;
+  a.k_total = c->cfg.n_modes;
+  a.obj_rows = c->cfg.obj_rows;
+  a.obj_cols = c->cfg.obj_cols;
+  a.phase_constraint = (flags & PTYK_FLAG_PHASE_CONSTRAINT) ? 1 : 0;
+  a.update_probe = (flags & PTYK_FLAG_UPDATE_PROBE) ? 1 : 0;
+  a.rms = (flags & PTYK_FLAG_RMS) ? 1 : 0;
+  a.tex_quantize = c->cfg.tex_quantize;
+  HIPCHK(c, epie_iteration(a, c->stream));
+  double s[PTYK_SCAL_COUNT];
+  HIPCHK(c, hipMemcpyAsync(s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (err) *err = a.rms ? s[PTYK_SCAL_ERROR] / n : 0.0;
+  return PTYK_OK;
+}
+
+// ---------------------------------------------------------------- DM
+static int dm_prepare(ptyk_ctx* c) {
+  const int n = c->cfg.n_frames, m = c->cfg.frame_size, rows = c->cfg.obj_rows;
+  int r = ensure_psi(c);
+  if (r) return r;
+  if (!c->row_ptr) {
+    // frames covering each object row, ascending frame id
+    std::vector<int> cnt(rows + 1, 0);
+    for (int j = 0; j < n; ++j)
+      for (int q = 0; q < m; ++q) cnt[c->roi[j].x + q + 1]++;
+    for (int p = 0; p < rows; ++p) cnt[p + 1] += cnt[p];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1), frames((size_t)n * m);
+    for (int j = 0; j < n; ++j)
+      for (int q = 0; q < m; ++q) frames[fill[c->roi[j].x + q]++] = j;
+    HIPCHK(c, dalloc(&c->row_ptr, rows + 1));
+    HIPCHK(c, dalloc(&c->row_frames, (size_t)n * m));
+    HIPCHK(c, hipMemcpyAsync(c->row_ptr, cnt.data(), (rows + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->row_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors die at the end of this block
+  }
+  if (!c->probe_partial) {
+    const int rpw = m >= PTYK_NT ? 1 : PTYK_NT / m;
+    const int gp = m / rpw;
+    c->probe_chunks = std::max(1, std::min(n, 2048 / gp));
+    c->frames_per_chunk = (n + c->probe_chunks - 1) / c->probe_chunks;
+    c->probe_chunks = (n + c->frames_per_chunk - 1) / c->frames_per_chunk;
+    HIPCHK(c, dalloc(&c->probe_partial, (size_t)c->probe_chunks * c->mm));
+    HIPCHK(c, dalloc(&c->obj_partial, (size_t)rows * dm_obj_col_chunks(c->cfg.obj_cols)));
+    HIPCHK(c, dalloc(&c->probe_partial_sum, (size_t)gp));
+  }
+  if (!c->dm_started) {
+    // DM.cu:88-92: cached factors {1/objMax, 1/probeMax} from the current maxima
+    double s[PTYK_SCAL_COUNT];
+    HIPCHK(c, hipMemcpyAsync(s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const float f[2] = {(float)(1.0 / (double)(float)s[0]), (float)(1.0 / (double)(float)s[1])};
+    HIPCHK(c, hipMemcpyAsync(c->factors, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // f lives on this stack frame
+    c->dm_started = true;
+  }
+  return PTYK_OK;
+}
+
+static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
+  if (!c) return PTYK_EINVAL;
+  if (!c->have_frames || !c->have_pos || !c->have_probe)
+    return fail(c, PTYK_ESTATE, "frames, positions and probe must be set");
+  hipSetDevice(c->cfg.device);
+  int r = dm_prepare(c);
+  if (r) return r;
+  DmArgs a{};
+  a.psi = c->psi;
+  a.obj = c->obj;
+  a.probe = c->probe;
+  a.amp = c->amp;
+  a.amp_scale = c->amp_scale;
+  a.roi = c->roi_dev;
+  a.row_ptr = c->row_ptr;
+  a.row_frames = c->row_frames;
+  a.probe_partial = c->probe_partial;
+  a.obj_partial = c->obj_partial;
+  a.probe_partial_sum = c->probe_partial_sum;
+  a.factors = c->factors;
+  a.scal = c->scal;
+  a.tw = c->tw;
+  a.n_frames = c->cfg.n_frames;
+  a.m = c->cfg.frame_size;
+  a.obj_rows = c->cfg.obj_rows;
+  a.obj_cols = c->cfg.obj_cols;
+  a.probe_chunks = c->probe_chunks;
+  a.frames_per_chunk = c->frames_per_chunk;
+  a.phase_constraint = (flags & PTYK_FLAG_PHASE_CONSTRAINT) ? 1 : 0;
+  a.update_probe = (flags & PTYK_FLAG_UPDATE_PROBE) ? 1 : 0;
+  HIPCHK(c, dm_iteration(a, c->stream, kernel_ms ? c->ev : nullptr));
+  if (kernel_ms) {
+    HIPCHK(c, hipEventSynchronize(c->ev[7]));
+    for (int i = 0; i < 7; ++i) HIPCHK(c, hipEventElapsedTime(&kernel_ms[i], c->ev[i], c->ev[i + 1]));
+  }
+  return PTYK_OK;
+}
+
+int ptyk_dm_iteration(ptyk_ctx* c, uint32_t flags) { return dm_run(c, flags, nullptr); }
+int ptyk_dm_iteration_timed(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
+  if (!kernel_ms) return fail(c, PTYK_EINVAL, "null argument");
+  return dm_run(c, flags, kernel_ms);
+}
+
+// ---------------------------------------------------------------- utilities
+int ptyk_fft2(ptyk_ctx* c, float* data, int64_t n, int inverse) {
+  if (!c || !data || n < 0) return fail(c, PTYK_EINVAL, "bad argument");
+  hipSetDevice(c->cfg.device);
+  const size_t bytes = (size_t)n * c->mm * sizeof(float2);
+  float2* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, bytes ? bytes : 8));
+  hipError_t e = hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = fft2_batch(d, n, c->cfg.frame_size, inverse != 0, c->tw, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(data, d, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  HIPCHK(c, e);
+  return PTYK_OK;
+}
+
+int ptyk_simulate(ptyk_ctx* c, const float* obj, int32_t rows, int32_t cols, const float* probe, const float* pos,
+                  int64_t n, int32_t squared, float* out) {
+  if (!c || !obj || !probe || !pos || !out || rows <= 0 || cols <= 0 || n < 0)
+    return fail(c, PTYK_EINVAL, "bad argument");
+  hipSetDevice(c->cfg.device);
+  const int m = c->cfg.frame_size;
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(pos[2 * i]) || !std::isfinite(pos[2 * i + 1]))
+      return fail(c, PTYK_EINVAL, "non-finite position");
+  const size_t canvas = (size_t)rows * cols;
+  float2 *d_obj = nullptr, *d_probe = nullptr, *d_pos = nullptr, *d_psi = nullptr;
+  float* d_out = nullptr;
+  hipError_t e = hipMalloc((void**)&d_obj, canvas * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_probe, c->mm * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_pos, (n ? n : 1) * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_psi, (n ? n : 1) * c->mm * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_out, (n ? n : 1) * c->mm * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpyAsync(d_obj, obj, canvas * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_probe, probe, c->mm * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(d_pos, pos, n * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && n)
+    e = simulate_exit_waves(d_obj, rows, cols, d_probe, d_pos, d_psi, n, m, c->cfg.tex_quantize, c->stream);
+  if (e == hipSuccess && n) e = fft2_batch(d_psi, n, m, false, c->tw, c->stream);
+  if (e == hipSuccess && n) e = abs_frames(d_psi, d_out, n * c->mm, squared, c->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(out, d_out, n * c->mm * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)d_obj, (void*)d_probe, (void*)d_pos, (void*)d_psi, (void*)d_out})
+    if (p) hipFree(p);
+  HIPCHK(c, e);
+  return PTYK_OK;
+}
+
+}  // extern "C"
