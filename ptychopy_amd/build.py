@@ -39,7 +39,8 @@ def _compile(src: Path) -> Path:
     if src.suffix == ".hip":
         cmd = [HIPCC, *COMMON, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
     else:  # host-only C++ (driver) compiled by hipcc's clang without device code
-        cmd = [HIPCC, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, *COMMON, "-x", "c++", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-c", str(src), "-o",
+               str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -1,0 +1,78 @@
+"""Synthetic workloads at the benchmark configurations (seeded; data formed on the GPU).
+
+dm_c1: plain DM at the reference simulation shape -- 30 x 30 raster, 512 x 512 detector
+(lambda = 0.248 nm, z = 1 m, 172 um pixels => dx_s = 2.8159 nm), 50 nm steps (17.76 px),
+1088 x 1088 random-field object, 110 nm FWHM Gaussian probe with a 2 rad defocus phase at the
+edge, 1e7 photons per frame with Poisson noise, one probe mode.
+epie_c0: plain ePIE -- 12 x 12 raster, 16 px steps with +-1 px jitter, 128 x 128 frames,
+320 x 320 object, 24 px FWHM probe, 1e6 photons per frame.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+
+import numpy as np
+
+from . import sim
+from ._driver import cartesian_mesh, sample_pixel_size
+from ._engine import Engine
+
+
+def _poisson_threaded(inten: np.ndarray, photons: float, seed_seq: np.random.SeedSequence, chunks: int = 16):
+    n = inten.shape[0]
+    bounds = np.linspace(0, n, min(chunks, n) + 1).astype(int)
+    seeds = seed_seq.spawn(len(bounds) - 1)
+    out = np.empty(inten.shape, np.float32)
+
+    def work(i):
+        a, b = bounds[i], bounds[i + 1]
+        out[a:b] = sim.poisson_amplitudes(inten[a:b], photons, np.random.default_rng(seeds[i]))
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, len(seeds))) as ex:
+        list(ex.map(work, range(len(seeds))))
+    return out
+
+
+def _simulate(obj, probe, pos_sim, m, device=0):
+    with Engine(1, m, (m, m), device=device) as e:
+        return e.simulate(obj, probe, pos_sim, squared=True)
+
+
+def dm_c1(n_side: int = 30, m: int = 512, photons: float = 1e7, sim_size: int = 1088, config_id: int = 1,
+          device: int = 0) -> dict:
+    lambd, z, dx_d = 0.248e-9, 1.0, 172e-6
+    dx_s = sample_pixel_size(lambd, z, m, dx_d)
+    step = 50e-9
+    pos, canvas, _ = cartesian_mesh(n_side, n_side, step, step, dx_s, m)
+    rng = sim.rng_for(config_id, 0)
+    obj = sim.random_field_object((sim_size, sim_size), rng, phase_std=0.5)
+    beam = 110e-9
+    probe = sim.gaussian_probe(m, fwhm=beam / dx_s, defocus_edge=2.0)
+    off = np.array([(sim_size - canvas[0]) // 2, (sim_size - canvas[1]) // 2], np.float32)
+    inten = _simulate(obj, probe, pos + off, m, device)
+    amp = _poisson_threaded(inten, photons, np.random.SeedSequence([sim.SEED_BASE + config_id, 1]))
+    o0 = (int(off[0]), int(off[1]))
+    return dict(name="dm_c1", method="DM", pos=pos, canvas=canvas, amp=amp, m=m, n=len(pos), dx_s=dx_s,
+                beam=beam, obj_true=obj[o0[0]:o0[0] + canvas[0], o0[1]:o0[1] + canvas[1]], probe_true=probe,
+                iterations=100)
+
+
+def epie_c0(m: int = 128, photons: float = 1e6, sim_size: int = 320, config_id: int = 0, device: int = 0) -> dict:
+    rng = sim.rng_for(config_id, 0)
+    pos = sim.raster_positions(12, 16.0, rng, jitter=1.0, origin=(1.0, 1.0))
+    canvas = sim.canvas_for(pos, m)
+    obj = sim.random_field_object((sim_size, sim_size), rng, phase_std=0.5)
+    probe = sim.gaussian_probe(m, fwhm=24.0, defocus_edge=2.0)
+    off = np.array([(sim_size - canvas[0]) // 2, (sim_size - canvas[1]) // 2], np.float32)
+    inten = _simulate(obj, probe, pos + off, m, device)
+    amp = _poisson_threaded(inten, photons, np.random.SeedSequence([sim.SEED_BASE + config_id, 1]))
+    o0 = (int(off[0]), int(off[1]))
+    return dict(name="epie_c0", method="ePIE", pos=pos, canvas=canvas, amp=amp, m=m, n=len(pos), dx_s=1e-8,
+                beam=24e-8, obj_true=obj[o0[0]:o0[0] + canvas[0], o0[1]:o0[1] + canvas[1]], probe_true=probe,
+                iterations=50)
+
+
+def dm_bytes_per_frame(m: int, k: int, obj_px: int, n: int) -> float:
+    """Compulsory HBM bytes per frame-iteration of the psi-resident DM form:
+    M^2 (2 [FP16 A] + 16 K [psi read + write]) + (16 obj_px + 16 K M^2) / N."""
+    return m * m * (2 + 16 * k) + (16 * obj_px + 16 * k * m * m) / n
