@@ -25,7 +25,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("dm_rows_fwd", "dm_cols", "dm_rows_inv", "dm_probe_partial", "dm_object", "dm_probe_apply", "dm_finish")
+KERNELS = ("row_pass", "column_pass", "inverse_row_pass", "probe_gather", "object_gather", "probe_apply",
+           "object_apply", "finish")
 
 
 def dm_flags(i: int) -> int:
@@ -39,17 +40,18 @@ def dm_flags(i: int) -> int:
     return f
 
 
-def kernel_bytes(name: str, n: int, m: int, obj_px: int, chunks: int) -> float:
-    """Algorithmic (compulsory) HBM bytes of one launch of each DM kernel."""
+def kernel_bytes(name: str, n: int, m: int, obj_px: int) -> float:
+    """Algorithmic (compulsory) HBM bytes of one iteration's launches of each DM kernel kind."""
     px = n * m * m
     return {
-        "dm_rows_fwd": 16.0 * px + 8.0 * m * m + 8.0 * obj_px,        # psi in, X out; P, O once
-        "dm_cols": 18.0 * px,                                          # X in, FP16 A in, Z out
-        "dm_rows_inv": 16.0 * px,                                      # Z in, psi out
-        "dm_probe_partial": 8.0 * px + 8.0 * obj_px + 16.0 * chunks * m * m,
-        "dm_object": 8.0 * px + 16.0 * obj_px + 8.0 * m * m,
-        "dm_probe_apply": 16.0 * chunks * m * m + 16.0 * m * m,
-        "dm_finish": 8.0,
+        "row_pass": 16.0 * px + 8.0 * m * m + 8.0 * obj_px,     # psi in, X out; P and O once
+        "column_pass": 18.0 * px,                                # X in, FP16 A in, Z out
+        "inverse_row_pass": 16.0 * px,                           # Z in, psi out
+        "probe_gather": 8.0 * px + 8.0 * obj_px,                 # psi in; O once
+        "object_gather": 8.0 * px + 8.0 * m * m,                 # psi in; P once
+        "probe_apply": 32.0 * m * m + 8.0 * m * m,
+        "object_apply": 24.0 * obj_px,
+        "finish": 8.0,
     }[name]
 
 
@@ -92,6 +94,8 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
+    ap.add_argument("--dm-wave", type=int, default=64,
+                    help="frames per DM wave (0 automatic, -1 one pass over all frames)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -110,6 +114,9 @@ def main() -> int:
     n, m = wl["n"], wl["m"]
     rows, cols = wl["canvas"]
     e = Engine(n, m, (rows, cols))
+    if args.dm_wave is not None:
+        from ptychopy_amd._engine import TUNE_DM_WAVE
+        e.set_tuning(TUNE_DM_WAVE, args.dm_wave)
     e.set_frames(wl["amp"])
     e.set_positions(wl["pos"])
     e.clear_object()
@@ -139,14 +146,9 @@ def main() -> int:
         kt += e.dm_iteration_timed(dm_flags(it))
     kt /= args.kernel_iters
     obj_px = rows * cols
-    # probe partial chunk count mirrors dm_prepare: max(1, min(N, 2048 / (M / rows_per_wg)))
-    gp = m // (1 if m >= 256 else 256 // m)
-    chunks = max(1, min(n, 2048 // gp))
-    fpc = -(-n // chunks)
-    chunks = -(-n // fpc)
     dom = int(np.argmax(kt))
     name = KERNELS[dom]
-    ach = kernel_bytes(name, n, m, obj_px, chunks) / (kt[dom] * 1e-3) / 1e9
+    ach = kernel_bytes(name, n, m, obj_px) / (kt[dom] * 1e-3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "dm_c1_traffic.json"
     if tfile.exists():
@@ -174,6 +176,7 @@ def main() -> int:
         "iteration_roofline": {"bytes_per_frame": bpf, "achieved": it_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": it_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: float(v) for k, v in zip(KERNELS, kt)},
+        "dm_wave_frames": e.get_tuning(1),
         "setup_s": t_gen,
     }
     if not args.no_cpu:

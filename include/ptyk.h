@@ -123,9 +123,18 @@ int ptyk_epie_iteration(ptyk_ctx* ctx, const int32_t* order, uint32_t flags, dou
 /* One DM iteration (DM::iteration, DM.cu:80-128).  The first call takes the cached
  * normalisation factors from the current objMax/probeMax (DM.cu:88-92). */
 int ptyk_dm_iteration(ptyk_ctx* ctx, uint32_t flags);
-/* Same, timing each of the 7 kernel launches with HIP events on the engine stream:
- * kernel_ms[7] = {rows_fwd, cols, rows_inv, probe_partial, object, probe_apply, finish}. */
+/* Same, timing the launches with HIP events on the engine stream, summed per kernel kind:
+ * kernel_ms[8] = {row pass, column pass, inverse row pass, probe gather, object gather,
+ *                 probe apply, object apply, finish}. */
 int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
+
+/* Tuning knobs (no reference counterpart; chosen by the autotuner).
+ * PTYK_TUNE_DM_WAVE: frames per DM wave for M in {128, 256, 512}; 0 = automatic (about
+ * 64 MiB of exit waves per wave), -1 = one pass over all frames.  get returns the value in
+ * effect (0 when the single-pass form runs). */
+enum { PTYK_TUNE_DM_WAVE = 1 };
+int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
+int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);
 
 /* Batched in-place 2D FFT of n frames of M x M complex64 (host buffers), forward
  * exp(-2 pi i kn/M), inverse unnormalised (CudaFFTPlan::getC2CPlan, CudaFFTPlan.cpp:63-101). */

@@ -34,8 +34,12 @@ EXPORTS = (
     "ptyk_get_object", "ptyk_clear_object", "ptyk_set_probe", "ptyk_get_probe", "ptyk_set_psi", "ptyk_get_psi",
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
-    "ptyk_simulate", "ptyk_synchronize",
+    "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning",
 )
+
+TUNE_DM_WAVE = 1
+DM_KERNELS = ("row_pass", "column_pass", "inverse_row_pass", "probe_gather", "object_gather", "probe_apply",
+              "object_apply", "finish")
 
 
 class PtykError(RuntimeError):
@@ -81,6 +85,8 @@ def load():
     lib.ptyk_dm_iteration_timed.argtypes = [P, u32, P]
     lib.ptyk_fft2.argtypes = [P, P, i64, C.c_int]
     lib.ptyk_simulate.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
+    lib.ptyk_set_tuning.argtypes = [P, i32, i32]
+    lib.ptyk_get_tuning.argtypes = [P, i32, C.POINTER(C.c_int32)]
     _lib = lib
     return lib
 
@@ -224,8 +230,16 @@ class Engine:
     def dm_iteration(self, flags: int = 0):
         self._chk(self.lib.ptyk_dm_iteration(self.h, int(flags)))
 
+    def set_tuning(self, key: int, value: int):
+        self._chk(self.lib.ptyk_set_tuning(self.h, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        v = C.c_int32(0)
+        self._chk(self.lib.ptyk_get_tuning(self.h, int(key), C.byref(v)))
+        return v.value
+
     def dm_iteration_timed(self, flags: int = 0) -> np.ndarray:
-        ms = np.zeros(7, np.float32)
+        ms = np.zeros(len(DM_KERNELS), np.float32)
         self._chk(self.lib.ptyk_dm_iteration_timed(self.h, int(flags), _ptr(ms)))
         return ms
 

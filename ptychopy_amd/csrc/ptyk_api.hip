@@ -46,7 +46,16 @@ struct ptyk_ctx {
   double* obj_partial = nullptr;
   double* probe_partial_sum = nullptr;
   int probe_chunks = 1, frames_per_chunk = 1;
+  // wave-scheduled DM
+  int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
+  int dm_wave = 0, dm_waves = 0;
+  double2* d_obj = nullptr;
+  double2* d_probe = nullptr;
+  int* wave_row_ptr = nullptr;
+  int* wave_row_frames = nullptr;
+  std::vector<int> wave_lo, wave_hi;
   hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> evpool;
   // host state
   std::vector<float2> pos;
   std::vector<int2> roi;
@@ -89,11 +98,12 @@ static void free_all(ptyk_ctx* c) {
   void* ptrs[] = {c->tw, c->obj, c->obj_int, c->probe, c->probe_int, c->probe_tmp, c->amp, c->amp_scale,
                   c->amp_sum, c->avg, c->pos_dev, c->roi_dev, c->order_dev, c->scal, c->factors, c->e_psi_old,
                   c->e_psi, c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial,
-                  c->probe_partial_sum};
+                  c->probe_partial_sum, c->d_obj, c->d_probe, c->wave_row_ptr, c->wave_row_frames};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
+  for (auto& e : c->evpool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -236,13 +246,17 @@ int ptyk_get_frame_scales(ptyk_ctx* c, float* out) {
 
 // ---------------------------------------------------------------- positions
 static void free_dm(ptyk_ctx* c) {
-  void* ptrs[] = {c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial, c->probe_partial_sum};
+  void* ptrs[] = {c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial, c->probe_partial_sum,
+                  c->d_obj, c->d_probe, c->wave_row_ptr, c->wave_row_frames};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->psi = nullptr;
   c->row_ptr = c->row_frames = nullptr;
   c->probe_partial = nullptr;
   c->obj_partial = c->probe_partial_sum = nullptr;
+  c->d_obj = c->d_probe = nullptr;
+  c->wave_row_ptr = c->wave_row_frames = nullptr;
+  c->dm_wave = c->dm_waves = 0;
   c->dm_started = false;
 }
 
@@ -275,6 +289,12 @@ int ptyk_set_positions(ptyk_ctx* c, const float* p) {
     hipFree(c->row_ptr);
     hipFree(c->row_frames);
     c->row_ptr = c->row_frames = nullptr;
+  }
+  if (c->wave_row_ptr) {
+    hipFree(c->wave_row_ptr);
+    hipFree(c->wave_row_frames);
+    c->wave_row_ptr = c->wave_row_frames = nullptr;
+    c->dm_wave = c->dm_waves = 0;
   }
   return PTYK_OK;
 }
@@ -508,11 +528,21 @@ int ptyk_epie_iteration(ptyk_ctx* c, const int32_t* order, uint32_t flags, doubl
 }
 
 // ---------------------------------------------------------------- DM
+static int dm_wave_frames(const ptyk_ctx* c) {
+  const int n = c->cfg.n_frames, m = c->cfg.frame_size;
+  if (c->dm_wave_cfg < 0 || !dm_wave_supported(m)) return 0;  // one pass over all frames
+  if (c->dm_wave_cfg > 0) return std::min(c->dm_wave_cfg, n);
+  // a wave's in-place intermediates (W x M x M complex64) kept near 64 MiB
+  const long w = (64L << 20) / ((long)m * m * 8);
+  return (int)std::max(1L, std::min((long)n, w));
+}
+
 static int dm_prepare(ptyk_ctx* c) {
   const int n = c->cfg.n_frames, m = c->cfg.frame_size, rows = c->cfg.obj_rows;
   int r = ensure_psi(c);
   if (r) return r;
-  if (!c->row_ptr) {
+  const int wave = dm_wave_frames(c);
+  if (wave == 0 && !c->row_ptr) {
     // frames covering each object row, ascending frame id
     std::vector<int> cnt(rows + 1, 0);
     for (int j = 0; j < n; ++j)
@@ -528,6 +558,48 @@ static int dm_prepare(ptyk_ctx* c) {
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors die at the end of this block
   }
+  if (wave > 0 && c->dm_wave != wave) {
+    // per wave: frames covering each object row (offsets into one frame-id array)
+    const int nw = (n + wave - 1) / wave;
+    std::vector<int> ptr((size_t)nw * (rows + 1)), frames((size_t)n * m);
+    std::vector<int> lo(nw), hi(nw);
+    int off = 0;
+    for (int w = 0; w < nw; ++w) {
+      const int f0 = w * wave, f1 = std::min(n, f0 + wave);
+      std::vector<int> cnt(rows + 1, 0);
+      int rlo = rows, rhi = 0;
+      for (int j = f0; j < f1; ++j) {
+        rlo = std::min(rlo, c->roi[j].x);
+        rhi = std::max(rhi, c->roi[j].x + m);
+        for (int q = 0; q < m; ++q) cnt[c->roi[j].x + q + 1]++;
+      }
+      for (int p = 0; p < rows; ++p) cnt[p + 1] += cnt[p];
+      int* wp = ptr.data() + (size_t)w * (rows + 1);
+      for (int p = 0; p <= rows; ++p) wp[p] = off + cnt[p];
+      std::vector<int> fill(rows);
+      for (int p = 0; p < rows; ++p) fill[p] = wp[p];
+      for (int j = f0; j < f1; ++j)
+        for (int q = 0; q < m; ++q) frames[fill[c->roi[j].x + q]++] = j;
+      off += cnt[rows];
+      lo[w] = rlo;
+      hi[w] = rhi;
+    }
+    if (c->wave_row_ptr) hipFree(c->wave_row_ptr);
+    if (c->wave_row_frames) hipFree(c->wave_row_frames);
+    c->wave_row_ptr = c->wave_row_frames = nullptr;
+    HIPCHK(c, dalloc(&c->wave_row_ptr, ptr.size()));
+    HIPCHK(c, dalloc(&c->wave_row_frames, frames.size()));
+    HIPCHK(c, hipMemcpyAsync(c->wave_row_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->wave_row_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors die at the end of this block
+    c->wave_lo = lo;
+    c->wave_hi = hi;
+    c->dm_wave = wave;
+    c->dm_waves = nw;
+    if (!c->d_obj) HIPCHK(c, dalloc(&c->d_obj, (size_t)rows * c->cfg.obj_cols));
+    if (!c->d_probe) HIPCHK(c, dalloc(&c->d_probe, (size_t)c->mm));
+  }
   if (!c->probe_partial) {
     const int rpw = m >= PTYK_NT ? 1 : PTYK_NT / m;
     const int gp = m / rpw;
@@ -536,7 +608,7 @@ static int dm_prepare(ptyk_ctx* c) {
     c->probe_chunks = (n + c->frames_per_chunk - 1) / c->frames_per_chunk;
     HIPCHK(c, dalloc(&c->probe_partial, (size_t)c->probe_chunks * c->mm));
     HIPCHK(c, dalloc(&c->obj_partial, (size_t)rows * dm_obj_col_chunks(c->cfg.obj_cols)));
-    HIPCHK(c, dalloc(&c->probe_partial_sum, (size_t)gp));
+    HIPCHK(c, dalloc(&c->probe_partial_sum, (size_t)std::max(gp, (int)((c->mm + PTYK_NT - 1) / PTYK_NT))));
   }
   if (!c->dm_started) {
     // DM.cu:88-92: cached factors {1/objMax, 1/probeMax} from the current maxima
@@ -558,33 +630,79 @@ static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   hipSetDevice(c->cfg.device);
   int r = dm_prepare(c);
   if (r) return r;
-  DmArgs a{};
-  a.psi = c->psi;
-  a.obj = c->obj;
-  a.probe = c->probe;
-  a.amp = c->amp;
-  a.amp_scale = c->amp_scale;
-  a.roi = c->roi_dev;
-  a.row_ptr = c->row_ptr;
-  a.row_frames = c->row_frames;
-  a.probe_partial = c->probe_partial;
-  a.obj_partial = c->obj_partial;
-  a.probe_partial_sum = c->probe_partial_sum;
-  a.factors = c->factors;
-  a.scal = c->scal;
-  a.tw = c->tw;
-  a.n_frames = c->cfg.n_frames;
-  a.m = c->cfg.frame_size;
-  a.obj_rows = c->cfg.obj_rows;
-  a.obj_cols = c->cfg.obj_cols;
-  a.probe_chunks = c->probe_chunks;
-  a.frames_per_chunk = c->frames_per_chunk;
-  a.phase_constraint = (flags & PTYK_FLAG_PHASE_CONSTRAINT) ? 1 : 0;
-  a.update_probe = (flags & PTYK_FLAG_UPDATE_PROBE) ? 1 : 0;
-  HIPCHK(c, dm_iteration(a, c->stream, kernel_ms ? c->ev : nullptr));
+  DmTimer tm;
   if (kernel_ms) {
-    HIPCHK(c, hipEventSynchronize(c->ev[7]));
-    for (int i = 0; i < 7; ++i) HIPCHK(c, hipEventElapsedTime(&kernel_ms[i], c->ev[i], c->ev[i + 1]));
+    if (c->evpool.empty()) {
+      c->evpool.resize(1024);
+      for (auto& e : c->evpool) HIPCHK(c, hipEventCreate(&e));
+    }
+    tm.ev = c->evpool.data();
+    tm.cap = (int)c->evpool.size();
+  }
+  const int pc = (flags & PTYK_FLAG_PHASE_CONSTRAINT) ? 1 : 0;
+  const int up = (flags & PTYK_FLAG_UPDATE_PROBE) ? 1 : 0;
+  if (dm_wave_frames(c) > 0) {
+    DmWaveArgs a{};
+    a.psi = c->psi;
+    a.obj = c->obj;
+    a.probe = c->probe;
+    a.amp = c->amp;
+    a.amp_scale = c->amp_scale;
+    a.roi = c->roi_dev;
+    a.wave_row_ptr = c->wave_row_ptr;
+    a.wave_row_frames = c->wave_row_frames;
+    a.wave_row_lo = c->wave_lo.data();
+    a.wave_row_hi = c->wave_hi.data();
+    a.d_obj = c->d_obj;
+    a.d_probe = c->d_probe;
+    a.obj_partial = c->obj_partial;
+    a.probe_partial_sum = c->probe_partial_sum;
+    a.factors = c->factors;
+    a.scal = c->scal;
+    a.tw = c->tw;
+    a.n_frames = c->cfg.n_frames;
+    a.m = c->cfg.frame_size;
+    a.obj_rows = c->cfg.obj_rows;
+    a.obj_cols = c->cfg.obj_cols;
+    a.wave = c->dm_wave;
+    a.n_waves = c->dm_waves;
+    a.phase_constraint = pc;
+    a.update_probe = up;
+    HIPCHK(c, dm_wave_iteration(a, c->stream, kernel_ms ? &tm : nullptr));
+  } else {
+    DmArgs a{};
+    a.psi = c->psi;
+    a.obj = c->obj;
+    a.probe = c->probe;
+    a.amp = c->amp;
+    a.amp_scale = c->amp_scale;
+    a.roi = c->roi_dev;
+    a.row_ptr = c->row_ptr;
+    a.row_frames = c->row_frames;
+    a.probe_partial = c->probe_partial;
+    a.obj_partial = c->obj_partial;
+    a.probe_partial_sum = c->probe_partial_sum;
+    a.factors = c->factors;
+    a.scal = c->scal;
+    a.tw = c->tw;
+    a.n_frames = c->cfg.n_frames;
+    a.m = c->cfg.frame_size;
+    a.obj_rows = c->cfg.obj_rows;
+    a.obj_cols = c->cfg.obj_cols;
+    a.probe_chunks = c->probe_chunks;
+    a.frames_per_chunk = c->frames_per_chunk;
+    a.phase_constraint = pc;
+    a.update_probe = up;
+    HIPCHK(c, dm_iteration(a, c->stream, kernel_ms ? &tm : nullptr));
+  }
+  if (kernel_ms) {
+    for (int i = 0; i < DM_T_COUNT; ++i) kernel_ms[i] = 0.f;
+    if (tm.n > 0) HIPCHK(c, hipEventSynchronize(tm.ev[tm.n - 1]));
+    for (int i = 0; i + 1 < tm.n; ++i) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
+      if (tm.cat[i] >= 0 && tm.cat[i] < DM_T_COUNT) kernel_ms[tm.cat[i]] += ms;
+    }
   }
   return PTYK_OK;
 }
@@ -593,6 +711,25 @@ int ptyk_dm_iteration(ptyk_ctx* c, uint32_t flags) { return dm_run(c, flags, nul
 int ptyk_dm_iteration_timed(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   if (!kernel_ms) return fail(c, PTYK_EINVAL, "null argument");
   return dm_run(c, flags, kernel_ms);
+}
+
+int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
+  if (!c) return PTYK_EINVAL;
+  if (key == PTYK_TUNE_DM_WAVE) {
+    if (value < -1) return fail(c, PTYK_EINVAL, "dm wave must be -1, 0 or a frame count");
+    c->dm_wave_cfg = value;
+    return PTYK_OK;
+  }
+  return fail(c, PTYK_EINVAL, "unknown tuning key");
+}
+
+int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
+  if (!c || !value) return PTYK_EINVAL;
+  if (key == PTYK_TUNE_DM_WAVE) {
+    *value = dm_wave_frames(c);
+    return PTYK_OK;
+  }
+  return fail(c, PTYK_EINVAL, "unknown tuning key");
 }
 
 // ---------------------------------------------------------------- utilities

@@ -283,6 +283,28 @@ __device__ __forceinline__ float block_max(float v, float* scratch /* NT/64 floa
   return s;
 }
 
+// DM iteration epilogue: FP64 totals of the per-block partial sums in a fixed order, then the
+// reference's cached-factor rule (DM::checkCache, DM.cu:54-73): a cached factor is replaced
+// only when it moved by more than 1e-3.  factors = {1/objMax, 1/probeMax}.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dm_finish(const double* __restrict__ obj_partial, int n_obj,
+                                                  const double* __restrict__ probe_partial, int n_probe,
+                                                  double* __restrict__ scal, float* __restrict__ factors) {
+  __shared__ double red[NT / 64];
+  double so = 0.0, sp = 0.0;
+  for (int i = threadIdx.x; i < n_obj; i += NT) so += obj_partial[i];
+  for (int i = threadIdx.x; i < n_probe; i += NT) sp += probe_partial[i];
+  so = block_sum<NT>(so, red);
+  sp = block_sum<NT>(sp, red);
+  if (threadIdx.x == 0) {
+    scal[0] = so;
+    scal[1] = sp;
+    const float passed0 = (float)(1.0 / so), passed1 = (float)(1.0 / sp);
+    if (fabsf(factors[0] - passed0) > 1e-3f) factors[0] = passed0;
+    if (fabsf(factors[1] - passed1) > 1e-3f) factors[1] = passed1;
+  }
+}
+
 // FP16 amplitude with a per-frame FP32 scale.
 __device__ __forceinline__ float load_amp(const __half* a, long idx, float scale) {
   return __half2float(a[idx]) * scale;

@@ -252,64 +252,45 @@ __global__ __launch_bounds__(PTYK_NT) void k_dm_object(float2* __restrict__ obj,
   if (threadIdx.x == 0) block_sums[(long)blockIdx.x * gridDim.y + blockIdx.y] = s2;
 }
 
-// FP64 totals in a fixed order, then the reference's cached-factor rule (DM.cu:46-73):
-// a factor is replaced only when it moved by more than 1e-3.
-__global__ __launch_bounds__(PTYK_NT) void k_dm_finish(const double* __restrict__ obj_partial, int n_obj,
-                                                       const double* __restrict__ probe_partial, int n_probe,
-                                                       double* __restrict__ scal, float* __restrict__ factors) {
-  __shared__ double red[PTYK_NT / 64];
-  double so = 0.0, sp = 0.0;
-  for (int i = threadIdx.x; i < n_obj; i += PTYK_NT) so += obj_partial[i];
-  for (int i = threadIdx.x; i < n_probe; i += PTYK_NT) sp += probe_partial[i];
-  so = block_sum<PTYK_NT>(so, red);
-  sp = block_sum<PTYK_NT>(sp, red);
-  if (threadIdx.x == 0) {
-    scal[0] = so;
-    scal[1] = sp;
-    const float passed0 = (float)(1.0 / so), passed1 = (float)(1.0 / sp);
-    if (fabsf(factors[0] - passed0) > 1e-3f) factors[0] = passed0;
-    if (fabsf(factors[1] - passed1) > 1e-3f) factors[1] = passed1;
-  }
-}
-
 int dm_obj_col_chunks(int obj_cols) { return (obj_cols + PTYK_OBJ_CHUNK - 1) / PTYK_OBJ_CHUNK; }
 
 template <int M>
-static hipError_t dm_impl(const DmArgs& a, hipStream_t s, hipEvent_t* ev) {
+static hipError_t dm_impl(const DmArgs& a, hipStream_t s, DmTimer* tm) {
   const long rows = a.n_frames * M;
   const long strips = a.n_frames * (M / 16);
   const unsigned gr = (unsigned)((rows + DmCfg<M>::RNB - 1) / DmCfg<M>::RNB);
   const unsigned gc = (unsigned)((strips + DmCfg<M>::CNB / 16 - 1) / (DmCfg<M>::CNB / 16));
   const unsigned gp = (unsigned)(M / DmCfg<M>::PRPW);
-  auto rec = [&](int i) {
-    if (ev) hipEventRecord(ev[i], s);
+  auto rec = [&](int cat) {
+    if (tm) tm->mark(s, cat);
   };
-  rec(0);
+  rec(DM_T_ROWS_FWD);
   hipLaunchKernelGGL((k_dm_rows_fwd<M>), dim3(gr), dim3(PTYK_NT), 0, s, a.psi, a.obj, a.obj_cols, a.probe, a.roi,
                      rows, a.tw);
-  rec(1);
+  rec(DM_T_COLS);
   hipLaunchKernelGGL((k_dm_cols<M>), dim3(gc), dim3(PTYK_NT), 0, s, a.psi, a.amp, a.amp_scale, strips, a.tw);
-  rec(2);
+  rec(DM_T_ROWS_INV);
   hipLaunchKernelGGL((k_dm_rows_inv<M>), dim3(gr), dim3(PTYK_NT), 0, s, a.psi, rows, a.tw);
-  rec(3);
-  if (a.update_probe)
+  if (a.update_probe) {
+    rec(DM_T_PROBE_GATHER);
     hipLaunchKernelGGL((k_dm_probe_partial<M>), dim3(gp, a.probe_chunks), dim3(PTYK_NT), 0, s, a.psi, a.obj,
                        a.obj_cols, a.roi, a.n_frames, a.frames_per_chunk, a.probe_partial);
-  rec(4);
+  }
   const int cch = dm_obj_col_chunks(a.obj_cols);
+  rec(DM_T_OBJ_GATHER);
   hipLaunchKernelGGL((k_dm_object<M>), dim3(a.obj_rows, cch), dim3(PTYK_NT), 0, s, a.obj, a.obj_cols, a.psi,
                      a.probe, a.roi, a.row_ptr, a.row_frames, a.factors, a.phase_constraint, a.obj_partial);
-  rec(5);
+  rec(DM_T_PROBE_APPLY);
   hipLaunchKernelGGL((k_dm_probe_apply<M>), dim3(gp), dim3(PTYK_NT), 0, s, a.probe, a.probe_partial,
                      a.probe_chunks, a.factors, a.update_probe, a.probe_partial_sum);
-  rec(6);
-  hipLaunchKernelGGL(k_dm_finish, dim3(1), dim3(PTYK_NT), 0, s, a.obj_partial, a.obj_rows * cch,
+  rec(DM_T_FINISH);
+  hipLaunchKernelGGL((k_dm_finish<PTYK_NT>), dim3(1), dim3(PTYK_NT), 0, s, a.obj_partial, a.obj_rows * cch,
                      a.probe_partial_sum, (int)gp, a.scal, a.factors);
-  rec(7);
+  rec(-1);
   return hipGetLastError();
 }
 
-hipError_t dm_iteration(const DmArgs& a, hipStream_t s, hipEvent_t* ev) {
+hipError_t dm_iteration(const DmArgs& a, hipStream_t s, DmTimer* ev) {
   switch (a.m) {
     case 16: return dm_impl<16>(a, s, ev);
     case 32: return dm_impl<32>(a, s, ev);

@@ -1,0 +1,330 @@
+// DM iteration, wave-scheduled form for M in {128, 256, 512} (same equations as ptyk_dm.hip;
+// replaces DM.cu:80-128 and the cuFFT callbacks of SampleKernels.cu:418-488).
+//
+// Column transforms are split as M = R0 * 64 (R0 = M/64):
+//   pass A (row groups): X = RowFFT(2 P O_r - psi) for the R0 rows {n1 + 64 n2}, then the
+//          radix-R0 step across those rows with twiddle W_M^(n1 k2); result row n1 + 64 k2.
+//   pass B (64 x 64 blocks): the remaining 64-point column transform over rows 64 k2 .. 64 k2 + 63
+//          (frequency row R0 k1 + k2), the modulus difference (A/|Phi| - 1) Phi / M^2, and the
+//          inverse 64-point transform -- contiguous 512 B row segments, 32 KB of LDS.
+//   pass C (row groups): inverse twiddle and radix-R0 step, inverse row FFTs: psi_new = Pf - x.
+// Frames are processed in waves small enough that a wave's in-place intermediates stay in the
+// on-package cache between passes.  Object and probe increments are accumulated per wave into
+// FP64 buffers in frame order (deterministic), and applied once after the last wave, so every
+// pass reads the object and probe as they were at the start of the iteration.
+#include "ptyk_device.h"
+#include "ptyk_internal.h"
+
+namespace ptyk {
+
+template <int M>
+struct WaveCfg {
+  static constexpr int L = 64;
+  static constexpr int R0 = M / 64;
+  static constexpr int RNB = plan_min_batch<M, PTYK_NT>();
+  static constexpr int GPW = RNB / R0 > 0 ? RNB / R0 : 1;  // row groups per workgroup
+  static constexpr int ROWS = GPW * R0;                     // rows held per workgroup
+  static constexpr int CPB = M / 64;                        // 64-column chunks per row
+  static constexpr int BPF = R0 * CPB;                      // 64 x 64 blocks per frame
+};
+
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi, const float2* __restrict__ obj,
+                                                        int obj_cols, const float2* __restrict__ probe,
+                                                        const int2* __restrict__ roi, long f0, long n_groups,
+                                                        const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  __shared__ float2 lds[C::ROWS * M];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
+    const int lr = e / M, c = e % M;
+    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
+    float2 v = make_float2(0.f, 0.f);
+    if (G < n_groups) {
+      const long f = f0 + G / C::L;
+      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
+      const int2 o = roi[f];
+      const float2 po = cmul(probe[r * M + c], obj[(long)(o.x + r) * obj_cols + (o.y + c)]);
+      const float2 s = psi[(f * M + r) * M + c];
+      v = make_float2(2.f * po.x - s.x, 2.f * po.y - s.y);
+    }
+    lds[e] = v;
+  }
+  __syncthreads();
+  fft_lds<M, false, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
+  for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
+    const int g = t / M, c = t % M;
+    const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
+    float2 v[C::R0];
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) v[q] = lds[(g * C::R0 + q) * M + c];
+    Dft<C::R0, false>::run(v);
+#pragma unroll
+    for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]);
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) lds[(g * C::R0 + q) * M + c] = v[q];
+  }
+  __syncthreads();
+  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
+    const int lr = e / M, c = e % M;
+    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
+    if (G < n_groups) {
+      const long f = f0 + G / C::L;
+      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
+      psi[(f * M + r) * M + c] = lds[e];
+    }
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_b(float2* __restrict__ psi, const __half* __restrict__ amp,
+                                                        const float* __restrict__ amp_scale, long f0,
+                                                        const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  __shared__ float2 lds[64 * 64];
+  const int tid = threadIdx.x;
+  const long f = f0 + blockIdx.x / C::BPF;
+  const int rem = blockIdx.x % C::BPF;
+  const int k2 = rem / C::CPB, c0 = (rem % C::CPB) * 64;
+  float2* base = psi + (f * M + 64 * k2) * M + c0;
+  for (int e = tid; e < 64 * 64; e += PTYK_NT) lds[e] = base[(e >> 6) * M + (e & 63)];
+  __syncthreads();
+  fft_lds<64, false, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
+  const float inv_m2 = 1.f / (float)(M * M);
+  const float sc = amp_scale[f];
+  const __half* a = amp + f * M * M + c0;
+  for (int e = tid; e < 64 * 64; e += PTYK_NT) {
+    const int k1 = e >> 6, c = e & 63;
+    const float A = __half2float(a[(long)(C::R0 * k1 + k2) * M + c]) * sc;
+    const float2 ph = lds[e];
+    const float m2 = cabs2(ph);
+    float2 d;
+    if (m2 > 0.f) {
+      const float k = (A * rsqrtf(m2) - 1.f) * inv_m2;
+      d = make_float2(ph.x * k, ph.y * k);
+    } else {
+      d = make_float2(A * inv_m2, 0.f);
+    }
+    lds[e] = d;
+  }
+  __syncthreads();
+  fft_lds<64, true, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
+  for (int e = tid; e < 64 * 64; e += PTYK_NT) base[(e >> 6) * M + (e & 63)] = lds[e];
+}
+
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi, long f0, long n_groups,
+                                                        const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  __shared__ float2 lds[C::ROWS * M];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
+    const int lr = e / M, c = e % M;
+    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
+    float2 v = make_float2(0.f, 0.f);
+    if (G < n_groups) {
+      const long f = f0 + G / C::L;
+      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
+      v = psi[(f * M + r) * M + c];
+    }
+    lds[e] = v;
+  }
+  __syncthreads();
+  for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
+    const int g = t / M, c = t % M;
+    const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
+    float2 v[C::R0];
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) v[q] = lds[(g * C::R0 + q) * M + c];
+#pragma unroll
+    for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], cconj(tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]));
+    Dft<C::R0, true>::run(v);
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) lds[(g * C::R0 + q) * M + c] = v[q];
+  }
+  __syncthreads();
+  fft_lds<M, true, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
+  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
+    const int lr = e / M, c = e % M;
+    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
+    if (G < n_groups) {
+      const long f = f0 + G / C::L;
+      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
+      psi[(f * M + r) * M + c] = lds[e];
+    }
+  }
+}
+
+#define PTYK_OBJ_CHUNK 256
+
+// dO[p][c] += sum over this wave's frames covering object row p (ascending frame id) of the
+// (optionally phase-clamped) increments s_O conj(P) psi_new.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_obj_accum(double2* __restrict__ d_obj, int obj_cols, int row_lo,
+                                                           const float2* __restrict__ psi,
+                                                           const float2* __restrict__ probe,
+                                                           const int2* __restrict__ roi,
+                                                           const int* __restrict__ row_ptr,
+                                                           const int* __restrict__ row_frames,
+                                                           const float* __restrict__ factors, int phase_constraint) {
+  const int p = row_lo + blockIdx.x;
+  const int cbeg = blockIdx.y * PTYK_OBJ_CHUNK, cend = cbeg + PTYK_OBJ_CHUNK;
+  const int c = cbeg + threadIdx.x;
+  const float so = factors[1];
+  double ax = 0.0, ay = 0.0;
+  const int k0 = row_ptr[p], k1 = row_ptr[p + 1];
+  for (int k = k0; k < k1; ++k) {
+    const int j = row_frames[k];
+    const int2 r = roi[j];
+    if (r.y >= cend || r.y + M <= cbeg) continue;
+    const int q = p - r.x, cc = c - r.y;
+    if (c < obj_cols && cc >= 0 && cc < M) {
+      float2 u = cscale(cmulcj(probe[q * M + cc], psi[((long)j * M + q) * M + cc]), so);
+      if (phase_constraint) u = clamp_phase(u);
+      ax += u.x;
+      ay += u.y;
+    }
+  }
+  if (c < obj_cols && k1 > k0) {
+    double2 d = d_obj[(long)p * obj_cols + c];
+    d.x += ax;
+    d.y += ay;
+    d_obj[(long)p * obj_cols + c] = d;
+  }
+}
+
+// dP[q][c] += sum over frames [f0, f1) of conj(O[r_j + (q, c)]) psi_new_j[q][c].
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_probe_accum(double2* __restrict__ d_probe,
+                                                             const float2* __restrict__ psi,
+                                                             const float2* __restrict__ obj, int obj_cols,
+                                                             const int2* __restrict__ roi, long f0, long f1) {
+  constexpr int CPT = M >= PTYK_NT ? M / PTYK_NT : 1;
+  constexpr int RPW = M >= PTYK_NT ? 1 : PTYK_NT / M;
+  const int q = blockIdx.x * RPW + (RPW > 1 ? threadIdx.x / M : 0);
+  const int c0 = RPW > 1 ? threadIdx.x % M : threadIdx.x;
+  double ax[CPT], ay[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) ax[k] = ay[k] = 0.0;
+  for (long j = f0; j < f1; ++j) {
+    const int2 r = roi[j];
+    const float2* prow = psi + (j * M + q) * M;
+    const float2* orow = obj + (long)(r.x + q) * obj_cols + r.y;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int c = c0 + k * PTYK_NT;
+      const float2 t = cmulcj(orow[c], prow[c]);
+      ax[k] += t.x;
+      ay[k] += t.y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int c = c0 + k * PTYK_NT;
+    double2 d = d_probe[q * M + c];
+    d.x += ax[k];
+    d.y += ay[k];
+    d_probe[q * M + c] = d;
+  }
+}
+
+// O += dO (one rounding), dO <- 0, per-block sum |O|^2.
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_obj_apply(float2* __restrict__ obj, double2* __restrict__ d_obj,
+                                                           int obj_cols, double* __restrict__ block_sums) {
+  __shared__ double red[PTYK_NT / 64];
+  const int p = blockIdx.x, c = blockIdx.y * PTYK_OBJ_CHUNK + threadIdx.x;
+  double s2 = 0.0;
+  if (c < obj_cols) {
+    const long i = (long)p * obj_cols + c;
+    const double2 d = d_obj[i];
+    float2 o = obj[i];
+    o.x = (float)((double)o.x + d.x);
+    o.y = (float)((double)o.y + d.y);
+    obj[i] = o;
+    d_obj[i] = make_double2(0.0, 0.0);
+    s2 = (double)o.x * o.x + (double)o.y * o.y;
+  }
+  s2 = block_sum<PTYK_NT>(s2, red);
+  if (threadIdx.x == 0) block_sums[(long)blockIdx.x * gridDim.y + blockIdx.y] = s2;
+}
+
+// P += s_P dP when updating, dP <- 0, per-block sum |P|^2.
+__global__ __launch_bounds__(PTYK_NT) void k_dmw_probe_apply(float2* __restrict__ probe, double2* __restrict__ d_probe,
+                                                             long mm, const float* __restrict__ factors, int update,
+                                                             double* __restrict__ block_sums) {
+  __shared__ double red[PTYK_NT / 64];
+  const long i = (long)blockIdx.x * PTYK_NT + threadIdx.x;
+  double s2 = 0.0;
+  if (i < mm) {
+    float2 p = probe[i];
+    if (update) {
+      const double2 d = d_probe[i];
+      const double sp = (double)factors[0];
+      p.x = (float)((double)p.x + sp * d.x);
+      p.y = (float)((double)p.y + sp * d.y);
+      probe[i] = p;
+      d_probe[i] = make_double2(0.0, 0.0);
+    }
+    s2 = (double)p.x * p.x + (double)p.y * p.y;
+  }
+  s2 = block_sum<PTYK_NT>(s2, red);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = s2;
+}
+
+template <int M>
+static hipError_t wave_impl(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
+  using C = WaveCfg<M>;
+  const int cch = (a.obj_cols + PTYK_OBJ_CHUNK - 1) / PTYK_OBJ_CHUNK;
+  constexpr int RPW = M >= PTYK_NT ? 1 : PTYK_NT / M;
+  for (int w = 0; w < a.n_waves; ++w) {
+    const long f0 = (long)w * a.wave, f1 = f0 + a.wave < a.n_frames ? f0 + a.wave : a.n_frames;
+    const long groups = (f1 - f0) * C::L;
+    const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
+    const unsigned gb = (unsigned)((f1 - f0) * C::BPF);
+    if (tm) tm->mark(s, DM_T_ROWS_FWD);
+    hipLaunchKernelGGL((k_dmw_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, a.psi, a.obj, a.obj_cols, a.probe, a.roi, f0,
+                       groups, a.tw);
+    if (tm) tm->mark(s, DM_T_COLS);
+    hipLaunchKernelGGL((k_dmw_pass_b<M>), dim3(gb), dim3(PTYK_NT), 0, s, a.psi, a.amp, a.amp_scale, f0, a.tw);
+    if (tm) tm->mark(s, DM_T_ROWS_INV);
+    hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, a.psi, f0, groups, a.tw);
+    if (a.update_probe) {
+      if (tm) tm->mark(s, DM_T_PROBE_GATHER);
+      hipLaunchKernelGGL((k_dmw_probe_accum<M>), dim3(M / RPW), dim3(PTYK_NT), 0, s, a.d_probe, a.psi, a.obj,
+                         a.obj_cols, a.roi, f0, f1);
+    }
+    const int lo = a.wave_row_lo[w], hi = a.wave_row_hi[w];
+    if (hi > lo) {
+      if (tm) tm->mark(s, DM_T_OBJ_GATHER);
+      hipLaunchKernelGGL((k_dmw_obj_accum<M>), dim3(hi - lo, cch), dim3(PTYK_NT), 0, s, a.d_obj, a.obj_cols, lo,
+                         a.psi, a.probe, a.roi, a.wave_row_ptr + (long)w * (a.obj_rows + 1), a.wave_row_frames,
+                         a.factors, a.phase_constraint);
+    }
+  }
+  if (tm) tm->mark(s, DM_T_PROBE_APPLY);
+  const unsigned gpa = (unsigned)((M * M + PTYK_NT - 1) / PTYK_NT);
+  hipLaunchKernelGGL(k_dmw_probe_apply, dim3(gpa), dim3(PTYK_NT), 0, s, a.probe, a.d_probe, (long)M * M, a.factors,
+                     a.update_probe, a.probe_partial_sum);
+  if (tm) tm->mark(s, DM_T_OBJ_APPLY);
+  hipLaunchKernelGGL(k_dmw_obj_apply, dim3(a.obj_rows, cch), dim3(PTYK_NT), 0, s, a.obj, a.d_obj, a.obj_cols,
+                     a.obj_partial);
+  if (tm) tm->mark(s, DM_T_FINISH);
+  hipLaunchKernelGGL((k_dm_finish<PTYK_NT>), dim3(1), dim3(PTYK_NT), 0, s, a.obj_partial, a.obj_rows * cch,
+                     a.probe_partial_sum, (int)gpa, a.scal, a.factors);
+  if (tm) tm->mark(s, -1);
+  return hipGetLastError();
+}
+
+bool dm_wave_supported(int m) { return m == 128 || m == 256 || m == 512; }
+int dm_wave_probe_blocks(int m) { return (m * m + PTYK_NT - 1) / PTYK_NT; }
+
+hipError_t dm_wave_iteration(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
+  switch (a.m) {
+    case 128: return wave_impl<128>(a, s, tm);
+    case 256: return wave_impl<256>(a, s, tm);
+    case 512: return wave_impl<512>(a, s, tm);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace ptyk

@@ -35,8 +35,51 @@ struct DmArgs {
   int probe_chunks, frames_per_chunk;
   int phase_constraint, update_probe;
 };
-hipError_t dm_iteration(const DmArgs& a, hipStream_t s, hipEvent_t* ev /* nullable: 8 events for per-kernel timing */);
+// HIP-event marks on the engine stream; the time from mark i to mark i+1 is charged to
+// the category given at mark i (category -1 closes the sequence).
+enum { DM_T_ROWS_FWD, DM_T_COLS, DM_T_ROWS_INV, DM_T_PROBE_GATHER, DM_T_OBJ_GATHER, DM_T_PROBE_APPLY,
+       DM_T_OBJ_APPLY, DM_T_FINISH, DM_T_COUNT };
+struct DmTimer {
+  hipEvent_t* ev = nullptr;
+  int cap = 0, n = 0;
+  int cat[1024];
+  void mark(hipStream_t s, int category) {
+    if (n < cap && n < 1024) {
+      hipEventRecord(ev[n], s);
+      cat[n++] = category;
+    }
+  }
+};
+
+hipError_t dm_iteration(const DmArgs& a, hipStream_t s, DmTimer* tm /* nullable */);
 int dm_obj_col_chunks(int obj_cols);
+
+// Wave-scheduled DM (ptyk_dm_wave.hip), M in {128, 256, 512}.
+struct DmWaveArgs {
+  float2* psi;
+  float2* obj;
+  float2* probe;
+  const __half* amp;
+  const float* amp_scale;
+  const int2* roi;
+  const int* wave_row_ptr;    // device: n_waves x (obj_rows + 1) offsets into wave_row_frames
+  const int* wave_row_frames; // device: frame ids, ascending per (wave, object row)
+  const int* wave_row_lo;     // host: first object row touched by each wave
+  const int* wave_row_hi;     // host: one past the last
+  double2* d_obj;             // FP64 object increment accumulator (canvas)
+  double2* d_probe;           // FP64 probe increment accumulator (M x M)
+  double* obj_partial;
+  double* probe_partial_sum;
+  float* factors;
+  double* scal;
+  const float2* tw;
+  long n_frames;
+  int m, obj_rows, obj_cols, wave, n_waves;
+  int phase_constraint, update_probe;
+};
+bool dm_wave_supported(int m);
+int dm_wave_probe_blocks(int m);
+hipError_t dm_wave_iteration(const DmWaveArgs& a, hipStream_t s, DmTimer* tm /* nullable */);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {

@@ -38,12 +38,22 @@ def _flags(i, phase_constraint=1, update_probe=10):
     return f
 
 
-def _check_step(e, orc, p, amp, flags, psi0):
+def _factors_for_next_call(e, first_call):
+    """Cached DM factors the next iteration uses: on the first call, {1/objMax, 1/probeMax} from
+    the current maxima (DM.cu:88-92, objMax = 0 after clearing gives inf); later the cache."""
+    from ptychopy_amd import _engine as E
+    sc = e.get_scalars()
+    if first_call:
+        om = np.float32(sc[E.SCAL_OBJ_MAX])
+        return np.array([np.inf if om == 0 else 1 / om, 1 / np.float32(sc[E.SCAL_PROBE_MAX])])
+    return np.array([sc[E.SCAL_DM_FACTOR_OBJ], sc[E.SCAL_DM_FACTOR_PROBE]])
+
+
+def _check_step(e, orc, p, amp, flags, psi0, first_call=False):
     from ptychopy_amd import _engine as E
     o0 = e.get_object()
     p0 = e.get_probe()[0]
-    sc = e.get_scalars()
-    fac = np.array([sc[E.SCAL_DM_FACTOR_OBJ], sc[E.SCAL_DM_FACTOR_PROBE]])
+    fac = _factors_for_next_call(e, first_call)
     e.dm_iteration(flags)
     o1, p1, s1 = e.get_object(), e.get_probe()[0], e.get_psi()
     sc1 = e.get_scalars()
@@ -121,3 +131,60 @@ def test_dm_many_iterations_stay_close_to_oracle(gpu, orc):
                                                   phase_constraint=i <= 1, update_probe=i >= 10)
         assert rel_err(e.get_object(), o) < 1e-4
         assert rel_err(e.get_probe()[0], pr) < 1e-4
+
+
+@pytest.mark.parametrize("m,n_side,wave", [(128, 4, 5), (256, 3, 2), (512, 2, 3), (128, 3, 0)])
+def test_dm_wave_schedule_matches_oracle(gpu, orc, m, n_side, wave):
+    """Wave-scheduled form (several waves per iteration) against the oracle."""
+    from ptychopy_amd import _engine as E
+    p = make_problem(m=m, n_side=n_side, seed=10 + m + wave)
+    with _engine_for(p) as e:
+        e.set_tuning(E.TUNE_DM_WAVE, wave)
+        amp = e.get_frames().astype(np.float64)
+        psi = np.zeros((p["n"], m, m))
+        for i in range(1, 4):
+            psi, errs = _check_step(e, orc, p, amp, _flags(i), psi, first_call=(i == 1))
+        assert e.get_tuning(E.TUNE_DM_WAVE) == (wave or e.get_tuning(E.TUNE_DM_WAVE))
+        for i in range(4, 10):
+            e.dm_iteration(_flags(i))
+        psi = e.get_psi()
+        for i in (10, 11):
+            psi, errs = _check_step(e, orc, p, amp, _flags(i), psi)
+
+
+def test_dm_wave_and_single_pass_agree(gpu):
+    from ptychopy_amd import _engine as E
+    p = make_problem(m=128, n_side=4, seed=21)
+    res = []
+    for wave in (-1, 3, 16):
+        with _engine_for(p) as e:
+            e.set_tuning(E.TUNE_DM_WAVE, wave)
+            for i in range(1, 13):
+                e.dm_iteration(_flags(i))
+            res.append((e.get_object(), e.get_probe()))
+    for o, pr in res[1:]:
+        assert rel_err(o, res[0][0]) < 1e-5
+        assert rel_err(pr, res[0][1]) < 1e-5
+
+
+def test_dm_wave_is_bitwise_deterministic(gpu):
+    from ptychopy_amd import _engine as E
+    p = make_problem(m=256, n_side=3, seed=22)
+    outs = []
+    for _ in range(2):
+        with _engine_for(p) as e:
+            e.set_tuning(E.TUNE_DM_WAVE, 4)
+            for i in range(1, 12):
+                e.dm_iteration(_flags(i))
+            outs.append((e.get_object(), e.get_probe(), e.get_psi()))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_dm_timed_iteration_reports_every_pass(gpu):
+    from ptychopy_amd import _engine as E
+    p = make_problem(m=128, n_side=4, seed=23)
+    with _engine_for(p) as e:
+        ms = e.dm_iteration_timed(_flags(12))
+    assert ms.shape == (len(E.DM_KERNELS),)
+    assert np.all(ms[[0, 1, 2, 3, 4]] > 0)
