@@ -32,6 +32,59 @@ typedef struct {
  * canvas_out[2] = ceil(max - min + M), extent_out[4] = {min_x, min_y, max_x, max_y} (nullable). */
 int ptycho_mesh_cartesian(const ptycho_mesh_params* p, float* pos_out, int32_t* canvas_out, float* extent_out);
 
+/* Position list given in metres (ListScanMesh, ListScanMesh.cpp:55-103): n x (x, y). */
+int ptycho_mesh_list(const ptycho_mesh_params* p, const double* metres, int64_t n, float* pos_out,
+                     int32_t* canvas_out, float* extent_out);
+/* Spiral of nx*ny points (SpiralScanMesh::generateMesh, SpiralScanMesh.cpp:53-78). */
+int ptycho_mesh_spiral(const ptycho_mesh_params* p, float* pos_out, int32_t* canvas_out, float* extent_out);
+/* Complex CSV ("a+bj", ", "-separated rows) or raw binary, as Cuda2DArray::save / load
+ * (Cuda2DArray.cpp:46-108, Cuda2DArray.hpp:439-464). */
+int ptycho_save_complex_csv(const char* path, const float* c64, int32_t rows, int32_t cols, int32_t binary);
+int ptycho_load_complex_csv(const char* path, float* out, int32_t max_elems, int32_t* rows, int32_t* cols);
+
+/* ---------------------------------------------------------------- IPhaser
+ * One reconstruction driver (IPhaser, IPhaser.h:135-184) with its own parameter set
+ * (CXParams, Parameters.h:168-203).  All functions return 0 on success, -1 on failure
+ * (message via ptycho_phaser_last_error); nothing exits the process. */
+typedef struct ptycho_phaser ptycho_phaser;
+
+int ptycho_phaser_create(ptycho_phaser** out);
+void ptycho_phaser_destroy(ptycho_phaser* h);
+const char* ptycho_phaser_last_error(const ptycho_phaser* h);
+
+/* Parameters::parseFromCPython, one keyword at a time (Parameters.cpp:380-653; keyword names
+ * of ptychopy.c:242-246 plus "algorithm", "RMS" and "seed"); value as text. */
+int ptycho_phaser_set_param(ptycho_phaser* h, const char* key, const char* value);
+/* Parameters::parseFromCommandLine on a "-key=value ..." string (Parameters.cpp:66-378). */
+int ptycho_phaser_parse_cmdline(ptycho_phaser* h, const char* cmdline);
+int ptycho_phaser_get_param(ptycho_phaser* h, const char* key, char* buf, int32_t len);
+
+/* numpy inputs of the *np entry points (ptychopy.c:346-441): frames n x rows x cols float32,
+ * positions n x 2 metres, object / probe guesses complex64. */
+int ptycho_phaser_set_diffraction(ptycho_phaser* h, const float* data, int64_t n, int32_t rows, int32_t cols);
+int ptycho_phaser_set_positions(ptycho_phaser* h, const double* xy, int64_t n);
+int ptycho_phaser_set_object_guess(ptycho_phaser* h, const float* c64, int32_t rows, int32_t cols);
+int ptycho_phaser_set_probe_guess(ptycho_phaser* h, const float* c64, int32_t rows, int32_t cols);
+
+int ptycho_phaser_init(ptycho_phaser* h);                                       /* IPhaser::init */
+int ptycho_phaser_add_method(ptycho_phaser* h, const char* name, uint32_t iterations); /* "ePIE" | "DM" | "MLs" */
+int ptycho_phaser_phase(ptycho_phaser* h);                                      /* IPhaser::phase */
+int ptycho_phaser_phaseinit(ptycho_phaser* h);                                  /* IPhaser::phaseinit */
+int ptycho_phaser_phasestep(ptycho_phaser* h, uint32_t i);                      /* IPhaser::phasestepvis */
+int ptycho_phaser_phasepost(ptycho_phaser* h);                                  /* IPhaser::phasepost */
+int ptycho_phaser_write_results(ptycho_phaser* h, int32_t r);                   /* writeResultsToDisk */
+
+int ptycho_phaser_object_shape(ptycho_phaser* h, int32_t* rows, int32_t* cols);
+int ptycho_phaser_get_object(ptycho_phaser* h, float* out);                     /* complex64 rows x cols */
+int ptycho_phaser_probe_shape(ptycho_phaser* h, int32_t* modes, int32_t* size);
+int ptycho_phaser_get_probe(ptycho_phaser* h, float* out);                      /* complex64 modes*size x size */
+int64_t ptycho_phaser_num_positions(ptycho_phaser* h);
+int ptycho_phaser_get_positions(ptycho_phaser* h, float* out, int64_t n);        /* pixels, canvas frame */
+int ptycho_phaser_get_errors(ptycho_phaser* h, double* out, int32_t n);          /* IPhaser::getPhaseErrors */
+double ptycho_phaser_phase_seconds(ptycho_phaser* h);
+/* The engine handle (ptyk.h) the driver runs on; valid after ptycho_phaser_init. */
+void* ptycho_phaser_engine(ptycho_phaser* h);
+
 #ifdef __cplusplus
 }
 #endif

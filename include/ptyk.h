@@ -130,7 +130,7 @@ int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
 
 /* Tuning knobs (no reference counterpart; chosen by the autotuner).
  * PTYK_TUNE_DM_WAVE: frames per DM wave for M in {128, 256, 512}; 0 = automatic (about
- * 64 MiB of exit waves per wave), -1 = one pass over all frames.  get returns the value in
+ * 128 MiB of exit waves per wave), -1 = one pass over all frames.  get returns the value in
  * effect (0 when the single-pass form runs). */
 enum { PTYK_TUNE_DM_WAVE = 1 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);

@@ -15,7 +15,17 @@ class MeshParams(C.Structure):
                 ("flip", C.c_int32), ("probe_size", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64)]
 
 
-DRIVER_EXPORTS = ("ptycho_sample_pixel_size", "ptycho_mesh_cartesian")
+DRIVER_EXPORTS = (
+    "ptycho_sample_pixel_size", "ptycho_mesh_cartesian", "ptycho_mesh_list", "ptycho_mesh_spiral",
+    "ptycho_save_complex_csv", "ptycho_load_complex_csv", "ptycho_phaser_create", "ptycho_phaser_destroy",
+    "ptycho_phaser_last_error", "ptycho_phaser_set_param", "ptycho_phaser_parse_cmdline", "ptycho_phaser_get_param",
+    "ptycho_phaser_set_diffraction", "ptycho_phaser_set_positions", "ptycho_phaser_set_object_guess",
+    "ptycho_phaser_set_probe_guess", "ptycho_phaser_init", "ptycho_phaser_add_method", "ptycho_phaser_phase",
+    "ptycho_phaser_phaseinit", "ptycho_phaser_phasestep", "ptycho_phaser_phasepost", "ptycho_phaser_write_results",
+    "ptycho_phaser_object_shape", "ptycho_phaser_get_object", "ptycho_phaser_probe_shape", "ptycho_phaser_get_probe",
+    "ptycho_phaser_num_positions", "ptycho_phaser_get_positions", "ptycho_phaser_get_errors",
+    "ptycho_phaser_phase_seconds", "ptycho_phaser_engine",
+)
 
 _bound = False
 
@@ -27,6 +37,12 @@ def lib():
         L.ptycho_sample_pixel_size.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_double]
         L.ptycho_sample_pixel_size.restype = C.c_double
         L.ptycho_mesh_cartesian.argtypes = [C.POINTER(MeshParams), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ptycho_mesh_list.argtypes = [C.POINTER(MeshParams), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]
+        L.ptycho_mesh_spiral.argtypes = [C.POINTER(MeshParams), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ptycho_save_complex_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.ptycho_load_complex_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int32)]
         _bound = True
     return L
 
@@ -49,3 +65,47 @@ def cartesian_mesh(nx: int, ny: int, step_x: float, step_y: float, dx_s: float, 
     if rc != 0:
         raise ValueError("bad mesh parameters")
     return pos, (int(canvas[0]), int(canvas[1])), ext
+
+
+def _params(nx, ny, step_x, step_y, dx_s, probe_size, jitter_radius=0, seed=0, mirror_x=False, mirror_y=False,
+            flip=False, drift=(0.0, 0.0)):
+    return MeshParams(nx, ny, step_x, step_y, dx_s, drift[0], drift[1], jitter_radius, int(mirror_x), int(mirror_y),
+                      int(flip), probe_size, 0, seed)
+
+
+def list_mesh(metres, dx_s: float, probe_size: int, *, jitter_radius=0, seed=0, flip=False, drift=(0.0, 0.0)):
+    m = np.ascontiguousarray(metres, dtype=np.float64)
+    p = _params(m.shape[0], 1, 0.0, 0.0, dx_s, probe_size, jitter_radius, seed, flip=flip, drift=drift)
+    pos = np.empty((m.shape[0], 2), np.float32)
+    canvas = np.empty(2, np.int32)
+    ext = np.empty(4, np.float32)
+    if lib().ptycho_mesh_list(C.byref(p), m.ctypes.data, m.shape[0], pos.ctypes.data, canvas.ctypes.data,
+                              ext.ctypes.data) != 0:
+        raise ValueError("bad list mesh parameters")
+    return pos, (int(canvas[0]), int(canvas[1])), ext
+
+
+def spiral_mesh(nx: int, ny: int, step_x: float, step_y: float, dx_s: float, probe_size: int, *, jitter_radius=0,
+                seed=0, flip=False, drift=(0.0, 0.0)):
+    p = _params(nx, ny, step_x, step_y, dx_s, probe_size, jitter_radius, seed, flip=flip, drift=drift)
+    pos = np.empty((nx * ny, 2), np.float32)
+    canvas = np.empty(2, np.int32)
+    ext = np.empty(4, np.float32)
+    if lib().ptycho_mesh_spiral(C.byref(p), pos.ctypes.data, canvas.ctypes.data, ext.ctypes.data) != 0:
+        raise ValueError("bad spiral mesh parameters")
+    return pos, (int(canvas[0]), int(canvas[1])), ext
+
+
+def save_complex(path: str, a, binary: bool = False) -> bool:
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    return lib().ptycho_save_complex_csv(str(path).encode(), a.ctypes.data, a.shape[0], a.shape[1], int(binary)) == 0
+
+
+def load_complex_csv(path: str) -> np.ndarray:
+    r, c = C.c_int32(), C.c_int32()
+    if lib().ptycho_load_complex_csv(str(path).encode(), None, 0, C.byref(r), C.byref(c)) != 0:
+        raise IOError(path)
+    out = np.empty((r.value, c.value), np.complex64)
+    if lib().ptycho_load_complex_csv(str(path).encode(), out.ctypes.data, out.size, C.byref(r), C.byref(c)) != 0:
+        raise IOError(path)
+    return out

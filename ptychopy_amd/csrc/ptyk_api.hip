@@ -532,8 +532,9 @@ static int dm_wave_frames(const ptyk_ctx* c) {
   const int n = c->cfg.n_frames, m = c->cfg.frame_size;
   if (c->dm_wave_cfg < 0 || !dm_wave_supported(m)) return 0;  // one pass over all frames
   if (c->dm_wave_cfg > 0) return std::min(c->dm_wave_cfg, n);
-  // a wave's in-place intermediates (W x M x M complex64) kept near 64 MiB
-  const long w = (64L << 20) / ((long)m * m * 8);
+  // a wave's in-place intermediates (W x M x M complex64) kept near 128 MiB (measured best of
+  // 16..128 frames of 512^2 on MI355X)
+  const long w = (128L << 20) / ((long)m * m * 8);
   return (int)std::max(1L, std::min((long)n, w));
 }
 

@@ -1,0 +1,182 @@
+"""Host driver (include/ptycho.h): parameters, scan geometry, CSV format, Python argument
+handling.  No GPU calls."""
+import numpy as np
+import pytest
+
+from ptychopy_amd import _driver as D
+
+
+def f32(x):
+    return np.float32(x)
+
+
+# ---------------------------------------------------------------- a1: sample-plane pixel size
+def test_dx_s_is_rounded_to_float():
+    v = D.sample_pixel_size(0.248e-9, 1.0, 512, 172e-6)
+    assert v == float(np.float32(0.248e-9 * 1.0 / (512 * 172e-6)))
+    assert abs(v - 2.81613e-9) < 1e-14  # 0.248 nm * 1 m / (512 * 172 um)
+    assert v != 0.248e-9 / (512 * 172e-6)  # the double value is not kept
+
+
+# ---------------------------------------------------------------- a2/a3: Cartesian mesh
+def _cartesian_closed_form(nx, ny, step, dx_s, m):
+    """Float32 arithmetic of d_generateMesh<1> with no jitter or drift, extents with the
+    pitched buffer's zero padding (rows padded to 32), canvas ceil(max - min + M)."""
+    o = f32(1.0 / dx_s)
+    s = f32(step)
+    px = np.empty((nx, ny), np.float32)
+    py = np.empty((nx, ny), np.float32)
+    for i in range(nx):
+        for j in range(ny):
+            vx = (f32(i) - f32(nx - 1) * f32(0.5)) * s
+            vy = (f32(j) - f32(ny - 1) * f32(0.5)) * s
+            px[i, j] = vx * o
+            py[i, j] = vy * o
+    mx = [px.max(), py.max()]
+    mn = [px.min(), py.min()]
+    if ny % 32:
+        mx = [max(mx[0], f32(0)), max(mx[1], f32(0))]
+        mn = [min(mn[0], f32(0)), min(mn[1], f32(0))]
+    canvas = (int(np.ceil(f32(mx[0] - mn[0]) + f32(m))), int(np.ceil(f32(mx[1] - mn[1]) + f32(m))))
+    pos = np.stack([(px - mn[0]).ravel(), (py - mn[1]).ravel()], axis=1)
+    return pos, canvas
+
+
+@pytest.mark.parametrize("nx,ny,step,size", [(30, 30, 50e-9, 512), (12, 12, 40e-9, 256), (51, 51, 100e-9, 256),
+                                             (5, 7, 33e-9, 128), (32, 32, 40e-9, 64)])
+def test_cartesian_mesh_bit_exact(nx, ny, step, size):
+    dx_s = D.sample_pixel_size(0.248e-9, 1.0, size, 172e-6)
+    pos, canvas, ext = D.cartesian_mesh(nx, ny, step, step, dx_s, size)
+    ref, rcanvas = _cartesian_closed_form(nx, ny, step, dx_s, size)
+    assert canvas == rcanvas
+    assert np.array_equal(pos.view(np.uint32), ref.astype(np.float32).view(np.uint32))
+    # raster order: row index i is the first axis (x), j the contiguous one (y)
+    assert pos[1, 1] > pos[0, 1] and pos[ny, 0] > pos[0, 0]
+
+
+def test_cartesian_c1_geometry():
+    dx_s = D.sample_pixel_size(0.248e-9, 1.0, 512, 172e-6)
+    pos, canvas, _ = D.cartesian_mesh(30, 30, 50e-9, 50e-9, dx_s, 512)
+    step_px = np.diff(pos[:30, 1]).mean()
+    assert abs(step_px - 50e-9 / 2.816134e-9) < 1e-3  # 17.7548 px
+    assert canvas == (1027, 1027)
+    assert pos.min() == 0.0
+
+
+def test_cartesian_mirror_flip_and_single_position():
+    dx_s = 1e-8
+    a, _, _ = D.cartesian_mesh(4, 6, 1e-7, 1e-7, dx_s, 64)
+    b, _, _ = D.cartesian_mesh(4, 6, 1e-7, 1e-7, dx_s, 64, mirror_x=True)
+    assert np.allclose(a[:, 0].reshape(4, 6)[::-1], b[:, 0].reshape(4, 6))
+    c, _, _ = D.cartesian_mesh(4, 6, 1e-7, 1e-7, dx_s, 64, flip=True)
+    assert np.array_equal(c[:, 0], a[:, 1]) and np.array_equal(c[:, 1], a[:, 0])
+    p, canvas, _ = D.cartesian_mesh(1, 1, 1e-7, 1e-7, dx_s, 64)
+    assert canvas == (64, 64) and p.tolist() == [[0.0, 0.0]]
+
+
+def test_cartesian_jitter_bounds_and_seed():
+    dx_s = 1e-8
+    a, _, _ = D.cartesian_mesh(12, 12, 16e-8, 16e-8, dx_s, 128)
+    j1, _, e1 = D.cartesian_mesh(12, 12, 16e-8, 16e-8, dx_s, 128, jitter_radius=1, seed=5)
+    j2, _, e2 = D.cartesian_mesh(12, 12, 16e-8, 16e-8, dx_s, 128, jitter_radius=1, seed=5)
+    j3, _, _ = D.cartesian_mesh(12, 12, 16e-8, 16e-8, dx_s, 128, jitter_radius=1, seed=6)
+    assert np.array_equal(j1, j2) and not np.array_equal(j1, j3)
+    raw_a = a + 0  # unshifted difference: jitter in [-1, 1)
+    d = (j1 + e1[:2]) - (raw_a + np.array([-88.0, -88.0], np.float32))
+    assert np.all(np.abs(d) <= 1.0 + 1e-4)
+
+
+# ---------------------------------------------------------------- list / spiral meshes
+def test_list_mesh_extents_include_zero_padding():
+    # all-positive positions: the reference's min over the padded buffer is 0, not min(pos)
+    dx_s = 1e-8
+    metres = np.array([[1e-6, 2e-6], [1.5e-6, 3e-6], [2e-6, 2.5e-6]])
+    pos, canvas, ext = D.list_mesh(metres, dx_s, 64)
+    o = f32(1 / dx_s)
+    raw = np.array([[f32(x) * o, f32(y) * o] for x, y in metres], np.float32)
+    assert ext[0] == 0.0 and ext[1] == 0.0
+    assert np.array_equal(pos, raw)                       # shifted by the padded minimum, 0
+    assert canvas == (int(np.ceil(raw[:, 0].max() + 64)), int(np.ceil(raw[:, 1].max() + 64)))
+
+
+def test_spiral_mesh_closed_form():
+    dx_s = 1e-8
+    pos, canvas, ext = D.spiral_mesh(5, 5, 1e-7, 1e-7, dx_s, 64)
+    o = f32(1 / dx_s)
+    q = np.arange(1024, dtype=np.float32)
+    s = np.sqrt(q)
+    x = (s * np.cos(f32(4) * s).astype(np.float32) * f32(1e-7)).astype(np.float32) * o
+    y = (s * np.sin(f32(4) * s).astype(np.float32) * f32(1e-7)).astype(np.float32) * o
+    # extents cover all 1024 entries of the padded buffer (only 25 are scan points)
+    assert abs(ext[2] - x.max()) <= 1e-3 * abs(x.max()) and abs(ext[0] - x.min()) <= 1e-3 * abs(x.min())
+    assert np.allclose(pos[:, 0], x[:25] - ext[0], rtol=1e-5, atol=1e-3)
+    assert np.allclose(pos[:, 1], y[:25] - ext[1], rtol=1e-5, atol=1e-3)
+
+
+# ---------------------------------------------------------------- CSV
+def test_complex_csv_format_and_round_trip(tmp_path):
+    a = np.array([[1.5 + 2j, -0.25 - 1e-7j], [3.14159265 + 0j, 1e-30 - 2.5e10j]], np.complex64)
+    f = tmp_path / "x.csv"
+    assert D.save_complex(f, a)
+    lines = f.read_text().splitlines()
+    assert lines[0] == "1.5+2j, -0.25-1e-07j"
+    assert lines[1] == "3.14159+0j, 1e-30-2.5e+10j"
+    b = D.load_complex_csv(f)
+    assert b.shape == (2, 2)
+    assert np.allclose(b, a, rtol=1e-5)
+    # exponent only on the imaginary part (misread by the reference's parser)
+    h = tmp_path / "z.csv"
+    h.write_text("0.5+2.3e-06j, (1-2j), 3, -4.5e-03-1e+02j\n")
+    assert np.allclose(D.load_complex_csv(h), [[0.5 + 2.3e-6j, 1 - 2j, 3, -4.5e-3 - 100j]], rtol=1e-6)
+    g = tmp_path / "y.bin"
+    assert D.save_complex(g, a, binary=True)
+    assert np.array_equal(np.fromfile(g, np.complex64).reshape(2, 2), a)
+
+
+# ---------------------------------------------------------------- parameters / Python surface
+def test_parameter_parsers():
+    from ptychopy_amd.ptychopy import Phaser
+    ph = Phaser()
+    ph.parse_cmdline("-jobID=abc -algorithm=DM -scanDims=30,20 -step=50e-9 -size=512 -lambda=0.248e-9 "
+                     "-z=1 -dx_d=172e-6 -i=7 -updateProbe=3")
+    assert ph.get_param("jobID") == "abc" and ph.get_param("algorithm") == "DM"
+    assert ph.get_param("scanDimsx") == "30" and ph.get_param("scanDimsy") == "20"
+    assert ph.get_param("iterations") == "7" and ph.get_param("updateProbe") == "3"
+    assert float(ph.get_param("dx_s")) == D.sample_pixel_size(0.248e-9, 1.0, 512, 172e-6)
+    with pytest.raises(RuntimeError):
+        ph.parse_cmdline("-notAnOption=1")
+    ph.set_param("T", 0)          # T > 0 only (Parameters.cpp:486-489)
+    assert float(ph.get_param("time")) == -1.0
+    ph.set_param("iter", 5)
+    assert ph.get_param("iterations") == "5"
+    with pytest.raises(RuntimeError):
+        ph.set_param("size", 16)  # "the size must be large than 16"
+    ph.close()
+
+
+def test_python_surface_argument_handling(capsys):
+    from ptychopy_amd import ptychopy
+    with pytest.raises(TypeError):
+        ptychopy.epie(notakeyword=1)
+    with pytest.raises(TypeError):
+        ptychopy.dm("job", iter="many")
+    assert ptychopy.epienp(jobID="x") is None  # no diffractionNP: message, None
+    assert "Please input a diffraction pattern numpy array" in capsys.readouterr().out
+    with pytest.raises(ValueError, match="2 dimensional object"):
+        ptychopy.epienp(diffractionNP=np.zeros((1, 32, 32)), objectNP=np.zeros(3, complex))
+    names = set(ptychopy.__all__)
+    assert {"epie", "dm", "mls", "epienp", "dmnp", "mlsnp", "epiecmdstr", "dmcmdstr", "mlscmdstr", "epieinit",
+            "epiepost", "epiestep", "epieresobj", "epieresprobe", "epienpinit", "mlsinit", "mlspost", "mlsstep",
+            "mlsresobj", "mlsresprobe", "helloworld"} <= names
+
+
+def test_driver_exports():
+    import ctypes
+    import re
+    from pathlib import Path
+    from ptychopy_amd import _engine
+    text = (Path(__file__).resolve().parent.parent / "include" / "ptycho.h").read_text()
+    declared = set(re.findall(r"\b(ptycho_[a-z0-9_]+)\s*\(", text))
+    assert declared == set(D.DRIVER_EXPORTS)
+    lib = ctypes.CDLL(str(_engine.LIB_PATH))
+    assert not [s for s in declared if not hasattr(lib, s)]

@@ -1,0 +1,92 @@
+"""End to end through the Python `ptychopy` surface and the C++ IPhaser (GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+LAMBD, Z, DX_D = 0.248e-9, 1.0, 172e-6
+
+
+def _dataset(m=64, n=5, step=6e-8, seed=0):
+    """Frames simulated on the driver's own Cartesian mesh (same arithmetic as the run)."""
+    import oracle
+    from ptychopy_amd import _driver as D, sim
+    dx_s = D.sample_pixel_size(LAMBD, Z, m, DX_D)
+    pos, canvas, _ = D.cartesian_mesh(n, n, step, step, dx_s, m)
+    rng = np.random.default_rng(seed)
+    obj = sim.random_field_object(canvas, rng)
+    probe = sim.gaussian_probe(m, fwhm=m / 3)
+    amp = sim.poisson_amplitudes(oracle.simulate(obj, probe, pos, squared=True), 1e6, rng)
+    kw = dict(size=m, scanDimsx=n, scanDimsy=n, stepx=step, stepy=step, lambd=LAMBD, z=Z, dx_d=DX_D,
+              beamSize=1e-7)
+    return amp, pos, canvas, kw, dx_s
+
+
+def test_dmnp_matches_engine_and_writes_results(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import _driver as D, ptychopy
+    from ptychopy_amd._engine import Engine, FLAG_PHASE_CONSTRAINT, FLAG_UPDATE_PROBE
+    monkeypatch.chdir(tmp_path)
+    amp, pos, canvas, kw, dx_s = _dataset()
+    assert ptychopy.dmnp(jobID="t", diffractionNP=amp.astype(np.float64), iter=12, **kw) is None
+    obj = D.load_complex_csv("data/t_object_0.csv")
+    prb = D.load_complex_csv("data/t_probes_0.csv")
+    assert obj.shape == canvas and prb.shape == (64, 64)
+    with Engine(len(pos), 64, canvas) as e:
+        e.set_frames(amp)
+        e.set_positions(pos)
+        e.clear_object()
+        e.init_probe(1e-7, dx_s)
+        for i in range(1, 13):
+            e.dm_iteration((FLAG_PHASE_CONSTRAINT if i <= 1 else 0) | (FLAG_UPDATE_PROBE if i >= 10 else 0))
+        ref = e.get_object()
+    assert rel_err(obj, ref) < 2e-6  # CSV keeps 6 significant digits
+
+
+def test_epie_step_mode_equals_whole_run(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    amp, pos, canvas, kw, _ = _dataset(seed=1)
+    assert ptychopy.epienpinit(jobID="s", diffractionNP=amp, iter=4, seed=3, **kw) is True
+    for _ in range(4):
+        assert ptychopy.epiestep() is True
+    o_step = ptychopy.epieresobj()
+    p_step = ptychopy.epieresprobe()
+    assert ptychopy.epiepost() is True
+    ph = ptychopy._make_phaser("ePIE", *ptychopy._collect((), dict(diffractionNP=amp, iter=4, seed=3, **kw), True))
+    ph.init()
+    ph.add_method("ePIE", 4)
+    ph.phase()
+    assert np.array_equal(ph.get_object(), o_step)
+    assert np.array_equal(ph.get_probe(), p_step)
+    assert p_step.shape == (64, 64) and np.isfinite(o_step).all()
+    ph.close()
+
+
+def test_cmdstr_simulated_run_returns_the_object(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    o = ptychopy.epiecmdstr("-jobID=c -simulate=1 -scanDims=4,4 -step=8e-8 -size=64 -lambda=0.248e-9 -z=1 "
+                            "-dx_d=172e-6 -beamSize=1e-7 -i=3")
+    assert o.dtype == np.complex64 and o.ndim == 2 and np.isfinite(o).all()
+    assert os.path.exists("data/c_object_0.csv") and os.path.exists("data/c_probes_0.csv")
+
+
+def test_dm_whole_mode_with_position_list(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import _driver as D, ptychopy
+    monkeypatch.chdir(tmp_path)
+    amp, pos, canvas, kw, dx_s = _dataset(seed=2)
+    # the same scan given as a list in metres (positionNP): list-mesh geometry
+    metres = (pos.astype(np.float64) - pos.mean(axis=0)) * dx_s
+    lpos, lcanvas, _ = D.list_mesh(metres, dx_s, 64)
+    ph = ptychopy._make_phaser("DM", *ptychopy._collect((), dict(diffractionNP=amp, positionNP=metres, iter=3,
+                                                                  **kw), True))
+    ph.init()
+    assert np.array_equal(ph.get_positions(), lpos)
+    ph.add_method("DM", 3)
+    ph.phase()
+    assert ph.get_object().shape == lcanvas
+    ph.close()
