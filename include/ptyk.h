@@ -136,6 +136,66 @@ enum { PTYK_TUNE_DM_WAVE = 1 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
 int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);
 
+/* ---------------------------------------------------------------- MLs
+ * Maximum-likelihood reconstruction with least-squares steps (MLS::iteration, MLS.cpp:74-490),
+ * the variable probe (Probe::update_variable_probe), sub-pixel probe shifts and gradient
+ * position refinement.  Positions are in pixels and NOT shifted to the canvas (MLs keeps them
+ * centred, CartesianScanMesh.cpp:94-177); ROI origins are round(pos + ceil(rows/2 - M/2)) - 1
+ * per axis (IPtychoScanMesh.cpp:123-172).  The object canvas and the K probe modes are the
+ * handle's (ptyk_set_object / ptyk_set_probe before ptyk_mls_begin). */
+typedef struct {
+  int32_t probe_pos_search;        /* PPS: position refinement from this iteration on (ptychopy: 20) */
+  int32_t probe_reconstruct;       /* probe updates from this iteration on (1) */
+  int32_t object_reconstruct;      /* object updates from this iteration on (1) */
+  int32_t variable_probe;          /* 1 */
+  int32_t variable_intensity;      /* 1 */
+  int32_t variable_probe_modes;    /* 1 (one extra mode) */
+  int32_t apply_subpix_shift;      /* 1 */
+  int32_t apply_multimodal_update; /* 0 */
+  int32_t n_objects;               /* 1 (the only supported value) */
+  int32_t n_rec;                   /* 1: modes are re-orthogonalised after an iteration when K > n_rec */
+  int32_t object_box[4];           /* calc_object_norm box, 1-based inclusive: row0, row1, col0, col1
+                                      (IPtychoScanMesh::precalculate_ROI, IPtychoScanMesh.cpp:179-184) */
+  double delta_p;                  /* preconditioner regulariser (0.1) */
+  double beta_lsq;                 /* 0.9 */
+  double beta_object;              /* 1 */
+  double beta_probe;               /* 1 */
+  int32_t reserved[4];
+} ptyk_mls_config;
+
+/* Start an MLs reconstruction (Probe::initEvo, Probe.cpp:507-613): positions n x 2 (px),
+ * variable_mode M x M complex64 (Probe::initVarModes draws), evolution_draws n values for the
+ * variable-mode coefficients (1e-6 N(0,1) draws in the reference), frame_power n sums of each
+ * frame's intensities (NULL: sums of the squared stored amplitudes).  Normalises the object,
+ * the probe modes and the variable mode, builds the illumination canvas; LSQ steps start at 1.
+ * PTYK_ERANGE if a ROI falls outside the canvas. */
+int ptyk_mls_begin(ptyk_ctx* ctx, const ptyk_mls_config* cfg, const float* positions, const float* variable_mode,
+                   const double* evolution_draws, const double* frame_power);
+/* One MLs iteration (iter is 1-based).  The frames are processed in groups: group g is
+ * group_frames[group_offsets[g] .. group_offsets[g+1]), in the given group order (the
+ * reference's random partition and shuffled order, MLS.cpp:106-113; IPtychoScanMesh.cpp:826-913).
+ * A frame may appear at most once.  fourier_errors (nullable, n values) receives the per-frame
+ * Fourier errors when iter < 10 (MLS.cpp:234-237).  PTYK_ERANGE if refined positions moved a
+ * ROI outside the canvas (the ROI is clamped; the state stays usable). */
+int ptyk_mls_iteration(ptyk_ctx* ctx, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
+                       int32_t n_groups, float* fourier_errors);
+/* MLs state; every non-null buffer is copied out (get) or in (set). */
+typedef struct {
+  float* positions;        /* n x 2 current positions (px) */
+  float* positions_start;  /* n x 2 positions at ptyk_mls_begin */
+  int32_t* roi;            /* n x 2 ROI origins (0-based) */
+  float* subpixel;         /* n x 2 sub-pixel shifts (column fraction, row fraction) */
+  double* evolution;       /* 2 x n probe-evolution coefficients */
+  float* variable_mode;    /* M x M complex64 */
+  float* illumination;     /* rows x cols cached illumination */
+  float* beta_object;      /* n complex64 LSQ object steps */
+  float* beta_probe;       /* n complex64 LSQ probe steps */
+  float* fourier_errors;   /* n */
+  double max_illumination; /* Probe::MAX_ILLUM (get: out; set: used when > 0) */
+} ptyk_mls_state;
+int ptyk_mls_get_state(ptyk_ctx* ctx, ptyk_mls_state* st);
+int ptyk_mls_set_state(ptyk_ctx* ctx, const ptyk_mls_state* st);
+
 /* Batched in-place 2D FFT of n frames of M x M complex64 (host buffers), forward
  * exp(-2 pi i kn/M), inverse unnormalised (CudaFFTPlan::getC2CPlan, CudaFFTPlan.cpp:63-101). */
 int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);

@@ -43,6 +43,14 @@ def lib():
         _lib.orc_simulate.argtypes = [P, i, i, P, P, i64, i, i, i, P]
         _lib.orc_epie_iteration.argtypes = [P, P, i, i, P, P, i, i, i, P, P, P, P, i64, i, i, i, i, i, P]
         _lib.orc_dm_iteration.argtypes = [P, i, i, P, P, P, P, i64, i, i, i, P, P]
+        _lib.orc_mls_roi.argtypes = [P, i, i, i, i, P, P]
+        _lib.orc_mls_init.argtypes = [i, i, i, i, i, P, P, P, P, i, P, P, P, P]
+        _lib.orc_mls_ortho_modes.argtypes = [P, i, i]
+        _lib.orc_mls_shift.argtypes = [P, i, f, f]
+        _lib.orc_mls_lsq.argtypes = [P, P, P, P, P, i, i, P]
+        _lib.orc_mls_position_step.argtypes = [P, P, P, i, P, P, P]
+        _lib.orc_mls_iteration.argtypes = [C.POINTER(MlsParams), i, i, i, i, i, P, P, P, P, P, d, P, P, P, P, P, P,
+                                           P, P, P, i, P]
     return _lib
 
 
@@ -145,3 +153,122 @@ def dm_iteration(obj, probe, psi, amp, pos, factors, *, phase_constraint: bool =
     lib().orc_dm_iteration(_ptr(o), o.shape[0], o.shape[1], _ptr(p), _ptr(s), _ptr(a), _ptr(ps), n, m,
                            int(phase_constraint), int(update_probe), _ptr(fac), _ptr(sums))
     return o, p, s, fac, sums
+
+
+# ---------------------------------------------------------------------------- MLs
+class MlsParams(C.Structure):
+    """Mirror of oracle_mls.cpp MlsPrm (reference defaults, Parameters.h:158-165)."""
+    _fields_ = [("iter", C.c_int32), ("pps", C.c_int32), ("probe_reconstruct", C.c_int32),
+                ("object_reconstruct", C.c_int32), ("variable_probe", C.c_int32), ("variable_intensity", C.c_int32),
+                ("variable_probe_modes", C.c_int32), ("apply_subpix_shift", C.c_int32),
+                ("apply_multimodal_update", C.c_int32), ("n_objects", C.c_int32), ("n_rec", C.c_int32),
+                ("reserved", C.c_int32), ("box", C.c_int32 * 4), ("delta_p", C.c_double), ("beta_lsq", C.c_double),
+                ("beta_object", C.c_double), ("beta_probe", C.c_double)]
+
+
+def mls_params(box, **kw) -> MlsParams:
+    d = dict(iter=1, pps=20, probe_reconstruct=1, object_reconstruct=1, variable_probe=1, variable_intensity=1,
+             variable_probe_modes=1, apply_subpix_shift=1, apply_multimodal_update=0, n_objects=1, n_rec=1,
+             delta_p=0.1, beta_lsq=0.9, beta_object=1.0, beta_probe=1.0)
+    d.update(kw)
+    p = MlsParams()
+    for key, v in d.items():
+        setattr(p, key, v)
+    for i in range(4):
+        p.box[i] = int(box[i])
+    return p
+
+
+def mls_roi(pos, m: int, rows: int, cols: int):
+    ps = np.ascontiguousarray(pos, dtype=np.float32)
+    n = ps.shape[0]
+    roi = np.zeros((n, 2), np.int32)
+    sub = np.zeros((n, 2), np.float32)
+    lib().orc_mls_roi(_ptr(ps), n, m, rows, cols, _ptr(roi), _ptr(sub))
+    return roi, sub
+
+
+def mls_shift(x, sx: float, sy: float) -> np.ndarray:
+    """Probe::shift_probe on one frame: sx along columns, sy along rows."""
+    a = _c128(x).copy()
+    lib().orc_mls_shift(_ptr(a), a.shape[-1], float(np.float32(sx)), float(np.float32(sy)))
+    return a
+
+
+def mls_position_step(op, vp=None, chi=None):
+    """(dx, dy, dX, dY) of the gradient position solver for one frame."""
+    o = _c128(op)
+    m = o.shape[-1]
+    v = None if vp is None else _c128(vp)
+    c = None if chi is None else _c128(chi)
+    out = np.zeros(2, np.float64)
+    dX = np.zeros((m, m), np.complex128)
+    dY = np.zeros((m, m), np.complex128)
+    lib().orc_mls_position_step(_ptr(o), None if v is None else _ptr(v), None if c is None else _ptr(c), m, _ptr(out),
+                                _ptr(dX), _ptr(dY))
+    return out[0], out[1], dX, dY
+
+
+def mls_lsq(oup, vp, pum, op, chi) -> np.ndarray:
+    """(G, 2) optimal (object, probe) steps of one group."""
+    arrs = [_c128(a) for a in (oup, vp, pum, op, chi)]
+    g, m = arrs[0].shape[0], arrs[0].shape[-1]
+    x = np.zeros((g, 2), np.complex128)
+    lib().orc_mls_lsq(*[_ptr(a) for a in arrs], g, m, _ptr(x))
+    return x
+
+
+def mls_ortho_modes(probe) -> np.ndarray:
+    p = _c128(probe).copy()
+    lib().orc_mls_ortho_modes(_ptr(p), p.shape[0], p.shape[-1])
+    return p
+
+
+class MlsState:
+    """Everything an MLs iteration reads and writes (FP64, positions float32)."""
+
+    def __init__(self, obj, probe, vmode, evo1, pos, amp):
+        self.obj = _c128(obj).copy()
+        self.probe = _c128(probe).copy()
+        self.vmode = _c128(vmode).copy()
+        self.pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+        self.pos_o = self.pos.copy()
+        n = self.pos.shape[0]
+        self.evo = np.zeros((2, n), np.float64)
+        self.evo[1] = evo1
+        self.amp = _f64(amp)
+        self.illum = np.zeros(self.obj.shape, np.float64)
+        self.max_illum = 0.0
+        self.roi, self.sub = mls_roi(self.pos, self.probe.shape[-1], *self.obj.shape)
+        self.beta_obj = np.ones(n, np.complex128)
+        self.beta_probe = np.ones(n, np.complex128)
+        self.ferr = np.zeros(n, np.float64)
+
+    def init(self, frame_power, variable_probe_modes: int = 1):
+        """Probe::initEvo; frame_power = per-frame sums of the intensities."""
+        k, m, _ = self.probe.shape
+        fp = _f64(frame_power)
+        mx = np.zeros(1, np.float64)
+        rows, cols = self.obj.shape
+        lib().orc_mls_init(self.pos.shape[0], m, k, rows, cols, _ptr(self.obj), _ptr(self.probe), _ptr(self.vmode),
+                           _ptr(self.evo), variable_probe_modes, _ptr(self.roi), _ptr(fp), _ptr(self.illum), _ptr(mx))
+        self.max_illum = float(mx[0])
+
+    def copy(self) -> "MlsState":
+        c = object.__new__(MlsState)
+        for key, v in self.__dict__.items():
+            setattr(c, key, v.copy() if isinstance(v, np.ndarray) else v)
+        return c
+
+    def iteration(self, prm: MlsParams, groups):
+        """groups: list of frame-index arrays in processing order."""
+        gf = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32) for g in groups]), dtype=np.int32)
+        go = np.zeros(len(groups) + 1, np.int32)
+        go[1:] = np.cumsum([len(g) for g in groups])
+        k, m, _ = self.probe.shape
+        rows, cols = self.obj.shape
+        lib().orc_mls_iteration(C.byref(prm), self.pos.shape[0], m, k, rows, cols, _ptr(self.obj), _ptr(self.probe),
+                                _ptr(self.vmode), _ptr(self.evo), _ptr(self.illum), self.max_illum, _ptr(self.pos),
+                                _ptr(self.pos_o), _ptr(self.roi), _ptr(self.sub), _ptr(self.beta_obj),
+                                _ptr(self.beta_probe), _ptr(self.amp), _ptr(gf), _ptr(go), len(groups),
+                                _ptr(self.ferr))

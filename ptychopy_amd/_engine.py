@@ -34,7 +34,8 @@ EXPORTS = (
     "ptyk_get_object", "ptyk_clear_object", "ptyk_set_probe", "ptyk_get_probe", "ptyk_set_psi", "ptyk_get_psi",
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
-    "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning",
+    "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
+    "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state",
 )
 
 TUNE_DM_WAVE = 1
@@ -52,6 +53,43 @@ class Config(C.Structure):
     _fields_ = [("n_frames", C.c_int32), ("frame_size", C.c_int32), ("n_modes", C.c_int32),
                 ("obj_rows", C.c_int32), ("obj_cols", C.c_int32), ("device", C.c_int32),
                 ("tex_quantize", C.c_int32), ("reserved", C.c_int32)]
+
+
+class MlsConfig(C.Structure):
+    """ptyk_mls_config (include/ptyk.h); defaults are the reference's (Parameters.h:158-165)."""
+    _fields_ = [("probe_pos_search", C.c_int32), ("probe_reconstruct", C.c_int32),
+                ("object_reconstruct", C.c_int32), ("variable_probe", C.c_int32),
+                ("variable_intensity", C.c_int32), ("variable_probe_modes", C.c_int32),
+                ("apply_subpix_shift", C.c_int32), ("apply_multimodal_update", C.c_int32),
+                ("n_objects", C.c_int32), ("n_rec", C.c_int32), ("object_box", C.c_int32 * 4),
+                ("delta_p", C.c_double), ("beta_lsq", C.c_double), ("beta_object", C.c_double),
+                ("beta_probe", C.c_double), ("reserved", C.c_int32 * 4)]
+
+
+MLS_DEFAULTS = dict(probe_pos_search=20, probe_reconstruct=1, object_reconstruct=1, variable_probe=1,
+                    variable_intensity=1, variable_probe_modes=1, apply_subpix_shift=1, apply_multimodal_update=0,
+                    n_objects=1, n_rec=1, delta_p=0.1, beta_lsq=0.9, beta_object=1.0, beta_probe=1.0)
+
+
+def mls_config(box, **kw) -> MlsConfig:
+    d = dict(MLS_DEFAULTS)
+    for key in kw:
+        if key not in d:
+            raise KeyError(key)
+    d.update(kw)
+    c = MlsConfig()
+    for key, v in d.items():
+        setattr(c, key, v)
+    for i in range(4):
+        c.object_box[i] = int(box[i])
+    return c
+
+
+class MlsState(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("positions_start", C.c_void_p), ("roi", C.c_void_p),
+                ("subpixel", C.c_void_p), ("evolution", C.c_void_p), ("variable_mode", C.c_void_p),
+                ("illumination", C.c_void_p), ("beta_object", C.c_void_p), ("beta_probe", C.c_void_p),
+                ("fourier_errors", C.c_void_p), ("max_illumination", C.c_double)]
 
 
 _lib = None
@@ -87,6 +125,10 @@ def load():
     lib.ptyk_simulate.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
     lib.ptyk_set_tuning.argtypes = [P, i32, i32]
     lib.ptyk_get_tuning.argtypes = [P, i32, C.POINTER(C.c_int32)]
+    lib.ptyk_mls_begin.argtypes = [P, C.POINTER(MlsConfig), P, P, P, P]
+    lib.ptyk_mls_iteration.argtypes = [P, i32, P, P, i32, P]
+    lib.ptyk_mls_get_state.argtypes = [P, C.POINTER(MlsState)]
+    lib.ptyk_mls_set_state.argtypes = [P, C.POINTER(MlsState)]
     _lib = lib
     return lib
 
@@ -263,3 +305,50 @@ class Engine:
         self._chk(self.lib.ptyk_simulate(self.h, _ptr(o), o.shape[0], o.shape[1], _ptr(p), _ptr(ps), ps.shape[0],
                                          int(squared), _ptr(out)))
         return out
+
+    # ---- MLs
+    def mls_begin(self, cfg: MlsConfig, positions, variable_mode, evolution_draws, frame_power=None):
+        p = np.ascontiguousarray(positions, dtype=np.float32)
+        v = np.ascontiguousarray(variable_mode, dtype=np.complex64)
+        e = np.ascontiguousarray(evolution_draws, dtype=np.float64)
+        assert p.shape == (self.n, 2) and v.shape == (self.m, self.m) and e.shape == (self.n,)
+        fp = None if frame_power is None else np.ascontiguousarray(frame_power, dtype=np.float64)
+        self._chk(self.lib.ptyk_mls_begin(self.h, C.byref(cfg), _ptr(p), _ptr(v), _ptr(e),
+                                          None if fp is None else _ptr(fp)))
+
+    def mls_iteration(self, it: int, groups):
+        """groups: sequence of frame-index arrays in processing order; returns Fourier errors."""
+        gf = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32) for g in groups]), dtype=np.int32)
+        go = np.zeros(len(groups) + 1, np.int32)
+        go[1:] = np.cumsum([len(g) for g in groups])
+        ferr = np.zeros(self.n, np.float32)
+        self._chk(self.lib.ptyk_mls_iteration(self.h, int(it), _ptr(gf), _ptr(go), len(groups), _ptr(ferr)))
+        return ferr
+
+    def mls_get_state(self) -> dict:
+        n, m = self.n, self.m
+        out = dict(positions=np.empty((n, 2), np.float32), positions_start=np.empty((n, 2), np.float32),
+                   roi=np.empty((n, 2), np.int32), subpixel=np.empty((n, 2), np.float32),
+                   evolution=np.empty((2, n), np.float64), variable_mode=np.empty((m, m), np.complex64),
+                   illumination=np.empty(self.obj_shape, np.float32), beta_object=np.empty(n, np.complex64),
+                   beta_probe=np.empty(n, np.complex64), fourier_errors=np.empty(n, np.float32))
+        st = MlsState(**{k: _ptr(v) for k, v in out.items()}, max_illumination=0.0)
+        self._chk(self.lib.ptyk_mls_get_state(self.h, C.byref(st)))
+        out["max_illumination"] = st.max_illumination
+        return out
+
+    def mls_set_state(self, **arrays):
+        """Copy in any of the mls_get_state arrays (and max_illumination)."""
+        kinds = dict(positions=np.float32, positions_start=np.float32, roi=np.int32, subpixel=np.float32,
+                     evolution=np.float64, variable_mode=np.complex64, illumination=np.float32,
+                     beta_object=np.complex64, beta_probe=np.complex64, fourier_errors=np.float32)
+        keep = {}
+        st = MlsState()
+        for key, v in arrays.items():
+            if key == "max_illumination":
+                st.max_illumination = float(v)
+                continue
+            a = np.ascontiguousarray(v, dtype=kinds[key])
+            keep[key] = a
+            setattr(st, key, _ptr(a))
+        self._chk(self.lib.ptyk_mls_set_state(self.h, C.byref(st)))

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +57,9 @@ struct ptyk_ctx {
   std::vector<int> wave_lo, wave_hi;
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> evpool;
+  // MLs state
+  MlsDev mls;
+  bool mls_ready = false;
   // host state
   std::vector<float2> pos;
   std::vector<int2> roi;
@@ -94,7 +98,19 @@ static hipError_t dalloc(T** p, size_t count) {
   return e;
 }
 
+static void free_mls(ptyk_ctx* c) {
+  MlsDev& d = c->mls;
+  void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
+                  d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal,
+                  d.gframes};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  d = MlsDev();
+  c->mls_ready = false;
+}
+
 static void free_all(ptyk_ctx* c) {
+  free_mls(c);
   void* ptrs[] = {c->tw, c->obj, c->obj_int, c->probe, c->probe_int, c->probe_tmp, c->amp, c->amp_scale,
                   c->amp_sum, c->avg, c->pos_dev, c->roi_dev, c->order_dev, c->scal, c->factors, c->e_psi_old,
                   c->e_psi, c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial,
@@ -731,6 +747,224 @@ int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
     return PTYK_OK;
   }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
+}
+
+// ---------------------------------------------------------------- MLs
+// Group scratch for groups of up to g frames (grown on demand).
+static int mls_reserve(ptyk_ctx* c, int g) {
+  MlsDev& d = c->mls;
+  const long mm = c->mm;
+  const int ch = mls_chunks(d.m);
+  const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
+  if (g > d.gmax) {
+    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.gscal};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+    d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = nullptr;
+    d.gscal = nullptr;
+    d.gmax = 0;
+    const long gm = (long)g * mm;
+    HIPCHK(c, dalloc(&d.op, gm));
+    HIPCHK(c, dalloc(&d.vp, gm));
+    HIPCHK(c, dalloc(&d.psi, gm * d.k));
+    HIPCHK(c, dalloc(&d.pu, gm));
+    HIPCHK(c, dalloc(&d.tmp, std::max<long>(g, d.k) * mm));
+    HIPCHK(c, dalloc(&d.tmp2, gm));
+    HIPCHK(c, dalloc(&d.gscal, (size_t)MLS_S_COEF + g));
+    d.gmax = g;
+  }
+  const long need = std::max<long>({6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L});
+  if (need > d.part_cap) {
+    if (d.part) hipFree(d.part);
+    d.part = nullptr;
+    d.part_cap = 0;
+    HIPCHK(c, dalloc(&d.part, (size_t)need));
+    d.part_cap = need;
+  }
+  return PTYK_OK;
+}
+
+int ptyk_mls_begin(ptyk_ctx* c, const ptyk_mls_config* cfg, const float* positions, const float* variable_mode,
+                   const double* evolution_draws, const double* frame_power) {
+  if (!c || !cfg || !positions || !variable_mode || !evolution_draws) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->have_frames || !c->have_probe) return fail(c, PTYK_ESTATE, "frames and probe must be set");
+  hipSetDevice(c->cfg.device);
+  const int n = c->cfg.n_frames, m = c->cfg.frame_size, rows = c->cfg.obj_rows, cols = c->cfg.obj_cols;
+  if (cfg->n_objects != 1) return fail(c, PTYK_EINVAL, "n_objects must be 1");
+  if (cfg->n_rec < 0 || cfg->delta_p < 0 || !(cfg->beta_lsq >= 0))
+    return fail(c, PTYK_EINVAL, "bad MLs parameters");
+  const int* b = cfg->object_box;
+  if (b[0] < 1 || b[1] < b[0] || b[1] > rows || b[2] < 1 || b[3] < b[2] || b[3] > cols)
+    return fail(c, PTYK_EINVAL, "object_box must lie inside the canvas (1-based, inclusive)");
+  for (int i = 0; i < 2 * n; ++i)
+    if (!std::isfinite(positions[i])) return fail(c, PTYK_EINVAL, "non-finite position");
+  std::vector<double> fp(n);
+  if (frame_power) {
+    for (int i = 0; i < n; ++i) {
+      if (!(frame_power[i] > 0) || !std::isfinite(frame_power[i]))
+        return fail(c, PTYK_EINVAL, "frame_power must be finite and > 0");
+      fp[i] = frame_power[i];
+    }
+  } else {
+    std::vector<float> a((size_t)n * c->mm);
+    int r = ptyk_get_frames_f32(c, a.data());
+    if (r) return r;
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (long q = 0; q < c->mm; ++q) s += (double)a[(size_t)i * c->mm + q] * a[(size_t)i * c->mm + q];
+      if (!(s > 0)) return fail(c, PTYK_EINVAL, "frame " + std::to_string(i) + " is all zero");
+      fp[i] = s;
+    }
+  }
+  free_mls(c);
+  MlsDev& d = c->mls;
+  d.n = n;
+  d.m = m;
+  d.k = c->cfg.n_modes;
+  d.rows = rows;
+  d.cols = cols;
+  d.pps = cfg->probe_pos_search;
+  d.probe_reconstruct = cfg->probe_reconstruct;
+  d.object_reconstruct = cfg->object_reconstruct;
+  d.variable_probe = cfg->variable_probe;
+  d.variable_intensity = cfg->variable_intensity;
+  d.variable_probe_modes = cfg->variable_probe_modes;
+  d.apply_subpix_shift = cfg->apply_subpix_shift;
+  d.apply_multimodal_update = cfg->apply_multimodal_update;
+  d.n_objects = cfg->n_objects;
+  d.n_rec = cfg->n_rec;
+  for (int i = 0; i < 4; ++i) d.box[i] = b[i];
+  d.delta_p = cfg->delta_p;
+  d.beta_lsq = cfg->beta_lsq;
+  d.beta_object = cfg->beta_object;
+  d.beta_probe = cfg->beta_probe;
+  d.obj = c->obj;
+  d.probe = c->probe;
+  d.amp = c->amp;
+  d.amp_scale = c->amp_scale;
+  d.tw = c->tw;
+  const long canvas = (long)rows * cols;
+  HIPCHK(c, dalloc(&d.vmode, c->mm));
+  HIPCHK(c, dalloc(&d.evo, 2 * (size_t)n));
+  HIPCHK(c, dalloc(&d.illum, canvas));
+  HIPCHK(c, dalloc(&d.pos, n));
+  HIPCHK(c, dalloc(&d.pos_o, n));
+  HIPCHK(c, dalloc(&d.roi, n));
+  HIPCHK(c, dalloc(&d.sub, n));
+  HIPCHK(c, dalloc(&d.step_obj, n));
+  HIPCHK(c, dalloc(&d.step_probe, n));
+  HIPCHK(c, dalloc(&d.ferr, n));
+  HIPCHK(c, dalloc(&d.flag, 1));
+  HIPCHK(c, dalloc(&d.spec, 2 * c->mm));
+  HIPCHK(c, dalloc(&d.pum, c->mm));
+  HIPCHK(c, dalloc(&d.acc, c->mm));
+  HIPCHK(c, dalloc(&d.precond, canvas));
+  HIPCHK(c, dalloc(&d.gframes, n));
+  int r = mls_reserve(c, 1);
+  if (r) return r;
+  std::vector<double> evo(2 * (size_t)n, 1.0);
+  for (int i = 0; i < n; ++i) evo[n + i] = evolution_draws[i];
+  HIPCHK(c, hipMemcpyAsync(d.pos, positions, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.pos_o, positions, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.vmode, variable_mode, c->mm * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.evo, evo.data(), evo.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, mls_roi(d, c->stream));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, d.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flag) {
+    free_mls(c);
+    return fail(c, PTYK_ERANGE, "a scan position puts its ROI outside the object canvas");
+  }
+  HIPCHK(c, mls_init(d, fp.data(), c->stream));
+  c->mls_ready = true;
+  return PTYK_OK;
+}
+
+int ptyk_mls_iteration(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
+                       int32_t n_groups, float* fourier_errors) {
+  if (!c || !group_frames || !group_offsets) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->mls_ready) return fail(c, PTYK_ESTATE, "ptyk_mls_begin first");
+  if (iter < 1 || n_groups < 0) return fail(c, PTYK_EINVAL, "iter must be >= 1");
+  hipSetDevice(c->cfg.device);
+  MlsDev& d = c->mls;
+  const int n = d.n;
+  if (group_offsets[0] != 0) return fail(c, PTYK_EINVAL, "group_offsets[0] must be 0");
+  int gmax = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    const int len = group_offsets[g + 1] - group_offsets[g];
+    if (len <= 0) return fail(c, PTYK_EINVAL, "empty group");
+    gmax = std::max(gmax, len);
+  }
+  const int total = group_offsets[n_groups];
+  if (total > n) return fail(c, PTYK_EINVAL, "more frames than positions in the groups");
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < total; ++i) {
+    const int f = group_frames[i];
+    if (f < 0 || f >= n || seen[f]) return fail(c, PTYK_EINVAL, "group frames must be distinct indices < n");
+    seen[f] = 1;
+  }
+  int r = mls_reserve(c, std::max(gmax, 1));
+  if (r) return r;
+  if (total > 0)
+    HIPCHK(c, hipMemcpyAsync(d.gframes, group_frames, total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, mls_iteration(d, iter, group_offsets, n_groups, c->stream));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, d.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (fourier_errors && iter < 10)
+    HIPCHK(c, hipMemcpyAsync(fourier_errors, d.ferr, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // also keeps group_frames alive until the copy ran
+  if (flag) {
+    HIPCHK(c, hipMemsetAsync(d.flag, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return fail(c, PTYK_ERANGE, "refined positions moved a ROI outside the object canvas (clamped)");
+  }
+  return PTYK_OK;
+}
+
+static int mls_state_copy(ptyk_ctx* c, const ptyk_mls_state* st, bool out) {
+  if (!c || !st) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->mls_ready) return fail(c, PTYK_ESTATE, "ptyk_mls_begin first");
+  hipSetDevice(c->cfg.device);
+  MlsDev& d = c->mls;
+  const size_t n = d.n;
+  const hipMemcpyKind kind = out ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+  auto cp = [&](void* host, void* dev, size_t bytes) -> hipError_t {
+    if (!host) return hipSuccess;
+    return out ? hipMemcpyAsync(host, dev, bytes, kind, c->stream) : hipMemcpyAsync(dev, host, bytes, kind, c->stream);
+  };
+  HIPCHK(c, cp(st->positions, d.pos, n * sizeof(float2)));
+  HIPCHK(c, cp(st->positions_start, d.pos_o, n * sizeof(float2)));
+  HIPCHK(c, cp(st->roi, d.roi, n * sizeof(int2)));
+  HIPCHK(c, cp(st->subpixel, d.sub, n * sizeof(float2)));
+  HIPCHK(c, cp(st->evolution, d.evo, 2 * n * sizeof(double)));
+  HIPCHK(c, cp(st->variable_mode, d.vmode, c->mm * sizeof(float2)));
+  HIPCHK(c, cp(st->illumination, d.illum, (size_t)d.rows * d.cols * sizeof(float)));
+  HIPCHK(c, cp(st->beta_object, d.step_obj, n * sizeof(float2)));
+  HIPCHK(c, cp(st->beta_probe, d.step_probe, n * sizeof(float2)));
+  HIPCHK(c, cp(st->fourier_errors, d.ferr, n * sizeof(float)));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+int ptyk_mls_get_state(ptyk_ctx* c, ptyk_mls_state* st) {
+  int r = mls_state_copy(c, st, true);
+  if (r == PTYK_OK) st->max_illumination = c->mls.max_illum;
+  return r;
+}
+
+int ptyk_mls_set_state(ptyk_ctx* c, const ptyk_mls_state* st) {
+  if (st && st->roi) {
+    const MlsDev& d = c->mls;
+    for (int i = 0; i < d.n; ++i) {
+      const int ox = st->roi[2 * i], oy = st->roi[2 * i + 1];
+      if (ox < 0 || oy < 0 || ox + d.m > d.rows || oy + d.m > d.cols)
+        return fail(c, PTYK_ERANGE, "ROI outside the object canvas");
+    }
+  }
+  int r = mls_state_copy(c, st, false);
+  if (r == PTYK_OK && st->max_illumination > 0) c->mls.max_illum = (float)st->max_illumination;
+  return r;
 }
 
 // ---------------------------------------------------------------- utilities

@@ -97,6 +97,53 @@ struct EpieArgs {
 };
 hipError_t epie_iteration(const EpieArgs& a, hipStream_t s);
 
+// ---- MLs (maximum likelihood with least-squares steps), ptyk_mls.hip.
+// Slots of MlsDev::gscal (device FP64 scalars written and read by the MLs kernels).
+enum { MLS_S_ONORM = 0, MLS_S_TMP = 1, MLS_S_BO = 2 /* re, im */, MLS_S_BP = 4 /* re, im */, MLS_S_PMAX = 6,
+       MLS_S_COEF = 16 /* one per frame of the group */ };
+struct MlsDev {
+  // geometry
+  int n = 0, m = 0, k = 0, rows = 0, cols = 0;
+  int gmax = 0;  // capacity (frames) of the group scratch
+  // MLs parameters (ReconstructionParams, Parameters.h:128-146)
+  int pps = 20, probe_reconstruct = 1, object_reconstruct = 1, variable_probe = 1, variable_intensity = 1;
+  int variable_probe_modes = 1, apply_subpix_shift = 1, apply_multimodal_update = 0, n_objects = 1, n_rec = 1;
+  int box[4] = {0, 0, 0, 0};
+  double delta_p = 0.1, beta_lsq = 0.9, beta_object = 1.0, beta_probe = 1.0;
+  float max_illum = 0.f;  // Probe::MAX_ILLUM, fixed by initEvo
+  // context arrays (not owned)
+  float2* obj = nullptr;
+  float2* probe = nullptr;
+  const __half* amp = nullptr;
+  const float* amp_scale = nullptr;
+  const float2* tw = nullptr;
+  // MLs state (owned)
+  float2* vmode = nullptr;      // variable (extra) probe mode, M x M
+  double* evo = nullptr;        // 2 x N probe-evolution coefficients
+  float* illum = nullptr;       // cached illumination canvas (Probe::p_object)
+  float2* pos = nullptr;        // N scan positions, px, not min-shifted
+  float2* pos_o = nullptr;      // N initial positions
+  int2* roi = nullptr;          // N ROI origins (0-based)
+  float2* sub = nullptr;        // N sub-pixel shifts (.x column, .y row)
+  float2* step_obj = nullptr;   // N LSQ object steps
+  float2* step_probe = nullptr; // N LSQ probe steps
+  float* ferr = nullptr;        // N Fourier errors
+  int* flag = nullptr;          // ROI outside the canvas
+  // scratch
+  float2 *op = nullptr, *vp = nullptr, *psi = nullptr, *pu = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+  float2 *spec = nullptr, *pum = nullptr, *acc = nullptr, *precond = nullptr;
+  double* part = nullptr;
+  long part_cap = 0;
+  double* gscal = nullptr;
+  int* gframes = nullptr;       // frames of all groups of one iteration
+};
+int mls_chunks(int m);  // frame chunks of the per-frame reductions
+// Probe::initEvo: normalisations and the initial illumination (host-synchronous).
+hipError_t mls_init(MlsDev& d, const double* frame_power /* host, n */, hipStream_t s);
+// One MLs iteration; group g covers d.gframes[goffs[g] .. goffs[g+1]) (goffs on the host).
+hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s);
+hipError_t mls_roi(MlsDev& d, hipStream_t s);
+
 // ---- misc kernels (ptyk_misc.hip)
 hipError_t quantize_frames(const float* src_dev, __half* dst, float* scales, long n, int m, hipStream_t s);
 hipError_t dequantize_frames(const __half* src, const float* scales, float* dst, long n, int m, hipStream_t s);

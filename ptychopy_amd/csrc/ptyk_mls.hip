@@ -1,0 +1,1035 @@
+// MLs: maximum-likelihood reconstruction with least-squares step sizes, gfx950 form.
+//
+// Follows MLS::iteration (MLS.cpp:74-490) group by group.  Every step of an iteration runs on
+// the device -- including the per-frame 2x2 LSQ solves, the group means of the steps, the
+// position refinement and the probe-evolution updates that the reference finishes on the host
+// after ~130 synchronising reductions per group -- so an iteration is one stream of launches
+// with no host round trip.  Scalars that a later launch needs live in MlsDev::gscal.
+//
+// Deviations from the reference's arithmetic, each deliberate:
+//  * scatters (set_projections, d_addToArray_c with float atomics, utilitiesKernels.cu:1021-1062)
+//    are gathers by canvas tile summing the group's frames in group order: deterministic;
+//  * per-frame reductions accumulate in FP64 in a fixed order (thrust float sums in the
+//    reference); the LSQ 2x2 systems are solved in FP64 (Eigen complex float there);
+//  * the sub-pixel shift of the illumination probe e0*P0 + e1*V is applied as
+//    IFFT(ramp * (e0 FFT(P0) + e1 FFT(V))) -- one inverse transform per frame instead of a
+//    forward and an inverse (Probe::shift_probe on the combined probe, Probe.cpp:1261-1337);
+//    equal up to FP32 rounding by linearity;
+//  * 1/M^2 normalisations (powers of two) are folded into the spectral multiplies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ptyk_device.h"
+#include "ptyk_internal.h"
+
+namespace ptyk {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr float kTwoPi = 6.283185307179586f;
+// float(DBL_EPSILON): the imaginary offset MLS.cpp:301-304 adds to the object update
+constexpr float kEpsF = 2.220446049250313e-16f;
+
+__host__ __device__ inline int ppb_of(int m) { return m * m < 1024 ? m * m : 1024; }
+
+// fftshifted frequency grid i/M - 0.5 rolled by M/2 (Probe.cpp:92-123 with d_shiftY)
+__device__ __forceinline__ float freq(int i, int m) { return (float)((i + (m >> 1)) & (m - 1)) / (float)m - 0.5f; }
+
+__device__ __forceinline__ float2 c_of(double re, double im) { return make_float2((float)re, (float)im); }
+
+// ------------------------------------------------------------------------ ROI origins
+// IPtychoScanMesh::find_reconstruction_ROI (IPtychoScanMesh.cpp:123-172), in its float
+// arithmetic: t = round(pos + ceil(Np_o/2 - M/2)), origin t - 1, sub = (col frac, row frac).
+// An origin outside the canvas is clamped and flagged (the reference reads out of bounds).
+__global__ void k_roi(const float2* __restrict__ pos, int n, int m, int rows, int cols, float cx, float cy,
+                      int2* __restrict__ roi, float2* __restrict__ sub, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float p1 = pos[i].x + cx, p2 = pos[i].y + cy;
+  const float t1 = roundf(p1), t2 = roundf(p2);
+  sub[i] = make_float2(p2 - t2, p1 - t1);
+  int ox = (int)t1 - 1, oy = (int)t2 - 1;
+  if (!(t1 >= 1.f && t2 >= 1.f) || ox + m > rows || oy + m > cols) {
+    flag[0] = 1;
+    ox = min(max(ox, 0), rows - m);
+    oy = min(max(oy, 0), cols - m);
+  }
+  roi[i] = make_int2(ox, oy);
+}
+
+// ------------------------------------------------------------------------ generic helpers
+__global__ void k_scale_c(float2* __restrict__ x, long count, const double* __restrict__ s, int invert) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float f = invert ? (float)(1.0 / *s) : (float)(*s);
+  x[i] = cscale(x[i], f);
+}
+
+__global__ void k_copy_c(float2* __restrict__ dst, const float2* __restrict__ src, long count) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = src[i];
+}
+
+__global__ void k_fill_c(float2* __restrict__ x, long count, float2 v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] = v;
+}
+
+// Canvas reductions into per-block partials: 0 = sum a, 1 = sum a |o|^2, 2 = max a,
+// 3 = sum a^2 over a box, 4 = sum a^2 |o|^2 over a box (calc_object_norm).
+__global__ __launch_bounds__(NT) void k_reduce(const float* __restrict__ a, const float2* __restrict__ o, long count,
+                                               int op, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  __shared__ float redf[NT / 64];
+  double acc = 0.0;
+  float mx = 0.f;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < count; i += (long)gridDim.x * NT) {
+    if (op == 0) acc += a[i];
+    else if (op == 1) acc += (double)a[i] * ((double)o[i].x * o[i].x + (double)o[i].y * o[i].y);
+    else mx = fmaxf(mx, a[i]);
+  }
+  if (op == 2) {
+    mx = block_max<NT>(mx, redf);
+    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+  } else {
+    acc = block_sum<NT>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_box_partial(const float* __restrict__ w, const float2* __restrict__ o,
+                                                    int cols, int x0, int y0, int by, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const long row = (long)(x0 + blockIdx.x) * cols + y0;
+  double sw = 0.0, so = 0.0;
+  for (int c = threadIdx.x; c < by; c += NT) {
+    const double wv = w[row + c];
+    const float2 ov = o[row + c];
+    sw += wv * wv;
+    so += ((double)ov.x * ov.x + (double)ov.y * ov.y) * wv * wv;
+  }
+  sw = block_sum<NT>(sw, red);
+  so = block_sum<NT>(so, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sw;
+    part[2 * blockIdx.x + 1] = so;
+  }
+}
+
+// Probe::calc_object_norm (Probe.cpp:671-699): object_norm = rms(O W) / rms(W) over the box.
+__global__ __launch_bounds__(NT) void k_objnorm(const double* __restrict__ part, int nb, double* __restrict__ gscal) {
+  __shared__ double red[NT / 64];
+  double sw = 0.0, so = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) {
+    sw += part[2 * i];
+    so += part[2 * i + 1];
+  }
+  sw = block_sum<NT>(sw, red);
+  so = block_sum<NT>(so, red);
+  if (threadIdx.x == 0) gscal[MLS_S_ONORM] = sqrt(so) / sqrt(sw);  // the box size cancels
+}
+
+__global__ __launch_bounds__(NT) void k_finish_max(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ float red[NT / 64];
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < nb; i += NT) mx = fmaxf(mx, (float)part[i]);
+  mx = block_max<NT>(mx, red);
+  if (threadIdx.x == 0) *out = mx;
+}
+
+// IPtychoScanMesh::updateScanPositionsDiff (IPtychoScanMesh.cpp:340-369): float sums in
+// position order (one thread, staged through LDS), then pos -= mean(pos) - mean(pos_o).
+__global__ __launch_bounds__(NT) void k_pos_diff(float2* __restrict__ pos, const float2* __restrict__ pos_o, int n) {
+  __shared__ float2 a[1024], b[1024];
+  __shared__ float dxy[2];
+  float s1x = 0.f, s1y = 0.f, s2x = 0.f, s2y = 0.f;
+  for (int base = 0; base < n; base += 1024) {
+    const int cnt = min(1024, n - base);
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      a[i] = pos[base + i];
+      b[i] = pos_o[base + i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < cnt; ++i) {
+        s1x += a[i].x;
+        s1y += a[i].y;
+        s2x += b[i].x;
+        s2y += b[i].y;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float fn = (float)n;
+    dxy[0] = s1x / fn - s2x / fn;
+    dxy[1] = s1y / fn - s2y / fn;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NT) pos[i] = make_float2(pos[i].x - dxy[0], pos[i].y - dxy[1]);
+}
+
+// Probe::remove_extra_degree (Probe.cpp:615-669).  gscal[TMP] holds sum |V|^2.  The mean and
+// the renormalisation of the evolution run over its first two entries (len = number of
+// evolution rows in the reference).
+__global__ __launch_bounds__(NT) void k_remove_extra(float2* __restrict__ vmode, float2* __restrict__ p0, int mm,
+                                                     double* __restrict__ evo, int n,
+                                                     const double* __restrict__ gscal) {
+  __shared__ double cn_s;
+  const float vn = sqrtf((float)(gscal[MLS_S_TMP] / mm));  // h_norm2: float result
+  const float inv = (float)(1.0 / (double)vn);
+  for (int q = threadIdx.x; q < mm; q += NT) vmode[q] = cscale(vmode[q], inv);
+  for (int i = threadIdx.x; i < n; i += NT) evo[n + i] *= vn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int len = n < 2 ? n : 2;
+    double a = 0, b = 0;
+    for (int i = 0; i < len; ++i) {
+      a += evo[i];
+      b += evo[n + i];
+    }
+    const double ma = (float)(a / len), mb = (float)(b / len);
+    double s0 = 0;
+    for (int i = 0; i < len; ++i) {
+      evo[i] = (evo[i] - ma) * 0.99 + 1;
+      evo[n + i] = evo[n + i] - mb;
+      s0 += evo[i] * evo[i];
+    }
+    const double cn = sqrt(s0 / len);
+    for (int i = 0; i < len; ++i) evo[i] /= cn;
+    cn_s = cn;
+  }
+  __syncthreads();
+  const float f = (float)cn_s;
+  for (int q = threadIdx.x; q < mm; q += NT) p0[q] = cscale(p0[q], f);
+}
+
+// Illumination canvas (h_preCalillum / updated_cached_illumination, ProbeKernels.cu:691-730,
+// Probe.cpp:701-729): s = sum over frames covering the pixel of |P0|^2 at the local offset, in
+// frame order; mode 0 stores s, mode 1 the running average (p + s) / 2.  Tiles of 4 x 64.
+__global__ __launch_bounds__(NT) void k_illum(float* __restrict__ illum, const float2* __restrict__ p0,
+                                              const int2* __restrict__ roi, int n, int m, int rows, int cols,
+                                              int mode) {
+  __shared__ int2 lr[NT];
+  const int r0 = blockIdx.y * 4, c0 = blockIdx.x * 64;
+  const int r = r0 + (threadIdx.x >> 6), c = c0 + (threadIdx.x & 63);
+  float s = 0.f;
+  for (int base = 0; base < n; base += NT) {
+    const int cnt = min(NT, n - base);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) lr[threadIdx.x] = roi[base + threadIdx.x];
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const int2 o = lr[j];
+      if (o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
+      const int lrw = r - o.x, lc = c - o.y;
+      if (lrw >= 0 && lrw < m && lc >= 0 && lc < m) {
+        const float2 v = p0[lrw * m + lc];
+        const float a = sqrtf(v.x * v.x + v.y * v.y);
+        s += a * a;
+      }
+    }
+  }
+  if (r < rows && c < cols) {
+    const long i = (long)r * cols + c;
+    illum[i] = mode ? (illum[i] + s) / 2.f : s;
+  }
+}
+
+// ------------------------------------------------------------------------ one group
+// Illumination probe (Probe::get_illumination_probe, Probe.cpp:887-954):
+// vp_j = e0_j P0 + e1_j V, or with the sub-pixel shift its spectrum ramp_j (e0 F0 + e1 FV) / M^2
+// (inverse-transformed by the caller).
+__global__ void k_vp(float2* __restrict__ vp, const float2* __restrict__ spec, const float2* __restrict__ p0,
+                     const float2* __restrict__ vmode, const double* __restrict__ evo, const float2* __restrict__ sub,
+                     const int* __restrict__ gf, int n, int m, int subpix, int var_any) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int j = blockIdx.y, f = gf[j];
+  const float e0 = var_any ? (float)evo[f] : 1.f, e1 = var_any ? (float)evo[n + f] : 0.f;
+  float2 v;
+  if (subpix) {
+    const float2 a = spec[q], b = spec[mm + q];
+    v = make_float2(e0 * a.x + e1 * b.x, e0 * a.y + e1 * b.y);
+    const int r = q / m, c = q % m;
+    const float2 s = sub[f];
+    const float nom = s.x * freq(c, m) + s.y * freq(r, m);
+    float sn, cs;
+    sincosf(nom * -kTwoPi, &sn, &cs);
+    v = cscale(cmul(v, make_float2(cs, sn)), 1.f / (float)mm);
+  } else {
+    const float2 a = p0[q], b = vmode[q];
+    v = make_float2(e0 * a.x + e1 * b.x, e0 * a.y + e1 * b.y);
+  }
+  vp[(long)j * mm + q] = v;
+}
+
+// obj_proj = O at the ROI (d_readFromArray_c, utilitiesKernels.cu:1064-1084);
+// psi_0 = obj_proj * vp (with the sub-pixel shift) or P0 * obj_proj, psi_k = P_k * obj_proj.
+__global__ void k_psi(float2* __restrict__ psi, float2* __restrict__ op, const float2* __restrict__ vp,
+                      const float2* __restrict__ obj, const float2* __restrict__ probe, const int2* __restrict__ roi,
+                      const int* __restrict__ gf, int G, int K, int m, int cols, int subpix) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int j = blockIdx.y;
+  const int2 o = roi[gf[j]];
+  const int r = q / m, c = q % m;
+  const float2 O = obj[(long)(o.x + r) * cols + o.y + c];
+  op[(long)j * mm + q] = O;
+  psi[(long)j * mm + q] = subpix ? cmul(O, vp[(long)j * mm + q]) : cmul(probe[q], O);
+  for (int k = 1; k < K; ++k) psi[((long)k * G + j) * mm + q] = cmul(probe[(long)k * mm + q], O);
+}
+
+// Fourier error partials and the modulus constraint (Diffractions::get_fourier_error and
+// modulus_constraint, Diffractions.cpp:261-310; d_modulus_amplitude DiffractionsKernels.cu:127-137):
+// R = A / (|Psi| + 1e-9) - 1 (the 1e-9 literal makes it a double expression), Psi_k *= R / M^2.
+__global__ __launch_bounds__(NT) void k_modulus(float2* __restrict__ psi, const __half* __restrict__ amp,
+                                                const float* __restrict__ amp_scale, const int* __restrict__ gf,
+                                                int G, int K, int m, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int mm = m * m, ppb = ppb_of(m);
+  const int j = blockIdx.y, f = gf[j];
+  const float sc = amp_scale[f];
+  const float inv_mm = 1.f / (float)mm;
+  double e = 0.0;
+  for (int t = threadIdx.x; t < ppb; t += NT) {
+    const int q = blockIdx.x * ppb + t;
+    float s2 = 0.f;
+    for (int k = 0; k < K; ++k) s2 += cabs2(psi[((long)k * G + j) * mm + q]);
+    const float ap = sqrtf(s2);
+    const float A = load_amp(amp, (long)f * mm + q, sc);
+    const double d = (double)(A - ap);
+    e += d * d * 4.0;
+    const float R = (float)((double)A / ((double)ap + 1e-9) - 1.0) * inv_mm;
+    for (int k = 0; k < K; ++k) {
+      const long i = ((long)k * G + j) * mm + q;
+      psi[i] = cscale(psi[i], R);
+    }
+  }
+  e = block_sum<NT>(e, red);
+  if (threadIdx.x == 0) part[(long)j * gridDim.x + blockIdx.x] = e;
+}
+
+// err_j = sqrt(mean((A - |Psi|)^2 / 0.25)) (the division by 0.25 doubles the RMS).
+__global__ void k_ferr(const double* __restrict__ part, int ch, const int* __restrict__ gf, int G, int mm,
+                       float* __restrict__ ferr) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  double s = 0.0;
+  for (int c = 0; c < ch; ++c) s += part[(long)j * ch + c];
+  ferr[gf[j]] = (float)sqrt(s / mm);
+}
+
+// probe_update_j = chi_0 conj(obj_proj_j) (h_multiplyConju).
+__global__ void k_pu(float2* __restrict__ pu, const float2* __restrict__ chi, const float2* __restrict__ op, long count) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    const float2 a = chi[i], b = op[i];
+    pu[i] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  }
+}
+
+// x_j *= exp(-2 pi i sign (sub.x f[col] + sub.y f[row])) / M^2 (spectral shift, 1/M^2 folded).
+__global__ void k_ramp(float2* __restrict__ x, const float2* __restrict__ sub, const int* __restrict__ gf, int m,
+                       float sign) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int j = blockIdx.y;
+  const float2 s = sub[gf[j]];
+  const int r = q / m, c = q % m;
+  const float nom = (sign * s.x) * freq(c, m) + (sign * s.y) * freq(r, m);
+  float sn, cs;
+  sincosf(nom * -kTwoPi, &sn, &cs);
+  const long i = (long)j * mm + q;
+  x[i] = cscale(cmul(x[i], make_float2(cs, sn)), 1.f / (float)mm);
+}
+
+// probe_update_m = mean_j (p / (p + 0.01 MAX_ILLUM))_roi_j probe_update_j  (MLS.cpp:251-279,
+// d_addFactorDivide utilitiesKernels.cu:171-179).  on_the_fly: probe_update_j = chi conj(op_j).
+__global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, const float2* __restrict__ op,
+                      const float* __restrict__ illum, const int2* __restrict__ roi, const int* __restrict__ gf, int G,
+                      int m, int cols, float fw, int on_the_fly) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int r = q / m, c = q % m;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int j = 0; j < G; ++j) {
+    const int2 o = roi[gf[j]];
+    const float p = illum[(long)(o.x + r) * cols + o.y + c];
+    const float w = p / (p + fw);
+    float2 u = src[(long)j * mm + q];
+    if (on_the_fly) {
+      const float2 b = op[(long)j * mm + q];
+      u = make_float2(u.x * b.x + u.y * b.y, u.y * b.x - u.x * b.y);
+    }
+    acc.x += w * u.x;
+    acc.y += w * u.y;
+  }
+  pum[q] = cscale(acc, (float)(1.0 / G));
+}
+
+// Object update direction of frame j at local pixel q: chi conj(vp_j) for the first probe mode,
+// P_llp chi (no conjugate, MLS.cpp:294-297) for the others.
+__device__ __forceinline__ float2 obj_dir(const float2* __restrict__ chi, const float2* __restrict__ vp,
+                                          const float2* __restrict__ pl, long i, int q) {
+  const float2 a = chi[i];
+  if (pl == nullptr) {
+    const float2 b = vp[i];
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  }
+  return cmul(pl[q], a);
+}
+
+// object_upd_sum = (0, eps) + set_projections(direction), preconditioned by
+// 1 / (p + delta_p MAX_ILLUM) (d_object_sum_update_Gfun, utilitiesKernels.cu:181-189).
+// Gather by 4 x 64 canvas tile over the group's frames in group order.
+__global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const float* __restrict__ illum,
+                                                const float2* __restrict__ chi, const float2* __restrict__ vp,
+                                                const float2* __restrict__ pl, const int2* __restrict__ roi,
+                                                const int* __restrict__ gf, int G, int m, int rows, int cols, float fd,
+                                                int scatter) {
+  __shared__ int2 lr[NT];
+  const int mm = m * m;
+  const int r0 = blockIdx.y * 4, c0 = blockIdx.x * 64;
+  const int r = r0 + (threadIdx.x >> 6), c = c0 + (threadIdx.x & 63);
+  float2 u = make_float2(0.f, kEpsF);
+  if (scatter) {
+    for (int base = 0; base < G; base += NT) {
+      const int cnt = min(NT, G - base);
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) lr[threadIdx.x] = roi[gf[base + threadIdx.x]];
+      __syncthreads();
+      for (int jj = 0; jj < cnt; ++jj) {
+        const int2 o = lr[jj];
+        if (o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
+        const int lrw = r - o.x, lc = c - o.y;
+        if (lrw >= 0 && lrw < m && lc >= 0 && lc < m) {
+          const int q = lrw * m + lc;
+          u = cadd(u, obj_dir(chi, vp, pl, (long)(base + jj) * mm + q, q));
+        }
+      }
+    }
+  }
+  if (r < rows && c < cols) {
+    const long i = (long)r * cols + c;
+    if (scatter) {
+      const float d = illum[i] + fd;
+      u = make_float2(u.x / d, u.y / d);
+    }
+    pre[i] = u;
+  }
+}
+
+// out_j = canvas at ROI j (get_projections).
+__global__ void k_gather(float2* __restrict__ out, const float2* __restrict__ canvas, const int2* __restrict__ roi,
+                         const int* __restrict__ gf, int m, int cols) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int j = blockIdx.y;
+  const int2 o = roi[gf[j]];
+  out[(long)j * mm + q] = canvas[(long)(o.x + q / m) * cols + o.y + q % m];
+}
+
+__global__ void k_dir(float2* __restrict__ out, const float2* __restrict__ chi, const float2* __restrict__ vp,
+                      const float2* __restrict__ pl, int m) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const long i = (long)blockIdx.y * mm + q;
+  out[i] = obj_dir(chi, vp, pl, i, q);
+}
+
+// LSQ sums per frame chunk (d_get_optimal_step_lsq, utilitiesKernels.cu:445-466), FP64:
+// [|dOP|^2, Re dOP conj dPO, Im dOP conj dPO, |dPO|^2, Re conj(dOP) chi, Re conj(dPO) chi].
+__global__ __launch_bounds__(NT) void k_lsq_partial(const float2* __restrict__ oup, const float2* __restrict__ vp,
+                                                    const float2* __restrict__ pum, const float2* __restrict__ op,
+                                                    const float2* __restrict__ chi, int m, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int mm = m * m, ppb = ppb_of(m);
+  const int j = blockIdx.y;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < ppb; t += NT) {
+    const int q = blockIdx.x * ppb + t;
+    const long i = (long)j * mm + q;
+    const float2 a = oup[i], b = vp[i], p = pum[q], o = op[i], x = chi[i];
+    const double opr = (double)a.x * b.x - (double)a.y * b.y, opi = (double)a.x * b.y + (double)a.y * b.x;
+    const double por = (double)p.x * o.x - (double)p.y * o.y, poi = (double)p.x * o.y + (double)p.y * o.x;
+    s[0] += opr * opr + opi * opi;
+    s[1] += opr * por + opi * poi;
+    s[2] += opi * por - opr * poi;
+    s[3] += por * por + poi * poi;
+    s[4] += opr * x.x + opi * x.y;
+    s[5] += por * x.x + poi * x.y;
+  }
+  for (int v = 0; v < 6; ++v) {
+    const double t = block_sum<NT>(s[v], red);
+    if (threadIdx.x == 0) part[((long)j * gridDim.x + blockIdx.x) * 6 + v] = t;
+  }
+}
+
+// Per frame: AA with the 0.1 x group-mean diagonal load, 2x2 solve in FP64, steps scaled by
+// beta * beta_LSQ (MLS.cpp:370-416); then the group means of the object and probe steps
+// (Sample::update_object with one scan id, Sample.cpp:277-331; Probe::update_probe,
+// Probe.cpp:1339-1364).  solve = 0: only the means of the current steps.
+__global__ __launch_bounds__(NT) void k_lsq_solve(const double* __restrict__ part, int ch, const int* __restrict__ gf,
+                                                  int G, double lam_mm, double c_obj, double c_probe, int solve,
+                                                  float2* __restrict__ beta_obj, float2* __restrict__ beta_probe,
+                                                  double* __restrict__ gscal) {
+  __shared__ double red[NT / 64];
+  auto tot = [&](int j, int v) {
+    double s = 0.0;
+    for (int c = 0; c < ch; ++c) s += part[((long)j * ch + c) * 6 + v];
+    return s;
+  };
+  if (solve) {
+    double a1 = 0.0, a4 = 0.0;
+    for (int j = threadIdx.x; j < G; j += NT) {
+      a1 += tot(j, 0) + lam_mm;
+      a4 += tot(j, 3) + lam_mm;
+    }
+    const double m1 = 0.1 * block_sum<NT>(a1, red) / G;
+    const double m4 = 0.1 * block_sum<NT>(a4, red) / G;
+    for (int j = threadIdx.x; j < G; j += NT) {
+      const double A11 = tot(j, 0) + lam_mm + m1, A22 = tot(j, 3) + lam_mm + m4;
+      const double a2r = tot(j, 1), a2i = tot(j, 2), b1 = tot(j, 4), b2 = tot(j, 5);
+      // A = [[A11, a2], [conj(a2), A22]], det real
+      const double det = A11 * A22 - (a2r * a2r + a2i * a2i);
+      const double x0r = (b1 * A22 - a2r * b2) / det, x0i = (-a2i * b2) / det;
+      const double x1r = (A11 * b2 - a2r * b1) / det, x1i = (a2i * b1) / det;
+      beta_obj[gf[j]] = c_of(c_obj * x0r, c_obj * x0i);
+      beta_probe[gf[j]] = c_of(c_probe * x1r, c_probe * x1i);
+    }
+    __syncthreads();
+  }
+  double s[4] = {0, 0, 0, 0};
+  for (int j = threadIdx.x; j < G; j += NT) {
+    const float2 bo = beta_obj[gf[j]], bp = beta_probe[gf[j]];
+    s[0] += bo.x;
+    s[1] += bo.y;
+    s[2] += bp.x;
+    s[3] += bp.y;
+  }
+  for (int v = 0; v < 4; ++v) {
+    const double t = block_sum<NT>(s[v], red);
+    if (threadIdx.x == 0) gscal[MLS_S_BO + v] = t / G;
+  }
+}
+
+// x += y * beta, beta from gscal (object: O += precond * mean step; probe: P_ll += pum * mean step).
+__global__ void k_axpy_beta(float2* __restrict__ x, const float2* __restrict__ y, long count,
+                            const double* __restrict__ beta) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] = cadd(x[i], cmul(y[i], c_of(beta[0], beta[1])));
+}
+
+// Gradient position solver spectra (Probe.cpp:956-1118): t = FFT(obj_proj) * 2 pi i;
+// x <- t f[col] / M^2 (d/dcol), y <- t f[row] / M^2 (d/drow).
+__global__ void k_deriv(float2* __restrict__ x, float2* __restrict__ y, int m) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const long i = (long)blockIdx.y * mm + q;
+  const float2 f = x[i];
+  const float2 t = make_float2(-f.y * kTwoPi, f.x * kTwoPi);
+  const float inv = 1.f / (float)mm;
+  x[i] = cscale(t, freq(q % m, m) * inv);
+  y[i] = cscale(t, freq(q / m, m) * inv);
+}
+
+// [sum |dX vp|^2, sum Re(chi conj(dX vp)), same for dY] per frame chunk.
+__global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ dx, const float2* __restrict__ dy,
+                                                    const float2* __restrict__ vp, const float2* __restrict__ chi, int m,
+                                                    double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int mm = m * m, ppb = ppb_of(m);
+  const int j = blockIdx.y;
+  double s[4] = {0, 0, 0, 0};
+  for (int t = threadIdx.x; t < ppb; t += NT) {
+    const long i = (long)j * mm + blockIdx.x * ppb + t;
+    const float2 v = vp[i], x = chi[i], a = dx[i], b = dy[i];
+    const double axr = (double)a.x * v.x - (double)a.y * v.y, axi = (double)a.x * v.y + (double)a.y * v.x;
+    const double ayr = (double)b.x * v.x - (double)b.y * v.y, ayi = (double)b.x * v.y + (double)b.y * v.x;
+    s[0] += axr * axr + axi * axi;
+    s[1] += x.x * axr + x.y * axi;
+    s[2] += ayr * ayr + ayi * ayi;
+    s[3] += x.x * ayr + x.y * ayi;
+  }
+  for (int v = 0; v < 4; ++v) {
+    const double t = block_sum<NT>(s[v], red);
+    if (threadIdx.x == 0) part[((long)j * gridDim.x + blockIdx.x) * 4 + v] = t;
+  }
+}
+
+// Step clipped to 0.2 px per axis; the column-derivative step is added to the row
+// coordinate and the row-derivative step to the column coordinate, as the reference does
+// (probe_positions[i].x += shift from dX, Probe.cpp:1022-1033).
+__global__ void k_pos_update(const double* __restrict__ part, int ch, const int* __restrict__ gf, int G,
+                             float2* __restrict__ pos) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= G) return;
+  double s[4] = {0, 0, 0, 0};
+  for (int c = 0; c < ch; ++c)
+    for (int v = 0; v < 4; ++v) s[v] += part[((long)j * ch + c) * 4 + v];
+  const double dx = s[1] / s[0], dy = s[3] / s[2];
+  const double sx = fmin(fabs(dx), 0.2) * (double)((0.0 < dx) - (dx < 0.0));
+  const double sy = fmin(fabs(dy), 0.2) * (double)((0.0 < dy) - (dy < 0.0));
+  const int f = gf[j];
+  pos[f] = make_float2((float)((double)pos[f].x + sx), (float)((double)pos[f].y + sy));
+}
+
+// Variable probe (Probe::update_variable_probe, Probe.cpp:1366-1464).
+// img_j = (p / max p)_roi_j (probe_update_j - probe_update_m); partial sums of Re(V conj img_j).
+__global__ __launch_bounds__(NT) void k_vp_resid(float2* __restrict__ img, const float2* __restrict__ pu,
+                                                 const float2* __restrict__ pum, const float* __restrict__ illum,
+                                                 const int2* __restrict__ roi, const int* __restrict__ gf, int m,
+                                                 int cols, const float2* __restrict__ vmode,
+                                                 const double* __restrict__ gscal, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int mm = m * m, ppb = ppb_of(m);
+  const int j = blockIdx.y;
+  const int2 o = roi[gf[j]];
+  const float wd = (float)(1.0 / gscal[MLS_S_PMAX]);
+  double s = 0.0;
+  for (int t = threadIdx.x; t < ppb; t += NT) {
+    const int q = blockIdx.x * ppb + t;
+    const long i = (long)j * mm + q;
+    const float w = illum[(long)(o.x + q / m) * cols + o.y + q % m] * wd;
+    const float2 d = cscale(csub(pu[i], pum[q]), w);
+    img[i] = d;
+    const float2 v = vmode[q];
+    s += (double)v.x * d.x + (double)v.y * d.y;
+  }
+  s = block_sum<NT>(s, red);
+  if (threadIdx.x == 0) part[(long)j * gridDim.x + blockIdx.x] = s;
+}
+
+// c_j = (mean_px Re(V conj img_j) + e1_j) / sum_g e1_g^2  -> gscal[COEF + j]
+__global__ __launch_bounds__(NT) void k_vp_coef(const double* __restrict__ part, int ch, const int* __restrict__ gf,
+                                                int G, int mm, const double* __restrict__ evo, int n,
+                                                double* __restrict__ gscal) {
+  __shared__ double red[NT / 64];
+  double e = 0.0;
+  for (int j = threadIdx.x; j < G; j += NT) e += evo[n + gf[j]] * evo[n + gf[j]];
+  e = block_sum<NT>(e, red);
+  for (int j = threadIdx.x; j < G; j += NT) {
+    double s = 0.0;
+    for (int c = 0; c < ch; ++c) s += part[(long)j * ch + c];
+    gscal[MLS_S_COEF + j] = (s / mm + evo[n + gf[j]]) / e;
+  }
+}
+
+// acc = (1/M^2) sum_j c_j img_j; partial sums of |acc|^2 per block.
+__global__ __launch_bounds__(NT) void k_vp_acc(float2* __restrict__ acc, const float2* __restrict__ img, int G, int mm,
+                                               const double* __restrict__ gscal, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int q = blockIdx.x * NT + threadIdx.x;
+  double s = 0.0;
+  if (q < mm) {
+    float2 a = make_float2(0.f, 0.f);
+    for (int j = 0; j < G; ++j) a = cadd(a, cscale(img[(long)j * mm + q], (float)gscal[MLS_S_COEF + j]));
+    a = cscale(a, (float)(1.0 / mm));
+    acc[q] = a;
+    s = (double)a.x * a.x + (double)a.y * a.y;
+  }
+  s = block_sum<NT>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// V += acc * relax_U / max(1, rms(acc)); partial sums of |V|^2.
+__global__ __launch_bounds__(NT) void k_vp_apply(float2* __restrict__ vmode, const float2* __restrict__ acc, int mm,
+                                                 const double* __restrict__ part_in, int nb, double relax_u,
+                                                 double* __restrict__ part_out) {
+  __shared__ double red[NT / 64];
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += part_in[i];
+  const float rms = sqrtf((float)(s / mm));
+  const float sc = (float)(relax_u / fmax(1.0, (double)rms));
+  const int q = blockIdx.x * NT + threadIdx.x;
+  double v2 = 0.0;
+  if (q < mm) {
+    const float2 v = cadd(vmode[q], cscale(acc[q], sc));
+    vmode[q] = v;
+    v2 = (double)v.x * v.x + (double)v.y * v.y;
+  }
+  v2 = block_sum<NT>(v2, red);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = v2;
+}
+
+__global__ void k_vp_norm(float2* __restrict__ vmode, int mm, const double* __restrict__ part, int nb) {
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += part[i];
+  const float rms = sqrtf((float)(s / mm));
+  const float inv = (float)(1.0 / (double)rms);
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < mm) vmode[q] = cscale(vmode[q], inv);
+}
+
+// [sum |op_j page|^2, sum Re(chi_j conj(op_j page))] per frame chunk (d_multiplyAbsConjuReal).
+__global__ __launch_bounds__(NT) void k_frame_dots(const float2* __restrict__ op, const float2* __restrict__ page,
+                                                   const float2* __restrict__ chi, int m, double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int mm = m * m, ppb = ppb_of(m);
+  const int j = blockIdx.y;
+  double d = 0.0, u = 0.0;
+  for (int t = threadIdx.x; t < ppb; t += NT) {
+    const int q = blockIdx.x * ppb + t;
+    const long i = (long)j * mm + q;
+    const float2 a = op[i], b = page[q], x = chi[i];
+    const double r = (double)a.x * b.x - (double)a.y * b.y, im = (double)a.x * b.y + (double)a.y * b.x;
+    d += r * r + im * im;
+    u += x.x * r + x.y * im;
+  }
+  d = block_sum<NT>(d, red);
+  u = block_sum<NT>(u, red);
+  if (threadIdx.x == 0) {
+    part[((long)j * gridDim.x + blockIdx.x) * 2] = d;
+    part[((long)j * gridDim.x + blockIdx.x) * 2 + 1] = u;
+  }
+}
+
+// Evolution updates: e1 += mean(u) / (mean(d) + 0.1 mean_g mean(d)) (variable probe);
+// e0 += 0.1 sum(u) / sum(d) (variable intensity, Probe.cpp:1496-1516).
+__global__ __launch_bounds__(NT) void k_evo_update(const double* __restrict__ part, int ch, const int* __restrict__ gf,
+                                                   int G, int mm, double* __restrict__ evo, int n, int which) {
+  __shared__ double red[NT / 64];
+  auto tot = [&](int j, int v) {
+    double s = 0.0;
+    for (int c = 0; c < ch; ++c) s += part[((long)j * ch + c) * 2 + v];
+    return s;
+  };
+  if (which == 1) {
+    double dm = 0.0;
+    for (int j = threadIdx.x; j < G; j += NT) dm += tot(j, 0) / mm;
+    const double m3 = block_sum<NT>(dm, red) / G;
+    for (int j = threadIdx.x; j < G; j += NT) evo[n + gf[j]] += (tot(j, 1) / mm) / (tot(j, 0) / mm + 0.1 * m3);
+  } else {
+    for (int j = threadIdx.x; j < G; j += NT) evo[gf[j]] += 0.1 * tot(j, 1) / tot(j, 0);
+  }
+}
+
+// Probe::ortho_modes (Probe.cpp:244-294): modified Gram-Schmidt in mode order with
+// proj = sum(P_i conj P_j) / sum |P_j|^2, then modes 1..K-1 take the orthogonalised modes in
+// order of decreasing power measured before each mode's own orthogonalisation (mode 0 stays).
+__global__ __launch_bounds__(1024) void k_ortho(float2* __restrict__ probe, float2* __restrict__ copy, int K, int mm) {
+  __shared__ double red[16];
+  __shared__ double pw[16];
+  __shared__ int order[16];
+  for (int i = 0; i < K; ++i) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < mm; q += 1024) s += cabs2(probe[(long)i * mm + q]);
+    s = block_sum<1024>(s, red);
+    if (threadIdx.x == 0) pw[i] = s;
+    for (int j = 0; j < i; ++j) {
+      double sr = 0.0, slr = 0.0, sli = 0.0;
+      for (int q = threadIdx.x; q < mm; q += 1024) {
+        const float2 a = probe[(long)i * mm + q], b = probe[(long)j * mm + q];
+        sr += (double)b.x * b.x + (double)b.y * b.y;
+        slr += (double)a.x * b.x + (double)a.y * b.y;
+        sli += (double)a.y * b.x - (double)a.x * b.y;
+      }
+      sr = block_sum<1024>(sr, red);
+      slr = block_sum<1024>(slr, red);
+      sli = block_sum<1024>(sli, red);
+      const float2 pr = c_of(slr / sr, sli / sr);
+      for (int q = threadIdx.x; q < mm; q += 1024) {
+        const long ii = (long)i * mm + q;
+        probe[ii] = csub(probe[ii], cmul(probe[(long)j * mm + q], pr));
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < K; ++i) order[i] = i;
+    for (int a = 1; a < K; ++a)  // insertion sort by (-power, index)
+      for (int b = a; b > 0 && pw[order[b]] > pw[order[b - 1]]; --b) {
+        const int t = order[b];
+        order[b] = order[b - 1];
+        order[b - 1] = t;
+      }
+  }
+  for (long q = threadIdx.x; q < (long)K * mm; q += 1024) copy[q] = probe[q];
+  __syncthreads();
+  for (int i = 1; i < K; ++i)
+    for (int q = threadIdx.x; q < mm; q += 1024) probe[(long)i * mm + q] = copy[(long)order[i] * mm + q];
+}
+
+inline dim3 g1(long count) { return dim3((unsigned)((count + NT - 1) / NT)); }
+
+}  // namespace
+
+int mls_chunks(int m) { return m * m / ppb_of(m); }
+
+#define LAUNCH(kernel, grid, block, ...) hipLaunchKernelGGL(kernel, grid, block, 0, s, __VA_ARGS__)
+#define CHK()                               \
+  do {                                      \
+    hipError_t e_ = hipGetLastError();      \
+    if (e_ != hipSuccess) return e_;        \
+  } while (0)
+#define TRY(x)                              \
+  do {                                      \
+    hipError_t e_ = (x);                    \
+    if (e_ != hipSuccess) return e_;        \
+  } while (0)
+
+hipError_t mls_roi(MlsDev& d, hipStream_t s) {
+  const float cx = (float)std::ceil(d.rows * 1.0 / 2 - d.m * 1.0 / 2);
+  const float cy = (float)std::ceil(d.cols * 1.0 / 2 - d.m * 1.0 / 2);
+  LAUNCH(k_roi, g1(d.n), dim3(NT), d.pos, d.n, d.m, d.rows, d.cols, cx, cy, d.roi, d.sub, d.flag);
+  return hipGetLastError();
+}
+
+// Canvas / array reduction finished on the host (initialisation only).
+static hipError_t host_reduce(MlsDev& d, const float* a, const float2* o, long count, int op, double* out,
+                              hipStream_t s) {
+  const int nb = (int)std::min<long>((count + NT - 1) / NT, 1024);
+  LAUNCH(k_reduce, dim3(nb), dim3(NT), a, o, count, op, d.part);
+  CHK();
+  std::vector<double> h(nb);
+  TRY(hipMemcpyAsync(h.data(), d.part, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  double r = 0.0;
+  for (int i = 0; i < nb; ++i) r = (op == 2) ? std::max(r, h[i]) : r + h[i];
+  *out = r;
+  return hipSuccess;
+}
+
+hipError_t mls_init(MlsDev& d, const double* frame_power, hipStream_t s) {
+  const int mm = d.m * d.m;
+  const long canvas = (long)d.rows * d.cols;
+  std::vector<double> evo(2 * (size_t)d.n);
+  TRY(hipMemcpyAsync(evo.data(), d.evo, evo.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < d.n; ++i) evo[i] = 1.0;
+  // variable mode to unit rms, its evolution scaled by the same norm (h_norm2: float result)
+  if (d.variable_probe_modes > 0) {
+    TRY(sum_abs2(d.vmode, mm, d.gscal + MLS_S_TMP, s));
+    double sv = 0.0;
+    TRY(hipMemcpyAsync(&sv, d.gscal + MLS_S_TMP, sizeof(double), hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    const float rv = sqrtf((float)(sv / mm));
+    const double inv = (float)(1.0 / (double)rv);
+    TRY(hipMemcpyAsync(d.gscal + MLS_S_TMP, &inv, sizeof(double), hipMemcpyHostToDevice, s));
+    LAUNCH(k_scale_c, g1(mm), dim3(NT), d.vmode, (long)mm, d.gscal + MLS_S_TMP, 0);
+    CHK();
+    for (int i = 0; i < d.n; ++i) evo[d.n + i] *= rv;
+  }
+  TRY(hipMemcpyAsync(d.evo, evo.data(), evo.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  // illumination = sum of |P0|^2 at every ROI
+  LAUNCH(k_illum, dim3((d.cols + 63) / 64, (d.rows + 3) / 4), dim3(NT), d.illum, d.probe, d.roi, d.n, d.m, d.rows,
+         d.cols, 0);
+  CHK();
+  // object normalised by the illumination-weighted rms
+  double si = 0.0, soi = 0.0;
+  TRY(host_reduce(d, d.illum, nullptr, canvas, 0, &si, s));
+  TRY(host_reduce(d, d.illum, d.obj, canvas, 1, &soi, s));
+  const float onorm = (float)std::sqrt(soi / si);
+  double f = (float)(1.0 / (double)onorm);
+  TRY(hipMemcpyAsync(d.gscal + MLS_S_TMP, &f, sizeof(double), hipMemcpyHostToDevice, s));
+  LAUNCH(k_scale_c, g1(canvas), dim3(NT), d.obj, canvas, d.gscal + MLS_S_TMP, 0);
+  CHK();
+  // probe power matched to the data: corr = sqrt(median_j(sum|P|^2 / sum I_j) M^2)
+  TRY(sum_abs2(d.probe, (long)d.k * mm, d.gscal + MLS_S_TMP + 1, s));
+  double pw = 0.0;
+  TRY(hipMemcpyAsync(&pw, d.gscal + MLS_S_TMP + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  const double meanillum = pw / mm;
+  std::vector<double> cv(d.n);
+  for (int i = 0; i < d.n; ++i) cv[i] = meanillum / (frame_power[i] / mm);
+  std::sort(cv.begin(), cv.end());
+  const double med = (d.n % 2 == 0) ? (cv[d.n / 2 - 1] + cv[d.n / 2]) / 2 : cv[d.n / 2];
+  const float corr = (float)std::sqrt(med * mm);
+  f = (float)(1.0 / (double)corr);
+  TRY(hipMemcpyAsync(d.gscal + MLS_S_TMP, &f, sizeof(double), hipMemcpyHostToDevice, s));
+  LAUNCH(k_scale_c, g1((long)d.k * mm), dim3(NT), d.probe, (long)d.k * mm, d.gscal + MLS_S_TMP, 0);
+  LAUNCH(k_scale_c, g1(mm), dim3(NT), d.vmode, (long)mm, d.gscal + MLS_S_TMP, 0);
+  CHK();
+  // illum /= corr^2 (float product), MAX_ILLUM = max illum
+  {
+    std::vector<float> h(canvas);
+    TRY(hipMemcpyAsync(h.data(), d.illum, canvas * sizeof(float), hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    const float c2 = corr * corr;
+    float mx = 0.f;
+    for (long i = 0; i < canvas; ++i) {
+      h[i] = h[i] / c2;
+      mx = std::max(mx, h[i]);
+    }
+    TRY(hipMemcpyAsync(d.illum, h.data(), canvas * sizeof(float), hipMemcpyHostToDevice, s));
+    TRY(hipStreamSynchronize(s));
+    d.max_illum = mx;
+  }
+  // LSQ steps start at 1 (IPhaser.cpp:343-349)
+  LAUNCH(k_fill_c, g1(d.n), dim3(NT), d.step_obj, (long)d.n, make_float2(1.f, 0.f));
+  LAUNCH(k_fill_c, g1(d.n), dim3(NT), d.step_probe, (long)d.n, make_float2(1.f, 0.f));
+  CHK();
+  return hipStreamSynchronize(s);
+}
+
+hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s) {
+  const int m = d.m, mm = m * m, K = d.k, ch = mls_chunks(m);
+  const long canvas = (long)d.rows * d.cols;
+  const dim3 tiles((d.cols + 63) / 64, (d.rows + 3) / 4);
+  const bool var_any = d.variable_probe || d.variable_intensity;
+  const int subpix = d.apply_subpix_shift ? 1 : 0;
+
+  // MLS.cpp:86-89
+  if (iter == 1 || iter >= d.pps) TRY(mls_roi(d, s));
+  // calc_object_norm over the precalculated box
+  {
+    const int x0 = d.box[0] - 1, y0 = d.box[2] - 1;
+    const int bx = d.box[1] - d.box[0] + 1, by = d.box[3] - d.box[2] + 1;
+    LAUNCH(k_box_partial, dim3(bx), dim3(NT), d.illum, d.obj, d.cols, x0, y0, by, d.part);
+    LAUNCH(k_objnorm, dim3(1), dim3(NT), d.part, bx, d.gscal);
+    LAUNCH(k_scale_c, g1((long)K * mm), dim3(NT), d.probe, (long)K * mm, d.gscal + MLS_S_ONORM, 0);
+    LAUNCH(k_scale_c, g1(mm), dim3(NT), d.vmode, (long)mm, d.gscal + MLS_S_ONORM, 0);
+    LAUNCH(k_scale_c, g1(canvas), dim3(NT), d.obj, canvas, d.gscal + MLS_S_ONORM, 1);
+    CHK();
+  }
+  LAUNCH(k_pos_diff, dim3(1), dim3(NT), d.pos, d.pos_o, d.n);
+  if (d.variable_probe_modes && iter > d.probe_reconstruct) {
+    TRY(sum_abs2(d.vmode, mm, d.gscal + MLS_S_TMP, s));
+    LAUNCH(k_remove_extra, dim3(1), dim3(NT), d.vmode, d.probe, mm, d.evo, d.n, d.gscal);
+  }
+  LAUNCH(k_illum, tiles, dim3(NT), d.illum, d.probe, d.roi, d.n, m, d.rows, d.cols, 1);
+  CHK();
+  const bool var_update = var_any && iter > d.probe_reconstruct + 1;
+  if (var_update && d.variable_probe) {  // max p is fixed for the rest of the iteration
+    const int nb = (int)std::min<long>((canvas + NT - 1) / NT, 1024);
+    LAUNCH(k_reduce, dim3(nb), dim3(NT), d.illum, nullptr, canvas, 2, d.part);
+    LAUNCH(k_finish_max, dim3(1), dim3(NT), d.part, nb, d.gscal + MLS_S_PMAX);
+    CHK();
+  }
+  const float fw = (float)((double)d.max_illum * 0.01);
+  const float fd = (float)((double)d.max_illum * d.delta_p);
+  const bool obj_rec_iter = iter >= d.object_reconstruct;
+  const bool probe_rec = iter >= d.probe_reconstruct;
+
+  for (int g = 0; g < n_groups; ++g) {
+    const int G = goffs[g + 1] - goffs[g];
+    const int* gf = d.gframes + goffs[g];
+    const long gm = (long)G * mm;
+    const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
+    // illumination probe, exit waves, Fourier transforms
+    if (subpix) {
+      LAUNCH(k_copy_c, g1(mm), dim3(NT), d.spec, d.probe, (long)mm);
+      LAUNCH(k_copy_c, g1(mm), dim3(NT), d.spec + mm, d.vmode, (long)mm);
+      CHK();
+      TRY(fft2_batch(d.spec, 2, m, false, d.tw, s));
+    }
+    LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
+    CHK();
+    if (subpix) TRY(fft2_batch(d.vp, G, m, true, d.tw, s));
+    LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, d.vp, d.obj, d.probe, d.roi, gf, G, K, m, d.cols, subpix);
+    CHK();
+    TRY(fft2_batch(d.psi, (long)K * G, m, false, d.tw, s));
+    LAUNCH(k_modulus, gch, dim3(NT), d.psi, d.amp, d.amp_scale, gf, G, K, m, d.part);
+    CHK();
+    TRY(fft2_batch(d.psi, (long)K * G, m, true, d.tw, s));
+    if (iter < 10) LAUNCH(k_ferr, g1(G), dim3(NT), d.part, ch, gf, G, mm, d.ferr);
+    if (!probe_rec) {  // probe_update / probe_update_m stay zero (MLS.cpp:146-153)
+      LAUNCH(k_fill_c, g1(gm), dim3(NT), d.pu, gm, make_float2(0.f, 0.f));
+      LAUNCH(k_fill_c, g1(mm), dim3(NT), d.pum, (long)mm, make_float2(0.f, 0.f));
+    }
+    CHK();
+    const float2* chi0 = d.psi;
+    const int ll_range = std::max(K, d.n_objects);
+    for (int ll = 1; ll <= ll_range; ++ll) {
+      const bool obj_rec = obj_rec_iter && (d.apply_multimodal_update || ll <= d.n_objects);
+      const int llp = std::min(K, ll);
+      const float2* chi = d.psi + (long)(ll - 1) * gm;
+      if (probe_rec) {
+        if (ll == 1) {
+          LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
+          CHK();
+          if (subpix) {
+            TRY(fft2_batch(d.pu, G, m, false, d.tw, s));
+            LAUNCH(k_ramp, gpix, dim3(NT), d.pu, d.sub, gf, m, -1.f);
+            CHK();
+            TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
+          }
+          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
+        } else {
+          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, chi, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 1);
+        }
+        CHK();
+      }
+      const float2* pl = llp == 1 ? nullptr : d.probe + (long)(llp - 1) * mm;
+      if (obj_rec) {
+        const int scatter = d.delta_p != 0 ? 1 : 0;
+        LAUNCH(k_precond, tiles, dim3(NT), d.precond, d.illum, chi, d.vp, pl, d.roi, gf, G, m, d.rows, d.cols, fd,
+               scatter);
+        CHK();
+        if (scatter && d.beta_lsq > 0) LAUNCH(k_gather, gpix, dim3(NT), d.tmp2, d.precond, d.roi, gf, m, d.cols);
+        else LAUNCH(k_dir, gpix, dim3(NT), d.tmp2, chi, d.vp, pl, m);
+        CHK();
+      }
+      const bool lsq = d.beta_lsq > 0 && ll == 1 && obj_rec && probe_rec;
+      if (lsq) {
+        LAUNCH(k_lsq_partial, gch, dim3(NT), d.tmp2, d.vp, d.pum, d.op, chi0, m, d.part);
+        CHK();
+      }
+      const double lam_mm = (double)(float)(1.1920928955078125e-07 / mm) * mm;
+      if (obj_rec || (probe_rec && ll <= K)) {
+        LAUNCH(k_lsq_solve, dim3(1), dim3(NT), d.part, ch, gf, G, lam_mm, d.beta_object * d.beta_lsq,
+               d.beta_probe * d.beta_lsq, lsq ? 1 : 0, d.step_obj, d.step_probe, d.gscal);
+        CHK();
+      }
+      if (obj_rec) LAUNCH(k_axpy_beta, g1(canvas), dim3(NT), d.obj, d.precond, canvas, d.gscal + MLS_S_BO);
+      if (probe_rec && ll <= K)
+        LAUNCH(k_axpy_beta, g1(mm), dim3(NT), d.probe + (long)(ll - 1) * mm, d.pum, (long)mm, d.gscal + MLS_S_BP);
+      CHK();
+      if (iter >= d.pps && ll == 1) {
+        LAUNCH(k_copy_c, g1(gm), dim3(NT), d.tmp, d.op, gm);
+        CHK();
+        TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
+        LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
+        CHK();
+        TRY(fft2_batch(d.tmp, G, m, true, d.tw, s));
+        TRY(fft2_batch(d.tmp2, G, m, true, d.tw, s));
+        LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
+        LAUNCH(k_pos_update, g1(G), dim3(NT), d.part, ch, gf, G, d.pos);
+        CHK();
+      }
+      if (var_update && ll == 1) {
+        if (d.variable_probe) {
+          const int nb = (mm + NT - 1) / NT;
+          double* p2 = d.part + (long)G * ch;  // |acc|^2 partials
+          double* p3 = p2 + nb;                // |V|^2 partials
+          LAUNCH(k_vp_resid, gch, dim3(NT), d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m, d.cols, d.vmode, d.gscal,
+                 d.part);
+          LAUNCH(k_vp_coef, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, d.gscal);
+          LAUNCH(k_vp_acc, dim3(nb), dim3(NT), d.acc, d.tmp, G, mm, d.gscal, p2);
+          LAUNCH(k_vp_apply, dim3(nb), dim3(NT), d.vmode, d.acc, mm, p2, nb, (double)G / d.n, p3);
+          LAUNCH(k_vp_norm, dim3(nb), dim3(NT), d.vmode, mm, p3, nb);
+          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.vmode, chi0, m, d.part);
+          LAUNCH(k_evo_update, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, 1);
+          CHK();
+        }
+        if (d.variable_intensity) {
+          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.probe, chi0, m, d.part);
+          LAUNCH(k_evo_update, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, 0);
+          CHK();
+        }
+      }
+    }
+  }
+  if (K > d.n_rec) {
+    LAUNCH(k_ortho, dim3(1), dim3(1024), d.probe, d.tmp, K, mm);
+    CHK();
+  }
+  return hipSuccess;
+}
+
+#undef LAUNCH
+#undef CHK
+#undef TRY
+
+}  // namespace ptyk

@@ -1,0 +1,131 @@
+"""MLs on the GPU through the C ABI against the FP64 oracle (oracle/oracle_mls.cpp).
+
+Each iteration check starts the oracle from the GPU's own state (object, probe modes, variable
+mode, evolution, illumination, positions, ROIs, steps), runs one iteration on both with the same
+groups, and compares: object and probe within 1e-5 relative (L2), ROI origins bit-exact."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from problems import make_mls_problem, mls_groups
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine_for(P, amp, **cfg_kw):
+    from ptychopy_amd import _engine
+    st = P["state"]
+    k, m, _ = P["probe0"].shape
+    e = _engine.Engine(P["n"], m, P["shape"], n_modes=k)
+    e.set_frames(amp)
+    e.set_object(P["obj0"])
+    e.set_probe(P["probe0"])
+    cfg = _engine.mls_config(P["box"], **cfg_kw)
+    e.mls_begin(cfg, P["pos"], P["vmode0"], P["evo1_0"], P["frame_power"])
+    return e
+
+
+def _oracle_from_engine(e, orc, amp):
+    g = e.mls_get_state()
+    o = orc.MlsState(e.get_object(), e.get_probe(), g["variable_mode"], g["evolution"][1], g["positions_start"],
+                     amp)
+    o.pos = g["positions"].copy()
+    o.evo = g["evolution"].astype(np.float64).copy()
+    o.illum = g["illumination"].astype(np.float64)
+    o.max_illum = float(g["max_illumination"])
+    o.roi = g["roi"].copy()
+    o.sub = g["subpixel"].copy()
+    o.beta_obj = g["beta_object"].astype(np.complex128)
+    o.beta_probe = g["beta_probe"].astype(np.complex128)
+    return o
+
+
+def _problem(k=2, m=32, n_side=5, seed=0):
+    # evolution draws all non-zero: a group whose draws are all zero divides by zero in the
+    # variable-probe update (Probe.cpp:1387-1441), in the reference as here
+    P = make_mls_problem(m=m, n_side=n_side, k=k, seed=seed, phase_object=False, photons=2e5, zero_evo01=False)
+    return P
+
+
+def test_mls_begin_matches_oracle_init(gpu, orc):
+    P = _problem(k=3)
+    e = _engine_for(P, P["state"].amp)
+    amp = e.get_frames().astype(np.float64)
+    o = orc.MlsState(P["obj0"], P["probe0"], P["vmode0"], P["evo1_0"], P["pos"], amp)
+    o.init(P["frame_power"])
+    g = e.mls_get_state()
+    assert np.array_equal(g["roi"], o.roi)
+    assert np.array_equal(g["subpixel"], o.sub)
+    assert rel_err(e.get_object(), o.obj) < 1e-6
+    assert rel_err(e.get_probe(), o.probe) < 1e-6
+    assert rel_err(g["variable_mode"], o.vmode) < 1e-6
+    assert np.allclose(g["evolution"], o.evo, rtol=1e-6, atol=0)
+    assert rel_err(g["illumination"], o.illum) < 1e-6
+    assert abs(g["max_illumination"] / o.max_illum - 1) < 1e-6
+    assert np.all(g["beta_object"] == 1) and np.all(g["beta_probe"] == 1)
+
+
+CONFIGS = [
+    dict(k=2, cfg=dict(probe_pos_search=3)),
+    dict(k=1, cfg=dict(probe_pos_search=2, variable_probe=0)),
+    dict(k=3, cfg=dict(apply_subpix_shift=0, probe_pos_search=99)),
+    dict(k=2, cfg=dict(variable_probe=0, variable_intensity=0, variable_probe_modes=0, beta_lsq=0.0)),
+    dict(k=2, cfg=dict(apply_multimodal_update=1, probe_pos_search=4)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CONFIGS)))
+def test_mls_iterations_match_oracle(gpu, orc, case):
+    c = CONFIGS[case]
+    P = _problem(k=c["k"], seed=case)
+    e = _engine_for(P, P["state"].amp, **c["cfg"])
+    amp = e.get_frames().astype(np.float64)
+    okw = {("pps" if key == "probe_pos_search" else key): v for key, v in c["cfg"].items()}
+    rng = np.random.default_rng(100 + case)
+    for it in range(1, 5):
+        groups = mls_groups(P["n"], 4, rng)  # six groups of 4 and one of 1
+        o = _oracle_from_engine(e, orc, amp)
+        o.iteration(orc.mls_params(P["box"], iter=it, **okw), groups)
+        ferr = e.mls_iteration(it, groups)
+        g = e.mls_get_state()
+        tag = f"case {case} iter {it}"
+        assert np.array_equal(g["roi"], o.roi), tag
+        assert rel_err(e.get_object(), o.obj) < 1e-5, tag
+        assert rel_err(e.get_probe(), o.probe) < 1e-5, tag
+        assert rel_err(g["variable_mode"], o.vmode) < 1e-4, tag
+        # vectors compared by norm: single evolution/step entries are differences of O(1) terms
+        assert rel_err(g["evolution"][0], o.evo[0]) < 1e-5, tag
+        assert rel_err(g["evolution"][1], o.evo[1]) < 1e-4, tag
+        assert np.max(np.abs(g["positions"] - o.pos)) < 1e-4, tag
+        assert rel_err(g["beta_object"], o.beta_obj) < 1e-4, tag
+        assert rel_err(g["beta_probe"], o.beta_probe) < 1e-4, tag
+        if it < 10:
+            assert rel_err(ferr, o.ferr) < 1e-5, tag
+
+
+def test_mls_deterministic_and_converging(gpu):
+    P = _problem(k=2, seed=7)
+    runs = []
+    for _ in range(2):
+        e = _engine_for(P, P["state"].amp, probe_pos_search=5)
+        rng = np.random.default_rng(3)
+        errs = []
+        for it in range(1, 10):
+            errs.append(float(np.mean(e.mls_iteration(it, mls_groups(P["n"], 4, rng)))))
+        runs.append((e.get_object(), e.get_probe(), errs))
+        e.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    errs = runs[0][2]
+    assert errs[-1] < errs[0]
+
+
+def test_mls_rejects_bad_groups(gpu):
+    from ptychopy_amd import _engine
+    P = _problem(k=1)
+    e = _engine_for(P, P["state"].amp)
+    with pytest.raises(_engine.PtykError):
+        e.mls_iteration(1, [np.array([0, 0])])
+    with pytest.raises(_engine.PtykError):
+        e.mls_iteration(1, [np.array([P["n"]])])
+    with pytest.raises(_engine.PtykError):
+        e.mls_iteration(0, [np.array([0])])
