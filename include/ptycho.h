@@ -37,6 +37,21 @@ int ptycho_mesh_list(const ptycho_mesh_params* p, const double* metres, int64_t 
                      int32_t* canvas_out, float* extent_out);
 /* Spiral of nx*ny points (SpiralScanMesh::generateMesh, SpiralScanMesh.cpp:53-78). */
 int ptycho_mesh_spiral(const ptycho_mesh_params* p, float* pos_out, int32_t* canvas_out, float* extent_out);
+/* Cartesian positions before any shift, nx*ny x (row, col) pixels (d_generateMesh<1>). */
+int ptycho_mesh_cartesian_raw(const ptycho_mesh_params* p, float* raw_out);
+
+/* MLs object canvas and normalisation box for unshifted positions (CartesianScanMesh::generateMeshML,
+ * CartesianScanMesh.cpp:94-177; IPtychoScanMesh::precalculate_ROI, IPtychoScanMesh.cpp:179-184):
+ * extent = {min_x, min_y, max_x, max_y} over the scan buffer (padding included), canvas_out[2] =
+ * ceil(1.2 x span of the rounded ROI bounds), box_out[4] = 1-based inclusive {row0, row1, col0, col1}. */
+int ptycho_mls_canvas(const float* pos, int64_t n, const float* extent, int32_t probe_size, int32_t* canvas_out,
+                      int32_t* box_out);
+/* MLs frame groups (IPtychoScanMesh::get_nonoverlapping_indices, IPtychoScanMesh.cpp:826-913):
+ * min(niter, 30) seeded random partitions of n frames into groups of max(1, n/100), largest first.
+ * frames_out: partitions x n, offsets_out: partitions x (groups + 1); pass NULL to get the counts. */
+int ptycho_mls_partitions(int64_t n, int32_t niter, uint64_t seed, int32_t* frames_out, int32_t* offsets_out,
+                          int32_t* n_groups, int32_t* n_partitions);
+
 /* Complex CSV ("a+bj", ", "-separated rows) or raw binary, as Cuda2DArray::save / load
  * (Cuda2DArray.cpp:46-108, Cuda2DArray.hpp:439-464). */
 int ptycho_save_complex_csv(const char* path, const float* c64, int32_t rows, int32_t cols, int32_t binary);
@@ -81,6 +96,9 @@ int ptycho_phaser_get_probe(ptycho_phaser* h, float* out);                      
 int64_t ptycho_phaser_num_positions(ptycho_phaser* h);
 int ptycho_phaser_get_positions(ptycho_phaser* h, float* out, int64_t n);        /* pixels, canvas frame */
 int ptycho_phaser_get_errors(ptycho_phaser* h, double* out, int32_t n);          /* IPhaser::getPhaseErrors */
+/* MLs per-frame Fourier errors of iteration iter (1-based; IPhaser::m_fourierErrors, filled for
+ * iter < 10, MLS.cpp:234-237); n frames. */
+int ptycho_phaser_get_fourier_errors(ptycho_phaser* h, uint32_t iter, float* out, int64_t n);
 double ptycho_phaser_phase_seconds(ptycho_phaser* h);
 /* The engine handle (ptyk.h) the driver runs on; valid after ptycho_phaser_init. */
 void* ptycho_phaser_engine(ptycho_phaser* h);

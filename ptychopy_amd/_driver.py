@@ -24,7 +24,8 @@ DRIVER_EXPORTS = (
     "ptycho_phaser_phaseinit", "ptycho_phaser_phasestep", "ptycho_phaser_phasepost", "ptycho_phaser_write_results",
     "ptycho_phaser_object_shape", "ptycho_phaser_get_object", "ptycho_phaser_probe_shape", "ptycho_phaser_get_probe",
     "ptycho_phaser_num_positions", "ptycho_phaser_get_positions", "ptycho_phaser_get_errors",
-    "ptycho_phaser_phase_seconds", "ptycho_phaser_engine",
+    "ptycho_phaser_phase_seconds", "ptycho_phaser_engine", "ptycho_mesh_cartesian_raw", "ptycho_mls_canvas",
+    "ptycho_mls_partitions", "ptycho_phaser_get_fourier_errors",
 )
 
 _bound = False
@@ -43,6 +44,10 @@ def lib():
         L.ptycho_save_complex_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
         L.ptycho_load_complex_csv.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32)]
+        L.ptycho_mesh_cartesian_raw.argtypes = [C.POINTER(MeshParams), C.c_void_p]
+        L.ptycho_mls_canvas.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.ptycho_mls_partitions.argtypes = [C.c_int64, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         _bound = True
     return L
 
@@ -109,3 +114,35 @@ def load_complex_csv(path: str) -> np.ndarray:
     if lib().ptycho_load_complex_csv(str(path).encode(), out.ctypes.data, out.size, C.byref(r), C.byref(c)) != 0:
         raise IOError(path)
     return out
+
+
+def cartesian_mesh_raw(nx: int, ny: int, step_x: float, step_y: float, dx_s: float, probe_size: int, **kw):
+    """Unshifted Cartesian positions (nx*ny, 2) float32 in pixels."""
+    p = _params(nx, ny, step_x, step_y, dx_s, probe_size, **kw)
+    out = np.empty((nx * ny, 2), np.float32)
+    if lib().ptycho_mesh_cartesian_raw(C.byref(p), out.ctypes.data) != 0:
+        raise ValueError("bad mesh parameters")
+    return out
+
+
+def mls_canvas(pos, extent, probe_size: int):
+    """(canvas (rows, cols), box (row0, row1, col0, col1)) of the MLs object for unshifted positions."""
+    ps = np.ascontiguousarray(pos, dtype=np.float32)
+    ex = np.ascontiguousarray(extent, dtype=np.float32)
+    canvas = np.empty(2, np.int32)
+    box = np.empty(4, np.int32)
+    if lib().ptycho_mls_canvas(ps.ctypes.data, ps.shape[0], ex.ctypes.data, probe_size, canvas.ctypes.data,
+                               box.ctypes.data) != 0:
+        raise ValueError("bad MLs canvas arguments")
+    return (int(canvas[0]), int(canvas[1])), tuple(int(b) for b in box)
+
+
+def mls_partitions(n: int, niter: int = 100, seed: int = 0):
+    """List of partitions; each a list of int32 frame-index arrays (largest group first)."""
+    g, p = C.c_int32(), C.c_int32()
+    if lib().ptycho_mls_partitions(n, niter, seed, None, None, C.byref(g), C.byref(p)) != 0:
+        raise ValueError("bad partition arguments")
+    fr = np.empty((p.value, n), np.int32)
+    of = np.empty((p.value, g.value + 1), np.int32)
+    lib().ptycho_mls_partitions(n, niter, seed, fr.ctypes.data, of.ctypes.data, C.byref(g), C.byref(p))
+    return [[fr[i, of[i, j]:of[i, j + 1]] for j in range(g.value)] for i in range(p.value)]

@@ -1,4 +1,5 @@
 // C ABI of the host driver (include/ptycho.h): an IPhaser behind an opaque handle.
+#include <algorithm>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -158,6 +159,19 @@ int ptycho_phaser_get_errors(ptycho_phaser* h, double* out, int32_t n) {
   if (!h || !out) return -1;
   for (int32_t i = 0; i < n; ++i) out[i] = i < (int32_t)h->ph.errors.size() ? h->ph.errors[i] : 0.0;
   return 0;
+}
+
+int ptycho_phaser_get_fourier_errors(ptycho_phaser* h, uint32_t iter, float* out, int64_t n) {
+  if (!h || !out || iter == 0) return -1;
+  for (auto& pm : h->ph.methods) {
+    auto* ml = dynamic_cast<ptycho::MLSMethod*>(pm.method.get());
+    if (!ml) continue;
+    if (iter > ml->fourierErrors.size() || (int64_t)ml->fourierErrors[iter - 1].size() != n) return -1;
+    std::copy(ml->fourierErrors[iter - 1].begin(), ml->fourierErrors[iter - 1].end(), out);
+    return 0;
+  }
+  h->ph.err = "no MLs method";
+  return -1;
 }
 
 void* ptycho_phaser_engine(ptycho_phaser* h) { return h ? (void*)h->ph.eng.ctx : nullptr; }

@@ -8,6 +8,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -203,9 +204,19 @@ void ScanMesh::finish(const std::vector<F2>& raw, unsigned rows, unsigned cols, 
   for (unsigned i = 0; i < rows; ++i)
     for (unsigned j = 0; j < cols; ++j) {
       const F2 v = raw[(size_t)i * pitch + j];
-      positions.push_back(flip ? F2{v.y - mn.y, v.x - mn.x} : F2{v.x - mn.x, v.y - mn.y});
+      if (ml)
+        positions.push_back(v);
+      else
+        positions.push_back(flip ? F2{v.y - mn.y, v.x - mn.x} : F2{v.x - mn.x, v.y - mn.y});
       indices.push_back(i * cols + j);
     }
+  if (ml) {
+    const float ext[4] = {mn.x, mn.y, mx.x, mx.y};
+    int32_t canvas[2];
+    ptycho_mls_canvas(&positions[0].x, (int64_t)positions.size(), ext, (int32_t)probe, canvas, box);
+    canvas_x = canvas[0];
+    canvas_y = canvas[1];
+  }
 }
 
 bool CartesianScanMesh::generate(const Parameters& prm, std::string* err) {
@@ -224,6 +235,22 @@ bool CartesianScanMesh::generate(const Parameters& prm, std::string* err) {
   mp.probe_size = (int32_t)prm.r.desiredShape;
   mp.seed = prm.r.seed;
   const size_t n = (size_t)mp.nx * mp.ny;
+  if (ml) {
+    // generateMeshML: the same mesh kernel (with dx = dx_s), extents over the pitched buffer
+    // (rows padded with zeros to a multiple of 32), positions kept unshifted
+    std::vector<float> raw(2 * n);
+    if (ptycho_mesh_cartesian_raw(&mp, raw.data()) != 0) {
+      if (err) *err = "bad Cartesian mesh parameters";
+      return false;
+    }
+    const unsigned pitch = ((unsigned)mp.ny + 31u) / 32u * 32u;
+    std::vector<F2> pitched((size_t)mp.nx * pitch, F2{0.f, 0.f});
+    for (int i = 0; i < mp.nx; ++i)
+      for (int j = 0; j < mp.ny; ++j)
+        pitched[(size_t)i * pitch + j] = F2{raw[2 * ((size_t)i * mp.ny + j)], raw[2 * ((size_t)i * mp.ny + j) + 1]};
+    finish(pitched, (unsigned)mp.nx, (unsigned)mp.ny, pitch, prm.r.desiredShape, prm.r.flipScanAxis);
+    return true;
+  }
   std::vector<float> pos(2 * n);
   int32_t canvas[2];
   float ext[4];
@@ -337,6 +364,40 @@ int DMMethod::iteration(Engine& eng, const ScanMesh&, bool phaseConstraint, bool
   return ptyk_dm_iteration(eng.ctx, flags);
 }
 
+void MLSMethod::initMem(const ScanMesh& mesh, int) {
+  n_ = (int64_t)mesh.positions.size();
+  ptycho_mls_partitions(n_, niter_, seed_ + 101, nullptr, nullptr, &groups_, &parts_);
+  frames_.assign((size_t)parts_ * n_, 0);
+  offsets_.assign((size_t)parts_ * (groups_ + 1), 0);
+  ptycho_mls_partitions(n_, niter_, seed_ + 101, frames_.data(), offsets_.data(), &groups_, &parts_);
+  fourierErrors.clear();
+}
+
+int MLSMethod::iteration(Engine& eng, const ScanMesh&, bool, bool, bool, unsigned iter, bool, double* err) {
+  if (parts_ == 0) return PTYK_ESTATE;
+  // randNoIndex = rand_gen_no(1, miniter) - 1 and random_shuffle of the group order (MLS.cpp:106-113)
+  const int32_t part = (int32_t)(rng_() % (uint64_t)parts_);
+  std::vector<int32_t> order(groups_);
+  for (int32_t g = 0; g < groups_; ++g) order[g] = g;
+  for (int32_t i = groups_ - 1; i > 0; --i) std::swap(order[i], order[rng_() % (uint64_t)(i + 1)]);
+  const int32_t* fr = frames_.data() + (size_t)part * n_;
+  const int32_t* of = offsets_.data() + (size_t)part * (groups_ + 1);
+  lastGroups.clear();
+  lastOffsets.assign(1, 0);
+  for (int32_t g : order) {
+    lastGroups.insert(lastGroups.end(), fr + of[g], fr + of[g + 1]);
+    lastOffsets.push_back((int32_t)lastGroups.size());
+  }
+  std::vector<float> fe((size_t)n_, 0.f);
+  const int rc = ptyk_mls_iteration(eng.ctx, (int32_t)iter, lastGroups.data(), lastOffsets.data(), groups_, fe.data());
+  if (iter >= 1) {
+    if (fourierErrors.size() < iter) fourierErrors.resize(iter);
+    fourierErrors[iter - 1] = fe;
+  }
+  if (err) *err = 1.0;  // MLS::iteration returns 1 (MLS.cpp:489)
+  return rc;
+}
+
 // ---------------------------------------------------------------- IPhaser
 bool IPhaser::simulate_frames(std::vector<float>* amp) {
   // Diffractions::simulate (Diffractions.cpp:80-116) over a seeded random-phase object of the
@@ -360,21 +421,24 @@ bool IPhaser::simulate_frames(std::vector<float>* amp) {
       probe[2 * (i * m + j)] = (float)std::fabs(v * std::exp(-r * r / (w * w)));
     }
   std::vector<float> pos(2 * (size_t)n);
+  // MLs positions are centred: simulate at the MLs ROI frame, round(pos + ceil(Np/2 - M/2)) - 1
+  // before rounding (the reference's simulateMLs shifts by the minima instead, a constant offset)
+  const float cx = mesh->ml ? (float)std::ceil(eng.rows * 1.0 / 2 - m * 1.0 / 2) - 1.f : 0.f;
+  const float cy = mesh->ml ? (float)std::ceil(eng.cols * 1.0 / 2 - m * 1.0 / 2) - 1.f : 0.f;
   for (int q = 0; q < n; ++q) {
-    pos[2 * q] = mesh->positions[q].x;
-    pos[2 * q + 1] = mesh->positions[q].y;
+    pos[2 * q] = mesh->positions[q].x + cx;
+    pos[2 * q + 1] = mesh->positions[q].y + cy;
   }
   amp->resize((size_t)n * m * m);
-  return ptyk_simulate(eng.ctx, obj.data(), eng.rows, eng.cols, probe.data(), pos.data(), n, 0, amp->data()) == 0;
+  // MLs consumes intensities (its simulator's output is square-rooted again in prePhase,
+  // Diffractions.cpp:150 with IPhaser.cpp:336-337; here the intensities are simulated directly)
+  return ptyk_simulate(eng.ctx, obj.data(), eng.rows, eng.cols, probe.data(), pos.data(), n, mesh->ml ? 1 : 0,
+                       amp->data()) == 0;
 }
 
 bool IPhaser::init() {
   prm.finalize();
   const bool mls = prm.r.algorithm == "MLs";
-  if (mls) {
-    err = "the MLs method is not available in this build";
-    return false;
-  }
   // initScanMesh (IPhaser.cpp:128-171)
   if (!positions_m.empty()) {
     auto* lm = new ListScanMesh();
@@ -404,6 +468,7 @@ bool IPhaser::init() {
   } else {
     mesh.reset(new CartesianScanMesh());
   }
+  mesh->ml = mls;  // generateMeshML + get_nonoverlapping_indices + precalculate_ROI for MLs
   if (!mesh->generate(prm, &err)) return false;
   const int n = (int)mesh->positions.size(), m = (int)prm.r.desiredShape;
   if (!prm.r.simulated && diffraction.empty()) {
@@ -437,14 +502,16 @@ bool IPhaser::init() {
   eng.k = cfg.n_modes;
   eng.rows = cfg.obj_rows;
   eng.cols = cfg.obj_cols;
-  std::vector<float> pos(2 * (size_t)n);
-  for (int q = 0; q < n; ++q) {
-    pos[2 * q] = mesh->positions[q].x;
-    pos[2 * q + 1] = mesh->positions[q].y;
-  }
-  if (ptyk_set_positions(eng.ctx, pos.data()) != PTYK_OK) {
-    err = ptyk_last_error(eng.ctx);
-    return false;
+  if (!mls) {  // MLs positions stay centred; their ROIs are set up by ptyk_mls_begin
+    std::vector<float> pos(2 * (size_t)n);
+    for (int q = 0; q < n; ++q) {
+      pos[2 * q] = mesh->positions[q].x;
+      pos[2 * q + 1] = mesh->positions[q].y;
+    }
+    if (ptyk_set_positions(eng.ctx, pos.data()) != PTYK_OK) {
+      err = ptyk_last_error(eng.ctx);
+      return false;
+    }
   }
   std::vector<float> amp;
   if (!diffraction.empty()) {
@@ -452,6 +519,29 @@ bool IPhaser::init() {
   } else if (!simulate_frames(&amp)) {
     err = ptyk_last_error(eng.ctx);
     return false;
+  }
+  if (mls) {
+    // MLs takes intensities: per-frame sums (fillSquaredSums) then the square root
+    // (squaredRoot), IPhaser.cpp:336-337
+    const size_t mm = (size_t)m * m;
+    frame_power_.assign(n, 0.0);
+    for (int q = 0; q < n; ++q) {
+      double sp = 0.0;
+      for (size_t i = 0; i < mm; ++i) {
+        float& v = amp[(size_t)q * mm + i];
+        if (!(v >= 0.f)) {
+          err = "MLs diffraction intensities must be >= 0";
+          return false;
+        }
+        sp += v;
+        v = std::sqrt(v);
+      }
+      if (!(sp > 0.0)) {
+        err = "MLs diffraction frame " + std::to_string(q) + " is all zero";
+        return false;
+      }
+      frame_power_[q] = sp;
+    }
   }
   if (ptyk_set_frames_f32(eng.ctx, amp.data()) != PTYK_OK) {
     err = ptyk_last_error(eng.ctx);
@@ -465,6 +555,9 @@ bool IPhaser::addPhasingMethod(const std::string& name, unsigned iterations) {
     methods.push_back(CXPhasing{name, std::unique_ptr<IPhasingMethod>(new EPIE(prm.r.seed)), iterations});
   else if (name == "DM")
     methods.push_back(CXPhasing{name, std::unique_ptr<IPhasingMethod>(new DMMethod()), iterations});
+  else if (name == "MLs")
+    methods.push_back(
+        CXPhasing{name, std::unique_ptr<IPhasingMethod>(new MLSMethod(prm.r.seed, prm.r.Niter)), iterations});
   else {
     err = "Reconstruction algorithm [" + name + "] not implemented";
     return false;
@@ -473,11 +566,12 @@ bool IPhaser::addPhasingMethod(const std::string& name, unsigned iterations) {
 }
 
 bool IPhaser::prePhase() {
-  // IPhaser::prePhase, non-MLs branch (IPhaser.cpp:296-333)
   if (!eng.ctx) {
     err = "init() must be called before phase()";
     return false;
   }
+  if (mesh->ml) return mls_prephase();
+  // IPhaser::prePhase, non-MLs branch (IPhaser.cpp:296-333)
   const int m = eng.m;
   if (!prm.r.probeGuess.empty()) {
     std::vector<float> g;
@@ -533,6 +627,288 @@ bool IPhaser::prePhase() {
       err = ptyk_last_error(eng.ctx);
       return false;
     }
+  }
+  errors.assign(prm.r.iterations, 0.0);
+  for (auto& pm : methods) pm.method->initMem(*mesh, m);
+  return true;
+}
+
+namespace {
+// Standard normal draws (Box-Muller over the seeded uniform stream); the reference's
+// normalRandom is time-seeded.
+struct Normal {
+  std::mt19937_64 g;
+  bool have = false;
+  double spare = 0.0;
+  explicit Normal(uint64_t seed) : g(seed) {}
+  double operator()() {
+    if (have) {
+      have = false;
+      return spare;
+    }
+    const double u1 = (double)uniform_open0(g), u2 = (double)uniform_open0(g);
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    spare = r * std::sin(2.0 * M_PI * u2);
+    have = true;
+    return r * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+// Eigen-decomposition of a Hermitian k x k matrix by cyclic complex Jacobi rotations (FP64):
+// a (row-major, destroyed) -> eigenvalues w, eigenvectors as the columns of v.
+void hermitian_eig(std::vector<std::complex<double>>& a, int k, std::vector<double>* w,
+                   std::vector<std::complex<double>>* v) {
+  typedef std::complex<double> cd;
+  v->assign((size_t)k * k, cd(0));
+  for (int i = 0; i < k; ++i) (*v)[(size_t)i * k + i] = 1.0;
+  auto A = [&](int r, int c) -> cd& { return a[(size_t)r * k + c]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = 0; q < k; ++q) (p == q ? dia : off) += std::norm(A(p, q));
+    if (off <= 1e-30 * (dia > 0 ? dia : 1.0)) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const cd g = A(p, q);
+        if (std::abs(g) == 0.0) continue;
+        // phase column/row q so that A(p, q) becomes |g|, then a real rotation of (p, q)
+        const cd e = std::conj(g) / std::abs(g);
+        for (int r = 0; r < k; ++r) A(r, q) *= e;
+        for (int c = 0; c < k; ++c) A(q, c) *= std::conj(e);
+        for (int r = 0; r < k; ++r) (*v)[(size_t)r * k + q] *= e;
+        const double aa = A(p, p).real(), bb = A(q, q).real(), rr = A(p, q).real();
+        const double th = 0.5 * std::atan2(-2.0 * rr, bb - aa);
+        const double c = std::cos(th), s = std::sin(th);
+        for (int r = 0; r < k; ++r) {
+          const cd x = A(r, p), y = A(r, q);
+          A(r, p) = c * x + s * y;
+          A(r, q) = -s * x + c * y;
+        }
+        for (int cc = 0; cc < k; ++cc) {
+          const cd x = A(p, cc), y = A(q, cc);
+          A(p, cc) = c * x + s * y;
+          A(q, cc) = -s * x + c * y;
+        }
+        for (int r = 0; r < k; ++r) {
+          const cd x = (*v)[(size_t)r * k + p], y = (*v)[(size_t)r * k + q];
+          (*v)[(size_t)r * k + p] = c * x + s * y;
+          (*v)[(size_t)r * k + q] = -s * x + c * y;
+        }
+      }
+  }
+  w->resize(k);
+  for (int i = 0; i < k; ++i) (*w)[i] = A(i, i).real();
+}
+}  // namespace
+
+// Probe::initMLH (Probe.cpp:376-494): mode 0 from the guess or the Fresnel-zone-plate model;
+// modes 1..K-1 are mode 0 with random linear phase tilts (h_initVarProbe, ProbeKernels.cu:756-772,
+// d_imshift_fft utilitiesKernels.cu:784-826); then all modes are replaced by the eigenmodes of
+// their overlap matrix A_ij = sum conj(P_i) P_j (ComplexEigenSolver), strongest first.
+bool IPhaser::mls_probe_modes(std::vector<float>* modes) {
+  typedef std::complex<double> cd;
+  const int m = eng.m, k = eng.k;
+  const size_t mm = (size_t)m * m;
+  modes->assign(2 * mm * k, 0.f);
+  if (!prm.r.probeGuess.empty()) {
+    std::vector<float> g;
+    int r = 0, c = 0;
+    if (!load_complex_csv(prm.r.probeGuess, &g, &r, &c) || r != m || c != m) {
+      err = "Probe guess file (" + prm.r.probeGuess + ") not found or not " + std::to_string(m) + "x" +
+            std::to_string(m);
+      return false;
+    }
+    std::copy(g.begin(), g.end(), modes->begin());
+  } else if (!probe_guess.empty()) {
+    // probeNP: one mode (size x size) as in the reference, or all K modes stacked (K*size x size)
+    if (probe_guess_cols != m || (probe_guess_rows != m && probe_guess_rows != k * m)) {
+      err = "probe guess must be size x size or (probeModes*size) x size";
+      return false;
+    }
+    std::copy(probe_guess.begin(), probe_guess.end(), modes->begin());
+  } else {
+    // Fresnel zone plate at the focal distance plus beamSize (h_initProbeMLH / d_initProbeMLH,
+    // ProbeKernels.cu:220-366, float arithmetic), then FFT and the far-field phase
+    const double lambda = prm.e.lambda, Rn = 90e-6, dRn = 50e-9, fl = 2 * Rn * dRn / lambda;
+    const double D_FZP = 180e-6, D_H = 60e-6;
+    const double dx_fzp = lambda * fl / (m * prm.e.dx_s);
+    const float fzp = (float)((-2 / (lambda * 2 * fl)) * M_PI);
+    const float step = (float)(dx_fzp * m / (m - 1)), start = (float)(-dx_fzp * m / 2);
+    const double kk = 2 * M_PI / lambda, z = fl + prm.e.beamSize;
+    const std::complex<float> kz((float)std::cos(kk * z), (float)std::sin(kk * z));
+    const float ekdz = (float)(kk / (2 * z));
+    const double fu = lambda * (fl + prm.e.beamSize) / dx_fzp;
+    const float fustep = (float)(fu / (m - 1)), startfu = (float)(-fu / 2 + fu / (m - 1) * (m / 2));
+    const float rf = (float)(D_FZP / 2), rh = (float)(D_H / 2), mfu2 = (float)(-fu / 2);
+    std::vector<std::complex<float>> kern(mm), pf(mm);
+    for (int b = 0; b < m; ++b)
+      for (int t = 0; t < m; ++t) {
+        const float x = start + step * t, y = start + step * b;
+        const float r2 = x * x + y * y;
+        const float rr = std::sqrt(r2);
+        const std::complex<float> T(std::cos(fzp * r2), std::sin(fzp * r2));
+        const float C = rr <= rf ? 1.f : 0.f, H = rr >= rh ? 1.f : 0.f;
+        const float lx = t < m / 2 ? startfu + fustep * t : mfu2 + fustep * (t - m / 2);
+        const float ly = b < m / 2 ? startfu + fustep * b : mfu2 + fustep * (b - m / 2);
+        const float su = ekdz * (lx * lx + ly * ly), sx = ekdz * r2;
+        pf[(size_t)b * m + t] = kz * std::complex<float>(std::cos(su), std::sin(su));
+        const std::complex<float> kv = (C * T * H) * std::complex<float>(std::cos(sx), std::sin(sx));
+        kern[(size_t)((b + m / 2) % m) * m + (t + m / 2) % m] = kv;  // fftshift placement
+      }
+    std::vector<float> f(2 * mm);
+    for (size_t q = 0; q < mm; ++q) {
+      f[2 * q] = kern[q].real();
+      f[2 * q + 1] = kern[q].imag();
+    }
+    if (ptyk_fft2(eng.ctx, f.data(), 1, 0) != PTYK_OK) {
+      err = ptyk_last_error(eng.ctx);
+      return false;
+    }
+    for (int b = 0; b < m; ++b)
+      for (int t = 0; t < m; ++t) {
+        const size_t q = (size_t)b * m + t;
+        const std::complex<float> v = std::complex<float>(f[2 * q], f[2 * q + 1]) * pf[q];
+        const size_t d = (size_t)((b + m / 2) % m) * m + (t + m / 2) % m;
+        (*modes)[2 * d] = v.real();
+        (*modes)[2 * d + 1] = v.imag();
+      }
+  }
+  const bool all_given = !probe_guess.empty() && prm.r.probeGuess.empty() && probe_guess_rows == k * m;
+  Normal nrm(prm.r.seed + 303);
+  if (!all_given)
+    for (int i = 1; i < k; ++i) {
+      const float r1 = (float)nrm(), r2 = (float)nrm();
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c) {
+          const float gx = r1 * ((float)c / m - 0.5f), gy = r2 * ((float)r / m - 0.5f);
+          const float ax = (float)(2 * M_PI * gx), ay = (float)(2 * M_PI * gy);
+          const std::complex<float> t = std::complex<float>(std::cos(ax), -std::sin(ax)) *
+                                        std::complex<float>(std::cos(ay), -std::sin(ay));
+          const size_t q = (size_t)r * m + c;
+          const std::complex<float> v = std::complex<float>((*modes)[2 * q], (*modes)[2 * q + 1]) * t;
+          (*modes)[2 * (i * mm + q)] = v.real();
+          (*modes)[2 * (i * mm + q) + 1] = v.imag();
+        }
+    }
+  if (k > 1) {
+    std::vector<cd> a((size_t)k * k), v;
+    std::vector<double> w;
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        cd s = 0;
+        for (size_t q = 0; q < mm; ++q)
+          s += std::conj(cd((*modes)[2 * (i * mm + q)], (*modes)[2 * (i * mm + q) + 1])) *
+               cd((*modes)[2 * (j * mm + q)], (*modes)[2 * (j * mm + q) + 1]);
+        a[(size_t)i * k + j] = s;
+      }
+    hermitian_eig(a, k, &w, &v);
+    std::vector<int> ord(k);
+    for (int i = 0; i < k; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return std::fabs(w[x]) > std::fabs(w[y]); });
+    std::vector<float> out(modes->size());
+    for (int o = 0; o < k; ++o) {
+      const int e = ord[o];
+      // eigenvector phase fixed by its largest component (real, positive)
+      int big = 0;
+      for (int r = 1; r < k; ++r)
+        if (std::abs(v[(size_t)r * k + e]) > std::abs(v[(size_t)big * k + e])) big = r;
+      const cd ph = std::conj(v[(size_t)big * k + e]) / std::abs(v[(size_t)big * k + e]);
+      for (size_t q = 0; q < mm; ++q) {
+        cd s = 0;
+        for (int r = 0; r < k; ++r)
+          s += cd((*modes)[2 * (r * mm + q)], (*modes)[2 * (r * mm + q) + 1]) * (v[(size_t)r * k + e] * ph);
+        out[2 * (o * mm + q)] = (float)s.real();
+        out[2 * (o * mm + q) + 1] = (float)s.imag();
+      }
+    }
+    *modes = out;
+  }
+  return true;
+}
+
+// IPhaser::prePhase, MLs branch (IPhaser.cpp:283-295, 318-358): probe modes, object (guess or
+// Sample::initObject's random start), the variable mode (Probe::initVarModes) and the evolution
+// draws, then Probe::initEvo in the engine.
+bool IPhaser::mls_prephase() {
+  const int m = eng.m, n = eng.n;
+  std::vector<float> modes;
+  if (!mls_probe_modes(&modes)) return false;
+  if (ptyk_set_probe(eng.ctx, modes.data()) != PTYK_OK) {
+    err = ptyk_last_error(eng.ctx);
+    return false;
+  }
+  if (!prm.r.objectGuess.empty()) {
+    std::vector<float> g;
+    int r = 0, c = 0;
+    if (!load_complex_csv(prm.r.objectGuess, &g, &r, &c)) {
+      err = "Object guess file (" + prm.r.objectGuess + ") not found";
+      return false;
+    }
+    object_guess = g;
+    obj_guess_rows = r;
+    obj_guess_cols = c;
+  }
+  std::vector<float> canvas((size_t)2 * eng.rows * eng.cols, 0.f);
+  if (!object_guess.empty()) {
+    for (int i = 0; i < std::min(eng.rows, obj_guess_rows); ++i)
+      for (int j = 0; j < std::min(eng.cols, obj_guess_cols); ++j) {
+        canvas[2 * ((size_t)i * eng.cols + j)] = object_guess[2 * ((size_t)i * obj_guess_cols + j)];
+        canvas[2 * ((size_t)i * eng.cols + j) + 1] = object_guess[2 * ((size_t)i * obj_guess_cols + j) + 1];
+      }
+  } else {
+    // d_initRandObjectArray (SampleKernels.cu:328-339): |O| = u1, arg O = 2 pi u2 / 10.  The
+    // simulated case starts here too (the reference starts it from O = 0, which its own
+    // normalisation turns into NaN, Probe.cpp:579-605)
+    std::mt19937_64 g(prm.r.seed + 404);
+    for (size_t q = 0; q < (size_t)eng.rows * eng.cols; ++q) {
+      const float u1 = uniform_open0(g), u2 = uniform_open0(g);
+      const float a = (float)(2 * M_PI * u2 * 0.1);
+      canvas[2 * q] = u1 * std::cos(a);
+      canvas[2 * q + 1] = u1 * std::sin(a);
+    }
+  }
+  if (ptyk_set_object(eng.ctx, canvas.data()) != PTYK_OK) {
+    err = ptyk_last_error(eng.ctx);
+    return false;
+  }
+  // d_initVarMOdes (ProbeKernels.cu:99-110): u1 exp(2 pi i u2)
+  std::vector<float> vmode(2 * (size_t)m * m);
+  {
+    std::mt19937_64 g(prm.r.seed + 505);
+    for (size_t q = 0; q < (size_t)m * m; ++q) {
+      const float u1 = uniform_open0(g), u2 = uniform_open0(g);
+      const float a = (float)(2 * M_PI * u2);
+      vmode[2 * q] = u1 * std::cos(a);
+      vmode[2 * q + 1] = u1 * std::sin(a);
+    }
+  }
+  std::vector<double> evo(n);
+  Normal nrm(prm.r.seed + 606);
+  for (int i = 0; i < n; ++i) evo[i] = 1e-6 * nrm();
+  std::vector<float> pos(2 * (size_t)n);
+  for (int q = 0; q < n; ++q) {
+    pos[2 * q] = mesh->positions[q].x;
+    pos[2 * q + 1] = mesh->positions[q].y;
+  }
+  ptyk_mls_config cfg{};
+  cfg.probe_pos_search = prm.r.probe_pos_search;
+  cfg.probe_reconstruct = prm.r.probe_reconstruct;
+  cfg.object_reconstruct = prm.r.object_reconstruct;
+  cfg.variable_probe = prm.r.variable_probe;
+  cfg.variable_intensity = prm.r.variable_intensity;
+  cfg.variable_probe_modes = prm.r.variable_probe_modes;
+  cfg.apply_subpix_shift = prm.r.apply_subpix_shift;
+  cfg.apply_multimodal_update = prm.r.apply_multimodal_update;
+  cfg.n_objects = (int32_t)prm.r.Nobjects;
+  cfg.n_rec = prm.r.Nrec;
+  for (int i = 0; i < 4; ++i) cfg.object_box[i] = mesh->box[i];
+  cfg.delta_p = prm.r.delta_p;
+  cfg.beta_lsq = prm.r.beta_LSQ;
+  cfg.beta_object = prm.r.beta_object;
+  cfg.beta_probe = prm.r.beta_probe;
+  if (ptyk_mls_begin(eng.ctx, &cfg, pos.data(), vmode.data(), evo.data(), frame_power_.data()) != PTYK_OK) {
+    err = ptyk_last_error(eng.ctx);
+    return false;
   }
   errors.assign(prm.r.iterations, 0.0);
   for (auto& pm : methods) pm.method->initMem(*mesh, m);

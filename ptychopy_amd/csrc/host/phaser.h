@@ -81,7 +81,12 @@ class Parameters {
 // ---------------------------------------------------------------- scan meshes
 class ScanMesh {
  public:
-  std::vector<F2> positions;  // pixels, shifted to the canvas (addScanPosition)
+  // MLs geometry (generateMeshML): positions stay unshifted and unflipped (addScanPositionMLs,
+  // IPtychoScanMesh.cpp:111-120), the canvas is the MLs object size and box the
+  // calc_object_norm region (precalculate_ROI).
+  bool ml = false;
+  int box[4] = {0, 0, 0, 0};
+  std::vector<F2> positions;  // pixels, shifted to the canvas (addScanPosition) unless ml
   std::vector<unsigned> indices;
   F2 minima{0, 0}, maxima{0, 0};
   int canvas_x = 0, canvas_y = 0;
@@ -151,6 +156,27 @@ class DMMethod : public IPhasingMethod {
                 unsigned iter, bool rms, double* err) override;
 };
 
+// MLS (MLS.h:48-69): the frame partitions of get_nonoverlapping_indices are drawn once; each
+// iteration takes a random one and a shuffled group order (MLS.cpp:106-113), and the engine runs
+// the whole iteration (ptyk_mls_iteration).
+class MLSMethod : public IPhasingMethod {
+ public:
+  MLSMethod(uint64_t seed, int niter) : rng_(seed + 202), seed_(seed), niter_(niter) {}
+  void initMem(const ScanMesh& mesh, int probe_size) override;
+  int iteration(Engine& eng, const ScanMesh& mesh, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
+                unsigned iter, bool rms, double* err) override;
+  std::vector<std::vector<float>> fourierErrors;  // IPhaser::m_fourierErrors, per iteration
+  std::vector<int32_t> lastGroups, lastOffsets;
+
+ private:
+  std::mt19937_64 rng_;
+  uint64_t seed_;
+  int niter_;
+  std::vector<int32_t> frames_, offsets_;
+  int32_t groups_ = 0, parts_ = 0;
+  int64_t n_ = 0;
+};
+
 struct CXPhasing {
   std::string name;
   std::unique_ptr<IPhasingMethod> method;
@@ -190,6 +216,9 @@ class IPhaser {
   bool phaseStep(IPhasingMethod* m, unsigned i, double* e);
   void postPhase();
   bool simulate_frames(std::vector<float>* amp);
+  bool mls_prephase();
+  bool mls_probe_modes(std::vector<float>* modes);
+  std::vector<double> frame_power_;  // MLs: sum of each frame's intensities
   double t0_ = 0.0;
 };
 

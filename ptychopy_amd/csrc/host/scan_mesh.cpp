@@ -15,6 +15,7 @@
 // u in (0, 1] and the jitter keeps the reference's form r - u * 2r per axis.
 // nvcc contracts a*b + c into one fused multiply-add by default, so the device arithmetic of
 // d_generateMesh is reproduced with explicit fmaf; this file is compiled with contraction off.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -37,15 +38,15 @@ double ptycho_sample_pixel_size(double lambda, double z, int32_t size, double dx
   return (double)v;
 }
 
-int ptycho_mesh_cartesian(const ptycho_mesh_params* p, float* pos_out, int32_t* canvas_out, float* extent_out) {
-  if (!p || !pos_out || p->nx <= 0 || p->ny <= 0 || p->probe_size <= 0 || !(p->dx_s > 0)) return -1;
+// Unshifted Cartesian positions in pixels, row-major nx x ny (d_generateMesh<1> arithmetic).
+int ptycho_mesh_cartesian_raw(const ptycho_mesh_params* p, float* raw_out) {
+  if (!p || !raw_out || p->nx <= 0 || p->ny <= 0 || p->probe_size <= 0 || !(p->dx_s > 0)) return -1;
   const int nx = p->nx, ny = p->ny;
   const float step_x = (float)p->step_x, step_y = (float)p->step_y;  // ExperimentParams::stepSize is float2
   const float one_over_dxs = (float)(1.0 / p->dx_s);                  // real_t argument of d_generateMesh
   const float drift_x = (float)p->drift_x, drift_y = (float)p->drift_y;
   const int jr = p->jitter_radius;
   std::mt19937_64 gen(p->seed);
-  std::vector<float> px((size_t)nx * ny), py((size_t)nx * ny);
   for (int i = 0; i < nx; ++i)
     for (int j = 0; j < ny; ++j) {
       float jx = 0.f, jy = 0.f;
@@ -60,9 +61,22 @@ int ptycho_mesh_cartesian(const ptycho_mesh_params* p, float* pos_out, int32_t* 
       float vy = (p->mirror_y ? (d - m) : (m - d)) * step_y;
       vx = std::fmaf((float)i, drift_x, vx);
       vy = std::fmaf((float)j, drift_y, vy);
-      px[(size_t)i * ny + j] = std::fmaf(vx, one_over_dxs, jx);
-      py[(size_t)i * ny + j] = std::fmaf(vy, one_over_dxs, jy);
+      raw_out[2 * ((size_t)i * ny + j)] = std::fmaf(vx, one_over_dxs, jx);
+      raw_out[2 * ((size_t)i * ny + j) + 1] = std::fmaf(vy, one_over_dxs, jy);
     }
+  return 0;
+}
+
+int ptycho_mesh_cartesian(const ptycho_mesh_params* p, float* pos_out, int32_t* canvas_out, float* extent_out) {
+  if (!p || !pos_out || p->nx <= 0 || p->ny <= 0 || p->probe_size <= 0 || !(p->dx_s > 0)) return -1;
+  const int nx = p->nx, ny = p->ny;
+  std::vector<float> raw(2 * (size_t)nx * ny);
+  if (ptycho_mesh_cartesian_raw(p, raw.data()) != 0) return -1;
+  std::vector<float> px((size_t)nx * ny), py((size_t)nx * ny);
+  for (size_t q = 0; q < px.size(); ++q) {
+    px[q] = raw[2 * q];
+    py[q] = raw[2 * q + 1];
+  }
   // extents over the pitched buffer: rows of ny values padded with zeros to a multiple of 32
   float mxx = FLT_MIN, mxy = FLT_MIN, mnx = FLT_MAX, mny = FLT_MAX;
   for (size_t q = 0; q < px.size(); ++q) {
@@ -98,3 +112,73 @@ int ptycho_mesh_cartesian(const ptycho_mesh_params* p, float* pos_out, int32_t* 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- MLs geometry
+extern "C" int ptycho_mls_canvas(const float* pos, int64_t n, const float* extent, int32_t probe_size,
+                                 int32_t* canvas_out, int32_t* box_out) {
+  if (!pos || !extent || n <= 0 || probe_size <= 0) return -1;
+  // generateMeshML (CartesianScanMesh.cpp:118-172): ROI bounds of every position on a 1e5 x 1e5
+  // sentinel canvas, centred on the middle of the extents.  The reference takes the row extent
+  // from the column coordinates and vice versa (pos.y paired with ceilvar.x), which leaves the
+  // ROIs of a non-square scan outside its canvas; here rows come from the row coordinates (the
+  // same canvas for square scans).
+  const float Mf = (float)probe_size;
+  const float cv = (float)std::ceil(1e5f * 1.0 / 2 - probe_size * 1.0 / 2);
+  const float mnx = extent[0], mny = extent[1], mxx = extent[2], mxy = extent[3];
+  const float rx = (mxx + mnx) / 2, ry = (mxy + mny) / 2;
+  unsigned lox = (unsigned)1e5f, loy = (unsigned)1e5f, hix = 0, hiy = 0;
+  for (int64_t q = 0; q < n; ++q) {
+    const unsigned tx = (unsigned)std::round(pos[2 * q] - rx + cv);
+    const unsigned ty = (unsigned)std::round(pos[2 * q + 1] - ry + cv);
+    lox = std::min(lox, tx - 1);
+    hix = std::max(hix, (unsigned)((float)tx + Mf - 2));
+    loy = std::min(loy, ty - 1);
+    hiy = std::max(hiy, (unsigned)((float)ty + Mf - 2));
+  }
+  const float extra = 1 + 0.2f;
+  const unsigned npx = (unsigned)std::ceil((float)(hix - lox) * extra);
+  const unsigned npy = (unsigned)std::ceil((float)(hiy - loy) * extra);
+  if (canvas_out) {
+    canvas_out[0] = (int32_t)npx;
+    canvas_out[1] = (int32_t)npy;
+  }
+  // IPtychoScanMesh::precalculate_ROI (IPtychoScanMesh.cpp:179-184), 1-based inclusive
+  if (box_out) {
+    box_out[0] = (int32_t)std::round(npx * 0.5 + (double)mnx);
+    box_out[1] = (int32_t)std::round(npx * 0.5 + (double)mxx) - 1;
+    box_out[2] = (int32_t)std::round(npy * 0.5 + (double)mny);
+    box_out[3] = (int32_t)std::round(npy * 0.5 + (double)mxy) - 1;
+  }
+  return 0;
+}
+
+// IPtychoScanMesh::get_nonoverlapping_indices (IPtychoScanMesh.cpp:826-913): min(Niter, 30)
+// random partitions of the frames into groups of floor(N/100) (at least 1), each partition's
+// groups ordered by decreasing size (ties in creation order).  random_shuffle becomes a seeded
+// Fisher-Yates shuffle.  Null outputs: only the counts.
+extern "C" int ptycho_mls_partitions(int64_t n, int32_t niter, uint64_t seed, int32_t* frames_out,
+                                     int32_t* offsets_out, int32_t* n_groups, int32_t* n_partitions) {
+  if (n <= 0 || niter <= 0 || !n_groups || !n_partitions) return -1;
+  const int64_t grouping = std::max<int64_t>(1, n / 100);
+  const int32_t groups = (int32_t)((n + grouping - 1) / grouping);
+  const int32_t parts = std::min(niter, 30);
+  *n_groups = groups;
+  *n_partitions = parts;
+  if (!frames_out || !offsets_out) return 0;
+  std::mt19937_64 g(seed);
+  std::vector<int32_t> idx(n);
+  for (int32_t p = 0; p < parts; ++p) {
+    for (int64_t i = 0; i < n; ++i) idx[i] = (int32_t)i;
+    for (int64_t i = n - 1; i > 0; --i) std::swap(idx[i], idx[g() % (uint64_t)(i + 1)]);
+    // groups j cover idx[j*grouping, min(n, (j+1)*grouping)); sizes are non-increasing in j
+    int32_t* fr = frames_out + (size_t)p * n;
+    int32_t* of = offsets_out + (size_t)p * (groups + 1);
+    of[0] = 0;
+    for (int32_t j = 0; j < groups; ++j) {
+      const int64_t b = j * grouping, e = std::min<int64_t>(n, (j + 1) * grouping);
+      for (int64_t i = b; i < e; ++i) fr[i] = idx[i];
+      of[j + 1] = (int32_t)e;
+    }
+  }
+  return 0;
+}

@@ -61,6 +61,7 @@ def _lib():
         L.ptycho_phaser_get_param.argtypes = [P, s, C.c_char_p, i32]
         L.ptycho_phaser_set_diffraction.argtypes = [P, P, i64, i32, i32]
         L.ptycho_phaser_set_positions.argtypes = [P, P, i64]
+        L.ptycho_phaser_get_fourier_errors.argtypes = [P, C.c_uint32, P, i64]
         L.ptycho_phaser_set_object_guess.argtypes = [P, P, i32, i32]
         L.ptycho_phaser_set_probe_guess.argtypes = [P, P, i32, i32]
         for n in ("init", "phase", "phaseinit", "phasepost"):
@@ -187,6 +188,13 @@ class Phaser:
         self._chk(self.L.ptycho_phaser_get_errors(self.h, out.ctypes.data, n), "errors")
         return out
 
+    def get_fourier_errors(self, it: int) -> np.ndarray:
+        """MLs per-frame Fourier errors of iteration it (filled for it < 10)."""
+        n = self.L.ptycho_phaser_num_positions(self.h)
+        out = np.zeros(max(n, 0), np.float32)
+        self._chk(self.L.ptycho_phaser_get_fourier_errors(self.h, int(it), out.ctypes.data, n), "fourier errors")
+        return out
+
     @property
     def engine_handle(self):
         return self.L.ptycho_phaser_engine(self.h)
@@ -290,6 +298,9 @@ def _cmdstr(algorithm: str, cmdstr: str) -> np.ndarray:
     ph = Phaser()
     try:
         ph.parse_cmdline(cmdstr)
+        # the entry point names the method; the reference also needs "-algorithm=MLs" in the
+        # string for the MLs geometry (IPhaser.cpp:156-166), which is implied here
+        ph.set_param("algorithm", algorithm)
         ph.init()
         ph.add_method(algorithm, int(ph.get_param("iterations")))
         ph.phase()
@@ -317,6 +328,7 @@ def _step_init(algorithm: str, ph: Phaser, iterations: int):
     if _phaser is not None:
         _phaser.close()
     _phaser, _istep = ph, 0
+    ph.set_param("algorithm", algorithm)
     ph.init()
     ph.add_method(algorithm, iterations)
     ph.phaseinit()

@@ -180,3 +180,43 @@ def test_driver_exports():
     assert declared == set(D.DRIVER_EXPORTS)
     lib = ctypes.CDLL(str(_engine.LIB_PATH))
     assert not [s for s in declared if not hasattr(lib, s)]
+
+
+# ---------------------------------------------------------------- MLs geometry
+def _padded_extent(raw, ny):
+    mn, mx = raw.min(axis=0), raw.max(axis=0)
+    if ny % 32:
+        mn, mx = np.minimum(mn, 0), np.maximum(mx, 0)
+    return np.array([mn[0], mn[1], mx[0], mx[1]], np.float32)
+
+
+@pytest.mark.parametrize("nx,ny", [(51, 51), (3, 40), (40, 3)])
+def test_mls_canvas_holds_every_roi(nx, ny):
+    import oracle
+    m = 256
+    dx_s = D.sample_pixel_size(0.248e-9, 1.0, m, 172e-6)
+    raw = D.cartesian_mesh_raw(nx, ny, 100e-9, 100e-9, dx_s, m)
+    # unshifted and centred (CartesianScanMesh::generateMeshML keeps the kernel output)
+    assert np.allclose(raw.mean(axis=0), 0, atol=1e-3)
+    (rows, cols), box = D.mls_canvas(raw, _padded_extent(raw, ny), m)
+    roi, sub = oracle.mls_roi(raw, m, rows, cols)
+    assert roi.min() >= 0 and roi[:, 0].max() + m <= rows and roi[:, 1].max() + m <= cols
+    # 20 % margin around the ROI span, per axis
+    span = roi.max(axis=0) - roi.min(axis=0) + m - 1
+    assert abs(rows - 1.2 * span[0]) <= 2 and abs(cols - 1.2 * span[1]) <= 2
+    # calc_object_norm box: 1-based, inside the canvas, centred on the scan
+    assert 1 <= box[0] <= box[1] <= rows and 1 <= box[2] <= box[3] <= cols
+    assert abs((box[0] + box[1]) / 2 - rows / 2) <= 1.5 and abs((box[2] + box[3]) / 2 - cols / 2) <= 1.5
+
+
+def test_mls_partitions():
+    P = D.mls_partitions(2601, 100, seed=3)
+    assert len(P) == 30  # min(Niter, 30)
+    for part in P:
+        assert [len(g) for g in part] == [26] * 100 + [1]
+        assert np.array_equal(np.sort(np.concatenate(part)), np.arange(2601))
+    assert not np.array_equal(P[0][0], P[1][0])
+    Q = D.mls_partitions(2601, 100, seed=3)
+    assert all(np.array_equal(a, b) for a, b in zip(P[5], Q[5]))
+    small = D.mls_partitions(50, 5, seed=1)  # fewer than 100 frames: groups of one
+    assert len(small) == 5 and all(len(g) == 1 for g in small[0]) and len(small[0]) == 50

@@ -90,3 +90,80 @@ def test_dm_whole_mode_with_position_list(gpu, tmp_path, monkeypatch):
     ph.phase()
     assert ph.get_object().shape == lcanvas
     ph.close()
+
+
+# ---------------------------------------------------------------- MLs through the driver
+MLS_CMD = ("-jobID=m -simulate=1 -scanDims=6,6 -step=5e-8 -size=64 -lambda=0.248e-9 -z=1 -dx_d=172e-6 "
+           "-beamSize=1e-7 -probeModes=2 -PPS=5")
+
+
+def test_mls_simulated_run_lowers_the_fourier_error(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    ph = ptychopy.Phaser()
+    ph.parse_cmdline(MLS_CMD + " -i=9")
+    ph.set_param("algorithm", "MLs")
+    ph.init()
+    ph.add_method("MLs", 9)
+    ph.phase()
+    e1, e9 = ph.get_fourier_errors(1), ph.get_fourier_errors(9)
+    assert np.isfinite(ph.get_object()).all() and np.isfinite(e9).all()
+    assert e9.mean() < 0.7 * e1.mean()
+    assert np.array_equal(ph.get_errors(9), np.ones(9))  # MLS::iteration returns 1 (MLS.cpp:489)
+    ph.close()
+
+
+def test_mls_initial_modes_are_orthogonal_by_power(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    assert ptychopy.mlsinit(MLS_CMD.replace("-probeModes=2", "-probeModes=3") + " -i=2") is True
+    P = ptychopy.mlsresprobe().reshape(3, -1).astype(np.complex128)
+    G = P.conj() @ P.T
+    pw = np.real(np.diag(G))
+    assert np.all(np.diff(pw) < 0)
+    off = np.abs(G - np.diag(np.diag(G)))
+    assert off.max() < 1e-5 * pw.max()
+    assert ptychopy.mlspost() is True
+
+
+def test_mls_step_mode_equals_whole_run(gpu, tmp_path, monkeypatch):
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    cmd = MLS_CMD + " -i=6"
+    assert ptychopy.mlsinit(cmd) is True
+    for _ in range(6):
+        assert ptychopy.mlsstep() is True
+    o_step = ptychopy.mlsresobj()
+    p_step = ptychopy.mlsresprobe()
+    assert ptychopy.mlspost() is True
+    o_whole = ptychopy.mlscmdstr(cmd)
+    assert np.array_equal(o_step, o_whole)
+    assert p_step.shape == (128, 64)
+
+
+def test_mlsnp_with_all_probe_modes_given(gpu, tmp_path, monkeypatch):
+    import oracle
+    from problems import mls_modes
+    from ptychopy_amd import _driver as D, ptychopy, sim
+    monkeypatch.chdir(tmp_path)
+    m, n, step = 64, 6, 5e-8
+    dx_s = D.sample_pixel_size(LAMBD, Z, m, DX_D)
+    raw = D.cartesian_mesh_raw(n, n, step, step, dx_s, m)
+    ext = np.array([min(raw[:, 0].min(), 0), min(raw[:, 1].min(), 0), max(raw[:, 0].max(), 0),
+                    max(raw[:, 1].max(), 0)], np.float32)
+    (rows, cols), _ = D.mls_canvas(raw, ext, m)
+    rng = np.random.default_rng(5)
+    obj = sim.random_field_object((rows, cols), rng)
+    modes = mls_modes(m, 2, rng)
+    sim_pos = raw + np.float32([np.ceil(rows / 2 - m / 2) - 1, np.ceil(cols / 2 - m / 2) - 1])
+    inten = sum(oracle.simulate(obj, modes[k], sim_pos, squared=True) for k in range(2))
+    inten = rng.poisson(inten / inten.sum(axis=(1, 2), keepdims=True) * 1e6).astype(np.float64)
+    ph = ptychopy._make_phaser("MLs", *ptychopy._collect(
+        (), dict(diffractionNP=inten, probeNP=modes.reshape(2 * m, m), iter=8, size=m, scanDimsx=n, scanDimsy=n,
+                 stepx=step, stepy=step, lambd=LAMBD, z=Z, dx_d=DX_D, probeModes=2, PPS=5), True))
+    ph.init()
+    ph.add_method("MLs", 8)
+    ph.phase()
+    assert ph.get_object().shape == (rows, cols)
+    assert ph.get_fourier_errors(8).mean() < 0.5 * ph.get_fourier_errors(1).mean()
+    ph.close()
