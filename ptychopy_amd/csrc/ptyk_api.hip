@@ -102,7 +102,7 @@ static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal,
-                  d.gframes};
+                  d.gframes, d.ftot};
   for (void* p : ptrs)
     if (p) hipFree(p);
   d = MlsDev();
@@ -757,11 +757,11 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const int ch = mls_chunks(d.m);
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
   if (g > d.gmax) {
-    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.gscal};
+    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.gscal, d.ftot};
     for (void* p : ptrs)
       if (p) hipFree(p);
     d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = nullptr;
-    d.gscal = nullptr;
+    d.gscal = d.ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
     HIPCHK(c, dalloc(&d.op, gm));
@@ -771,6 +771,7 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.tmp, std::max<long>(g, d.k) * mm));
     HIPCHK(c, dalloc(&d.tmp2, gm));
     HIPCHK(c, dalloc(&d.gscal, (size_t)MLS_S_COEF + g));
+    HIPCHK(c, dalloc(&d.ftot, (size_t)8 * g));
     d.gmax = g;
   }
   const long need = std::max<long>({6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L});

@@ -134,6 +134,7 @@ struct MlsDev {
   float2 *spec = nullptr, *pum = nullptr, *acc = nullptr, *precond = nullptr;
   double* part = nullptr;
   long part_cap = 0;
+  double* ftot = nullptr;       // per-frame totals of the group (gmax x 8)
   double* gscal = nullptr;
   int* gframes = nullptr;       // frames of all groups of one iteration
 };

@@ -41,6 +41,21 @@ __device__ __forceinline__ float freq(int i, int m) { return (float)((i + (m >> 
 
 __device__ __forceinline__ float2 c_of(double re, double im) { return make_float2((float)re, (float)im); }
 
+// Per-frame totals of chunk partials: one wave per frame, lanes over chunks, fixed order.
+// tot[j * nv + v] = sum_c part[(j * ch + c) * nv + v].  Ends with a block barrier.
+template <int BT>
+__device__ void frame_totals(const double* __restrict__ part, int ch, int nv, int G, double* __restrict__ tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < G; j += BT / 64)
+    for (int v = 0; v < nv; ++v) {
+      double s = 0.0;
+      for (int c = lane; c < ch; c += 64) s += part[((long)j * ch + c) * nv + v];
+      s = wave_sum(s);
+      if (lane == 0) tot[(long)j * nv + v] = s;
+    }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------------ ROI origins
 // IPtychoScanMesh::find_reconstruction_ROI (IPtychoScanMesh.cpp:123-172), in its float
 // arithmetic: t = round(pos + ceil(Np_o/2 - M/2)), origin t - 1, sub = (col frac, row frac).
@@ -72,6 +87,15 @@ __global__ void k_scale_c(float2* __restrict__ x, long count, const double* __re
 __global__ void k_copy_c(float2* __restrict__ dst, const float2* __restrict__ src, long count) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) dst[i] = src[i];
+}
+
+__global__ void k_copy2(float2* __restrict__ dst, const float2* __restrict__ a, const float2* __restrict__ b,
+                        long count) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    dst[i] = a[i];
+    dst[count + i] = b[i];
+  }
 }
 
 __global__ void k_fill_c(float2* __restrict__ x, long count, float2 v) {
@@ -209,23 +233,39 @@ __global__ __launch_bounds__(NT) void k_remove_extra(float2* __restrict__ vmode,
 
 // Illumination canvas (h_preCalillum / updated_cached_illumination, ProbeKernels.cu:691-730,
 // Probe.cpp:701-729): s = sum over frames covering the pixel of |P0|^2 at the local offset, in
-// frame order; mode 0 stores s, mode 1 the running average (p + s) / 2.  Tiles of 4 x 64.
+// frame order; mode 0 stores s, mode 1 the running average (p + s) / 2.  Tiles of 4 x 64; each
+// chunk of 256 frames is first compacted (in frame order) to those whose ROI meets the tile.
 __global__ __launch_bounds__(NT) void k_illum(float* __restrict__ illum, const float2* __restrict__ p0,
                                               const int2* __restrict__ roi, int n, int m, int rows, int cols,
                                               int mode) {
   __shared__ int2 lr[NT];
+  __shared__ int wcnt[NT / 64];
   const int r0 = blockIdx.y * 4, c0 = blockIdx.x * 64;
   const int r = r0 + (threadIdx.x >> 6), c = c0 + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float s = 0.f;
   for (int base = 0; base < n; base += NT) {
-    const int cnt = min(NT, n - base);
+    const int f = base + threadIdx.x;
+    int2 o = make_int2(0, 0);
+    bool hit = false;
+    if (f < n) {
+      o = roi[f];
+      hit = !(o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0);
+    }
+    const unsigned long long bal = __ballot(hit);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) lr[threadIdx.x] = roi[base + threadIdx.x];
+    if (lane == 0) wcnt[w] = __popcll(bal);
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      const int2 o = lr[j];
-      if (o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
-      const int lrw = r - o.x, lc = c - o.y;
+    int off = 0, tot = 0;
+    for (int i = 0; i < NT / 64; ++i) {
+      if (i < w) off += wcnt[i];
+      tot += wcnt[i];
+    }
+    if (hit) lr[off + __popcll(bal & ((1ull << lane) - 1ull))] = o;
+    __syncthreads();
+    for (int j = 0; j < tot; ++j) {
+      const int2 q = lr[j];
+      const int lrw = r - q.x, lc = c - q.y;
       if (lrw >= 0 && lrw < m && lc >= 0 && lc < m) {
         const float2 v = p0[lrw * m + lc];
         const float a = sqrtf(v.x * v.x + v.y * v.y);
@@ -270,9 +310,10 @@ __global__ void k_vp(float2* __restrict__ vp, const float2* __restrict__ spec, c
 
 // obj_proj = O at the ROI (d_readFromArray_c, utilitiesKernels.cu:1064-1084);
 // psi_0 = obj_proj * vp (with the sub-pixel shift) or P0 * obj_proj, psi_k = P_k * obj_proj.
-__global__ void k_psi(float2* __restrict__ psi, float2* __restrict__ op, const float2* __restrict__ vp,
-                      const float2* __restrict__ obj, const float2* __restrict__ probe, const int2* __restrict__ roi,
-                      const int* __restrict__ gf, int G, int K, int m, int cols, int subpix) {
+__global__ void k_psi(float2* __restrict__ psi, float2* __restrict__ op, float2* __restrict__ op2,
+                      const float2* __restrict__ vp, const float2* __restrict__ obj, const float2* __restrict__ probe,
+                      const int2* __restrict__ roi, const int* __restrict__ gf, int G, int K, int m, int cols,
+                      int subpix) {
   const int mm = m * m;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= mm) return;
@@ -281,6 +322,7 @@ __global__ void k_psi(float2* __restrict__ psi, float2* __restrict__ op, const f
   const int r = q / m, c = q % m;
   const float2 O = obj[(long)(o.x + r) * cols + o.y + c];
   op[(long)j * mm + q] = O;
+  if (op2) op2[(long)j * mm + q] = O;  // input of the position solver's transform
   psi[(long)j * mm + q] = subpix ? cmul(O, vp[(long)j * mm + q]) : cmul(probe[q], O);
   for (int k = 1; k < K; ++k) psi[((long)k * G + j) * mm + q] = cmul(probe[(long)k * mm + q], O);
 }
@@ -360,6 +402,7 @@ __global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, 
   if (q >= mm) return;
   const int r = q / m, c = q % m;
   float2 acc = make_float2(0.f, 0.f);
+#pragma unroll 4
   for (int j = 0; j < G; ++j) {
     const int2 o = roi[gf[j]];
     const float p = illum[(long)(o.x + r) * cols + o.y + c];
@@ -373,6 +416,45 @@ __global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, 
     acc.y += w * u.y;
   }
   pum[q] = cscale(acc, (float)(1.0 / G));
+}
+
+// Probe updates of modes 2..K in one pass (MLS.cpp:251-279 for ll >= 2, no sub-pixel shift):
+// pum_ll = mean_j w_j chi_ll,j conj(op_j), P_ll += pum_ll * mean probe step.  Nothing else in the
+// group reads these modes between the reference's per-mode passes, so they fuse exactly.
+template <int KMAX>
+__global__ void k_pum_modes(float2* __restrict__ probe, const float2* __restrict__ psi, const float2* __restrict__ op,
+                            const float* __restrict__ illum, const int2* __restrict__ roi, const int* __restrict__ gf,
+                            int G, int K, int m, int cols, float fw, const double* __restrict__ gscal) {
+  const int mm = m * m;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int r = q / m, c = q % m;
+  float2 acc[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) acc[k] = make_float2(0.f, 0.f);
+  const long gm = (long)G * mm;
+  for (int j = 0; j < G; ++j) {
+    const int2 o = roi[gf[j]];
+    const float p = illum[(long)(o.x + r) * cols + o.y + c];
+    const float w = p / (p + fw);
+    const float2 b = op[(long)j * mm + q];
+#pragma unroll
+    for (int k = 1; k < KMAX; ++k) {
+      if (k >= K) break;
+      const float2 u = psi[k * gm + (long)j * mm + q];
+      const float2 d = make_float2(u.x * b.x + u.y * b.y, u.y * b.x - u.x * b.y);
+      acc[k].x += w * d.x;
+      acc[k].y += w * d.y;
+    }
+  }
+  const float inv = (float)(1.0 / G);
+  const float2 bp = c_of(gscal[MLS_S_BP], gscal[MLS_S_BP + 1]);
+#pragma unroll
+  for (int k = 1; k < KMAX; ++k) {
+    if (k >= K) break;
+    const long i = (long)k * mm + q;
+    probe[i] = cadd(probe[i], cmul(cscale(acc[k], inv), bp));
+  }
 }
 
 // Object update direction of frame j at local pixel q: chi conj(vp_j) for the first probe mode,
@@ -479,27 +561,25 @@ __global__ __launch_bounds__(NT) void k_lsq_partial(const float2* __restrict__ o
 // beta * beta_LSQ (MLS.cpp:370-416); then the group means of the object and probe steps
 // (Sample::update_object with one scan id, Sample.cpp:277-331; Probe::update_probe,
 // Probe.cpp:1339-1364).  solve = 0: only the means of the current steps.
-__global__ __launch_bounds__(NT) void k_lsq_solve(const double* __restrict__ part, int ch, const int* __restrict__ gf,
-                                                  int G, double lam_mm, double c_obj, double c_probe, int solve,
-                                                  float2* __restrict__ beta_obj, float2* __restrict__ beta_probe,
-                                                  double* __restrict__ gscal) {
-  __shared__ double red[NT / 64];
-  auto tot = [&](int j, int v) {
-    double s = 0.0;
-    for (int c = 0; c < ch; ++c) s += part[((long)j * ch + c) * 6 + v];
-    return s;
-  };
+__global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ part, int ch,
+                                                    const int* __restrict__ gf, int G, double lam_mm, double c_obj,
+                                                    double c_probe, int solve, float2* __restrict__ beta_obj,
+                                                    float2* __restrict__ beta_probe, double* __restrict__ ftot,
+                                                    double* __restrict__ gscal) {
+  __shared__ double red[16];
   if (solve) {
+    frame_totals<1024>(part, ch, 6, G, ftot);
     double a1 = 0.0, a4 = 0.0;
-    for (int j = threadIdx.x; j < G; j += NT) {
-      a1 += tot(j, 0) + lam_mm;
-      a4 += tot(j, 3) + lam_mm;
+    for (int j = threadIdx.x; j < G; j += 1024) {
+      a1 += ftot[j * 6 + 0] + lam_mm;
+      a4 += ftot[j * 6 + 3] + lam_mm;
     }
-    const double m1 = 0.1 * block_sum<NT>(a1, red) / G;
-    const double m4 = 0.1 * block_sum<NT>(a4, red) / G;
-    for (int j = threadIdx.x; j < G; j += NT) {
-      const double A11 = tot(j, 0) + lam_mm + m1, A22 = tot(j, 3) + lam_mm + m4;
-      const double a2r = tot(j, 1), a2i = tot(j, 2), b1 = tot(j, 4), b2 = tot(j, 5);
+    const double m1 = 0.1 * block_sum<1024>(a1, red) / G;
+    const double m4 = 0.1 * block_sum<1024>(a4, red) / G;
+    for (int j = threadIdx.x; j < G; j += 1024) {
+      const double* t = ftot + j * 6;
+      const double A11 = t[0] + lam_mm + m1, A22 = t[3] + lam_mm + m4;
+      const double a2r = t[1], a2i = t[2], b1 = t[4], b2 = t[5];
       // A = [[A11, a2], [conj(a2), A22]], det real
       const double det = A11 * A22 - (a2r * a2r + a2i * a2i);
       const double x0r = (b1 * A22 - a2r * b2) / det, x0i = (-a2i * b2) / det;
@@ -509,16 +589,16 @@ __global__ __launch_bounds__(NT) void k_lsq_solve(const double* __restrict__ par
     }
     __syncthreads();
   }
-  double s[4] = {0, 0, 0, 0};
-  for (int j = threadIdx.x; j < G; j += NT) {
+  double sm[4] = {0, 0, 0, 0};
+  for (int j = threadIdx.x; j < G; j += 1024) {
     const float2 bo = beta_obj[gf[j]], bp = beta_probe[gf[j]];
-    s[0] += bo.x;
-    s[1] += bo.y;
-    s[2] += bp.x;
-    s[3] += bp.y;
+    sm[0] += bo.x;
+    sm[1] += bo.y;
+    sm[2] += bp.x;
+    sm[3] += bp.y;
   }
   for (int v = 0; v < 4; ++v) {
-    const double t = block_sum<NT>(s[v], red);
+    const double t = block_sum<1024>(sm[v], red);
     if (threadIdx.x == 0) gscal[MLS_S_BO + v] = t / G;
   }
 }
@@ -612,18 +692,15 @@ __global__ __launch_bounds__(NT) void k_vp_resid(float2* __restrict__ img, const
 }
 
 // c_j = (mean_px Re(V conj img_j) + e1_j) / sum_g e1_g^2  -> gscal[COEF + j]
-__global__ __launch_bounds__(NT) void k_vp_coef(const double* __restrict__ part, int ch, const int* __restrict__ gf,
-                                                int G, int mm, const double* __restrict__ evo, int n,
-                                                double* __restrict__ gscal) {
-  __shared__ double red[NT / 64];
+__global__ __launch_bounds__(1024) void k_vp_coef(const double* __restrict__ part, int ch, const int* __restrict__ gf,
+                                                  int G, int mm, const double* __restrict__ evo, int n,
+                                                  double* __restrict__ ftot, double* __restrict__ gscal) {
+  __shared__ double red[16];
+  frame_totals<1024>(part, ch, 1, G, ftot);
   double e = 0.0;
-  for (int j = threadIdx.x; j < G; j += NT) e += evo[n + gf[j]] * evo[n + gf[j]];
-  e = block_sum<NT>(e, red);
-  for (int j = threadIdx.x; j < G; j += NT) {
-    double s = 0.0;
-    for (int c = 0; c < ch; ++c) s += part[(long)j * ch + c];
-    gscal[MLS_S_COEF + j] = (s / mm + evo[n + gf[j]]) / e;
-  }
+  for (int j = threadIdx.x; j < G; j += 1024) e += evo[n + gf[j]] * evo[n + gf[j]];
+  e = block_sum<1024>(e, red);
+  for (int j = threadIdx.x; j < G; j += 1024) gscal[MLS_S_COEF + j] = (ftot[j] / mm + evo[n + gf[j]]) / e;
 }
 
 // acc = (1/M^2) sum_j c_j img_j; partial sums of |acc|^2 per block.
@@ -649,7 +726,8 @@ __global__ __launch_bounds__(NT) void k_vp_apply(float2* __restrict__ vmode, con
                                                  double* __restrict__ part_out) {
   __shared__ double red[NT / 64];
   double s = 0.0;
-  for (int i = 0; i < nb; ++i) s += part_in[i];
+  for (int i = threadIdx.x; i < nb; i += NT) s += part_in[i];
+  s = block_sum<NT>(s, red);
   const float rms = sqrtf((float)(s / mm));
   const float sc = (float)(relax_u / fmax(1.0, (double)rms));
   const int q = blockIdx.x * NT + threadIdx.x;
@@ -663,9 +741,12 @@ __global__ __launch_bounds__(NT) void k_vp_apply(float2* __restrict__ vmode, con
   if (threadIdx.x == 0) part_out[blockIdx.x] = v2;
 }
 
-__global__ void k_vp_norm(float2* __restrict__ vmode, int mm, const double* __restrict__ part, int nb) {
+__global__ __launch_bounds__(NT) void k_vp_norm(float2* __restrict__ vmode, int mm, const double* __restrict__ part,
+                                                int nb) {
+  __shared__ double red[NT / 64];
   double s = 0.0;
-  for (int i = 0; i < nb; ++i) s += part[i];
+  for (int i = threadIdx.x; i < nb; i += NT) s += part[i];
+  s = block_sum<NT>(s, red);
   const float rms = sqrtf((float)(s / mm));
   const float inv = (float)(1.0 / (double)rms);
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -697,21 +778,20 @@ __global__ __launch_bounds__(NT) void k_frame_dots(const float2* __restrict__ op
 
 // Evolution updates: e1 += mean(u) / (mean(d) + 0.1 mean_g mean(d)) (variable probe);
 // e0 += 0.1 sum(u) / sum(d) (variable intensity, Probe.cpp:1496-1516).
-__global__ __launch_bounds__(NT) void k_evo_update(const double* __restrict__ part, int ch, const int* __restrict__ gf,
-                                                   int G, int mm, double* __restrict__ evo, int n, int which) {
-  __shared__ double red[NT / 64];
-  auto tot = [&](int j, int v) {
-    double s = 0.0;
-    for (int c = 0; c < ch; ++c) s += part[((long)j * ch + c) * 2 + v];
-    return s;
-  };
+__global__ __launch_bounds__(1024) void k_evo_update(const double* __restrict__ part, int ch,
+                                                     const int* __restrict__ gf, int G, int mm,
+                                                     double* __restrict__ evo, int n, int which,
+                                                     double* __restrict__ ftot) {
+  __shared__ double red[16];
+  frame_totals<1024>(part, ch, 2, G, ftot);
   if (which == 1) {
     double dm = 0.0;
-    for (int j = threadIdx.x; j < G; j += NT) dm += tot(j, 0) / mm;
-    const double m3 = block_sum<NT>(dm, red) / G;
-    for (int j = threadIdx.x; j < G; j += NT) evo[n + gf[j]] += (tot(j, 1) / mm) / (tot(j, 0) / mm + 0.1 * m3);
+    for (int j = threadIdx.x; j < G; j += 1024) dm += ftot[2 * j] / mm;
+    const double m3 = block_sum<1024>(dm, red) / G;
+    for (int j = threadIdx.x; j < G; j += 1024)
+      evo[n + gf[j]] += (ftot[2 * j + 1] / mm) / (ftot[2 * j] / mm + 0.1 * m3);
   } else {
-    for (int j = threadIdx.x; j < G; j += NT) evo[gf[j]] += 0.1 * tot(j, 1) / tot(j, 0);
+    for (int j = threadIdx.x; j < G; j += 1024) evo[gf[j]] += 0.1 * ftot[2 * j + 1] / ftot[2 * j];
   }
 }
 
@@ -918,16 +998,17 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const long gm = (long)G * mm;
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
+    const bool pos_step = iter >= d.pps;
     if (subpix) {
-      LAUNCH(k_copy_c, g1(mm), dim3(NT), d.spec, d.probe, (long)mm);
-      LAUNCH(k_copy_c, g1(mm), dim3(NT), d.spec + mm, d.vmode, (long)mm);
+      LAUNCH(k_copy2, g1(mm), dim3(NT), d.spec, d.probe, d.vmode, (long)mm);
       CHK();
       TRY(fft2_batch(d.spec, 2, m, false, d.tw, s));
     }
     LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
     CHK();
     if (subpix) TRY(fft2_batch(d.vp, G, m, true, d.tw, s));
-    LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, d.vp, d.obj, d.probe, d.roi, gf, G, K, m, d.cols, subpix);
+    LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K, m,
+           d.cols, subpix);
     CHK();
     TRY(fft2_batch(d.psi, (long)K * G, m, false, d.tw, s));
     LAUNCH(k_modulus, gch, dim3(NT), d.psi, d.amp, d.amp_scale, gf, G, K, m, d.part);
@@ -941,7 +1022,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     CHK();
     const float2* chi0 = d.psi;
     const int ll_range = std::max(K, d.n_objects);
+    // modes 2..K take one fused pass when no object update runs for them
+    const bool fuse_modes = probe_rec && !d.apply_multimodal_update && K > 1 && K <= 16;
     for (int ll = 1; ll <= ll_range; ++ll) {
+      if (fuse_modes && ll > 1) break;
       const bool obj_rec = obj_rec_iter && (d.apply_multimodal_update || ll <= d.n_objects);
       const int llp = std::min(K, ll);
       const float2* chi = d.psi + (long)(ll - 1) * gm;
@@ -977,18 +1061,16 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         CHK();
       }
       const double lam_mm = (double)(float)(1.1920928955078125e-07 / mm) * mm;
-      if (obj_rec || (probe_rec && ll <= K)) {
-        LAUNCH(k_lsq_solve, dim3(1), dim3(NT), d.part, ch, gf, G, lam_mm, d.beta_object * d.beta_lsq,
-               d.beta_probe * d.beta_lsq, lsq ? 1 : 0, d.step_obj, d.step_probe, d.gscal);
+      if (ll == 1 && (obj_rec_iter || probe_rec)) {  // the group's steps do not change for ll > 1
+        LAUNCH(k_lsq_solve, dim3(1), dim3(1024), d.part, ch, gf, G, lam_mm, d.beta_object * d.beta_lsq,
+               d.beta_probe * d.beta_lsq, lsq ? 1 : 0, d.step_obj, d.step_probe, d.ftot, d.gscal);
         CHK();
       }
       if (obj_rec) LAUNCH(k_axpy_beta, g1(canvas), dim3(NT), d.obj, d.precond, canvas, d.gscal + MLS_S_BO);
       if (probe_rec && ll <= K)
         LAUNCH(k_axpy_beta, g1(mm), dim3(NT), d.probe + (long)(ll - 1) * mm, d.pum, (long)mm, d.gscal + MLS_S_BP);
       CHK();
-      if (iter >= d.pps && ll == 1) {
-        LAUNCH(k_copy_c, g1(gm), dim3(NT), d.tmp, d.op, gm);
-        CHK();
+      if (pos_step && ll == 1) {
         TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
         LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
         CHK();
@@ -1005,20 +1087,29 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
           double* p3 = p2 + nb;                // |V|^2 partials
           LAUNCH(k_vp_resid, gch, dim3(NT), d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m, d.cols, d.vmode, d.gscal,
                  d.part);
-          LAUNCH(k_vp_coef, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, d.gscal);
+          LAUNCH(k_vp_coef, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, d.ftot, d.gscal);
           LAUNCH(k_vp_acc, dim3(nb), dim3(NT), d.acc, d.tmp, G, mm, d.gscal, p2);
           LAUNCH(k_vp_apply, dim3(nb), dim3(NT), d.vmode, d.acc, mm, p2, nb, (double)G / d.n, p3);
           LAUNCH(k_vp_norm, dim3(nb), dim3(NT), d.vmode, mm, p3, nb);
           LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.vmode, chi0, m, d.part);
-          LAUNCH(k_evo_update, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, 1);
+          LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, 1, d.ftot);
           CHK();
         }
         if (d.variable_intensity) {
           LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.probe, chi0, m, d.part);
-          LAUNCH(k_evo_update, dim3(1), dim3(NT), d.part, ch, gf, G, mm, d.evo, d.n, 0);
+          LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, 0, d.ftot);
           CHK();
         }
       }
+    }
+    if (fuse_modes) {
+      if (K <= 4) LAUNCH(k_pum_modes<4>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m, d.cols,
+                         fw, d.gscal);
+      else if (K <= 8) LAUNCH(k_pum_modes<8>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m,
+                              d.cols, fw, d.gscal);
+      else LAUNCH(k_pum_modes<16>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m, d.cols, fw,
+                  d.gscal);
+      CHK();
     }
   }
   if (K > d.n_rec) {

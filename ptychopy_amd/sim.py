@@ -42,6 +42,32 @@ def gaussian_probe(m: int, fwhm: float, defocus_edge: float = 2.0) -> np.ndarray
     return (amp * np.exp(1j * ph)).astype(np.complex64)
 
 
+def hermite_gaussian_modes(m: int, k: int, fwhm: float, powers=(0.6, 0.2, 0.1, 0.06, 0.04),
+                           defocus_edge: float = 2.0) -> np.ndarray:
+    """k incoherent probe modes: Hermite-Gaussian HG_ab (a + b = 0, 1, 1, 2, 2, ...) of the given
+    fundamental FWHM with a common defocus phase, orthogonalised (Gram-Schmidt on the grid) and
+    scaled to the power fractions (total power that of the unit-peak fundamental)."""
+    from numpy.polynomial.hermite import hermval
+    y, x = np.mgrid[0:m, 0:m].astype(np.float64)
+    x, y = x - m / 2, y - m / 2
+    sigma = fwhm / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+    w = np.sqrt(2.0) * sigma
+    g = np.exp(-(x * x + y * y) / (w * w))
+    order = [(a, n - a) for n in range(8) for a in range(n + 1)][:k]
+    ph = np.exp(1j * defocus_edge * (x * x + y * y) / (m / 2) ** 2)
+    modes = []
+    for a, b in order:
+        v = hermval(np.sqrt(2) * x / w, [0] * a + [1]) * hermval(np.sqrt(2) * y / w, [0] * b + [1]) * g * ph
+        for u in modes:
+            v = v - np.vdot(u, v) / np.vdot(u, u) * u
+        modes.append(v)
+    tot = float(np.sum(g * g))
+    pw = np.asarray(powers[:k], np.float64)
+    pw = pw / pw.sum()
+    out = np.array([mo * np.sqrt(pw[i] * tot / np.sum(np.abs(mo) ** 2)) for i, mo in enumerate(modes)])
+    return out.astype(np.complex64)
+
+
 def raster_positions(n_side: int, step: float, rng=None, jitter: float = 0.0, origin=(0.0, 0.0)) -> np.ndarray:
     """(n_side^2, 2) float32 (row, col) positions, row-major scan order, min at ``origin``."""
     i, j = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")

@@ -4,6 +4,10 @@ dm_c1: plain DM at the reference simulation shape -- 30 x 30 raster, 512 x 512 d
 (lambda = 0.248 nm, z = 1 m, 172 um pixels => dx_s = 2.8159 nm), 50 nm steps (17.76 px),
 1088 x 1088 random-field object, 110 nm FWHM Gaussian probe with a 2 rad defocus phase at the
 edge, 1e7 photons per frame with Poisson noise, one probe mode.
+mls_c2: MLs with mixed modes -- 51 x 51 raster, 256 x 256 frames, 100 nm steps = 14 px
+(dx_s = 100/14 nm), object on the MLs canvas (about 1146 x 1146), five orthogonalised
+Hermite-Gaussian modes (60 px fundamental FWHM) with power fractions 0.6/0.2/0.1/0.06/0.04,
+1e5 photons per frame with Poisson noise, delta_p = 0.1.
 epie_c0: plain ePIE -- 12 x 12 raster, 16 px steps with +-1 px jitter, 128 x 128 frames,
 320 x 320 object, 24 px FWHM probe, 1e6 photons per frame.
 """
@@ -14,7 +18,7 @@ import concurrent.futures as cf
 import numpy as np
 
 from . import sim
-from ._driver import cartesian_mesh, sample_pixel_size
+from ._driver import cartesian_mesh, cartesian_mesh_raw, mls_canvas, sample_pixel_size
 from ._engine import Engine
 
 
@@ -55,6 +59,25 @@ def dm_c1(n_side: int = 30, m: int = 512, photons: float = 1e7, sim_size: int = 
     return dict(name="dm_c1", method="DM", pos=pos, canvas=canvas, amp=amp, m=m, n=len(pos), dx_s=dx_s,
                 beam=beam, obj_true=obj[o0[0]:o0[0] + canvas[0], o0[1]:o0[1] + canvas[1]], probe_true=probe,
                 iterations=100)
+
+
+def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwhm: float = 60.0,
+           config_id: int = 2, device: int = 0) -> dict:
+    dx_s = 100e-9 / 14.0
+    step = 100e-9
+    raw = cartesian_mesh_raw(n_side, n_side, step, step, dx_s, m)
+    lo = np.minimum(raw.min(axis=0), 0) if n_side % 32 else raw.min(axis=0)
+    hi = np.maximum(raw.max(axis=0), 0) if n_side % 32 else raw.max(axis=0)
+    canvas, box = mls_canvas(raw, np.array([lo[0], lo[1], hi[0], hi[1]], np.float32), m)
+    rng = sim.rng_for(config_id, 0)
+    obj = sim.random_field_object(canvas, rng, phase_std=0.5)
+    modes = sim.hermite_gaussian_modes(m, k, fwhm)
+    # frames at the MLs ROI frame: origin round(pos + ceil(Np/2 - M/2)) - 1
+    off = np.array([np.ceil(canvas[0] / 2 - m / 2) - 1, np.ceil(canvas[1] / 2 - m / 2) - 1], np.float32)
+    inten = sum(_simulate(obj, modes[i], raw + off, m, device) for i in range(k))
+    amp = _poisson_threaded(inten, photons, np.random.SeedSequence([sim.SEED_BASE + config_id, 1]))
+    return dict(name="mls_c2", method="MLs", pos=raw, canvas=canvas, box=box, amp=amp, m=m, n=len(raw), k=k,
+                dx_s=dx_s, obj_true=obj, probe_true=modes, iterations=100, photons=photons)
 
 
 def epie_c0(m: int = 128, photons: float = 1e6, sim_size: int = 320, config_id: int = 0, device: int = 0) -> dict:
