@@ -196,6 +196,32 @@ typedef struct {
 int ptyk_mls_get_state(ptyk_ctx* ctx, ptyk_mls_state* st);
 int ptyk_mls_set_state(ptyk_ctx* ctx, const ptyk_mls_state* st);
 
+/* ---------------------------------------------------------------- batched scans
+ * Several independent reconstructions (handles on one device, any sizes) advanced together
+ * (no reference counterpart: the reference runs one reconstruction per process).  One DM
+ * iteration of every scan is captured once per flag set into a HIP graph with one branch per
+ * scan and replayed afterwards; each scan's result is bitwise that of ptyk_dm_iteration on it
+ * alone.  The first call (and any call after a scan's buffers changed) runs eagerly or
+ * re-captures.  Calls return after the launch; ptyk_batch_synchronize waits. */
+typedef struct ptyk_batch ptyk_batch;
+int ptyk_batch_create(ptyk_ctx* const* ctxs, int32_t n, ptyk_batch** out);
+void ptyk_batch_destroy(ptyk_batch* batch);
+const char* ptyk_batch_last_error(const ptyk_batch* batch);
+int ptyk_batch_dm_iteration(ptyk_batch* batch, uint32_t flags);
+int ptyk_batch_synchronize(ptyk_batch* batch);
+int ptyk_batch_graph_count(const ptyk_batch* batch); /* graphs captured so far */
+
+/* ---------------------------------------------------------------- object error
+ * Normalised object error against a known object (no reference counterpart; the reference's
+ * d_realSpaceER compares magnitudes only and is never called, utilitiesKernels.cu:714-735):
+ *   e = min_gamma ||gamma O - T||_2 / ||T||_2,  gamma = <O, T> / <O, O>,
+ * over the pixels whose scan illumination sum_j sum_k |P_k|^2 (each frame at its ROI: the MLs
+ * ROIs once ptyk_mls_begin has run, else those of ptyk_set_positions) exceeds frac times its
+ * maximum (0.01 for the benchmark).  truth: rows x cols complex64, kept on the device.
+ * out[4] = {e, Re gamma, Im gamma, pixels used}. */
+int ptyk_set_true_object(ptyk_ctx* ctx, const float* truth);
+int ptyk_object_error(ptyk_ctx* ctx, double frac, double* out);
+
 /* Batched in-place 2D FFT of n frames of M x M complex64 (host buffers), forward
  * exp(-2 pi i kn/M), inverse unnormalised (CudaFFTPlan::getC2CPlan, CudaFFTPlan.cpp:63-101). */
 int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);

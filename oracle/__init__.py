@@ -43,6 +43,7 @@ def lib():
         _lib.orc_simulate.argtypes = [P, i, i, P, P, i64, i, i, i, P]
         _lib.orc_epie_iteration.argtypes = [P, P, i, i, P, P, i, i, i, P, P, P, P, i64, i, i, i, i, i, P]
         _lib.orc_dm_iteration.argtypes = [P, i, i, P, P, P, P, i64, i, i, i, P, P]
+        _lib.orc_object_error.argtypes = [P, P, P, i, i, P, i64, i, i, d, P]
         _lib.orc_mls_roi.argtypes = [P, i, i, i, i, P, P]
         _lib.orc_mls_init.argtypes = [i, i, i, i, i, P, P, P, P, i, P, P, P, P]
         _lib.orc_mls_ortho_modes.argtypes = [P, i, i]
@@ -153,6 +154,18 @@ def dm_iteration(obj, probe, psi, amp, pos, factors, *, phase_constraint: bool =
     lib().orc_dm_iteration(_ptr(o), o.shape[0], o.shape[1], _ptr(p), _ptr(s), _ptr(a), _ptr(ps), n, m,
                            int(phase_constraint), int(update_probe), _ptr(fac), _ptr(sums))
     return o, p, s, fac, sums
+
+
+def object_error(obj, truth, probe, roi, frac: float = 0.01):
+    """(e, gamma, pixels) of the normalised object error over the illuminated region."""
+    o, t = _c128(obj), _c128(truth)
+    p = _c128(probe)
+    p = p.reshape(-1, p.shape[-2], p.shape[-1])
+    r = np.ascontiguousarray(roi, dtype=np.int32)
+    out = np.zeros(4, np.float64)
+    lib().orc_object_error(_ptr(o), _ptr(t), _ptr(p), p.shape[0], p.shape[-1], _ptr(r), r.shape[0], o.shape[0],
+                           o.shape[1], float(frac), _ptr(out))
+    return float(out[0]), complex(out[1], out[2]), int(out[3])
 
 
 # ---------------------------------------------------------------------------- MLs

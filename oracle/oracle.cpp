@@ -268,4 +268,39 @@ void orc_dm_iteration(double* obj, int rows, int cols, double* probe, double* ps
     if (std::fabs(factors[i] - passed[i]) > 1e-3) factors[i] = passed[i];
 }
 
+// Normalised object error against a known object T over the illuminated region (the engine's
+// metric; no reference counterpart): coverage c = sum_j sum_k |P_k|^2 at ROI j; pixels with
+// c > frac max(c); gamma = <O,T>/<O,O>; e = ||gamma O - T|| / ||T||.  out = {e, Re g, Im g, pixels}.
+void orc_object_error(const double* obj, const double* truth, const double* probe, int k, int m, const int32_t* roi,
+                      int64_t n, int rows, int cols, double frac, double* out) {
+  const long canvas = (long)rows * cols, mm = (long)m * m;
+  std::vector<double> pw(mm, 0.0), cov(canvas, 0.0);
+  for (int i = 0; i < k; ++i)
+    for (long q = 0; q < mm; ++q) pw[q] += std::norm(C(probe)[i * mm + q]);
+  for (int64_t j = 0; j < n; ++j)
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) cov[(long)(roi[2 * j] + r) * cols + roi[2 * j + 1] + c] += pw[(long)r * m + c];
+  double mx = 0;
+  for (long q = 0; q < canvas; ++q) mx = std::max(mx, cov[q]);
+  const cd* O = C(obj);
+  const cd* T = C(truth);
+  double so = 0, st = 0, cnt = 0;
+  cd ot = 0;
+  for (long q = 0; q < canvas; ++q)
+    if (cov[q] > frac * mx) {
+      so += std::norm(O[q]);
+      st += std::norm(T[q]);
+      ot += std::conj(O[q]) * T[q];
+      cnt += 1;
+    }
+  const cd g = so > 0 ? ot / so : cd(0);
+  double sd = 0;
+  for (long q = 0; q < canvas; ++q)
+    if (cov[q] > frac * mx) sd += std::norm(g * O[q] - T[q]);
+  out[0] = st > 0 ? std::sqrt(sd / st) : 0.0;
+  out[1] = g.real();
+  out[2] = g.imag();
+  out[3] = cnt;
+}
+
 }  // extern "C"

@@ -35,7 +35,9 @@ EXPORTS = (
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
     "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
-    "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state",
+    "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
+    "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
+    "ptyk_batch_synchronize", "ptyk_batch_graph_count",
 )
 
 TUNE_DM_WAVE = 1
@@ -129,6 +131,16 @@ def load():
     lib.ptyk_mls_iteration.argtypes = [P, i32, P, P, i32, P]
     lib.ptyk_mls_get_state.argtypes = [P, C.POINTER(MlsState)]
     lib.ptyk_mls_set_state.argtypes = [P, C.POINTER(MlsState)]
+    lib.ptyk_set_true_object.argtypes = [P, P]
+    lib.ptyk_batch_create.argtypes = [P, i32, C.POINTER(C.c_void_p)]
+    lib.ptyk_batch_destroy.argtypes = [P]
+    lib.ptyk_batch_destroy.restype = None
+    lib.ptyk_batch_last_error.argtypes = [P]
+    lib.ptyk_batch_last_error.restype = C.c_char_p
+    lib.ptyk_batch_dm_iteration.argtypes = [P, u32]
+    lib.ptyk_batch_synchronize.argtypes = [P]
+    lib.ptyk_batch_graph_count.argtypes = [P]
+    lib.ptyk_object_error.argtypes = [P, C.c_double, P]
     _lib = lib
     return lib
 
@@ -352,3 +364,60 @@ class Engine:
             keep[key] = a
             setattr(st, key, _ptr(a))
         self._chk(self.lib.ptyk_mls_set_state(self.h, C.byref(st)))
+
+    # ---- object error
+    def set_true_object(self, truth):
+        t = np.ascontiguousarray(truth, dtype=np.complex64)
+        assert t.shape == self.obj_shape
+        self._chk(self.lib.ptyk_set_true_object(self.h, _ptr(t)))
+
+    def object_error(self, frac: float = 0.01):
+        """(e, gamma, pixels): min_gamma ||gamma O - T|| / ||T|| over the illuminated region."""
+        out = np.zeros(4, np.float64)
+        self._chk(self.lib.ptyk_object_error(self.h, float(frac), _ptr(out)))
+        return float(out[0]), complex(out[1], out[2]), int(out[3])
+
+
+class Batch:
+    """Several Engines advanced together: one captured HIP graph per DM iteration flag set."""
+
+    def __init__(self, engines):
+        self.lib = load()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = C.c_void_p()
+        rc = self.lib.ptyk_batch_create(arr, len(self.engines), C.byref(h))
+        if rc != PTYK_OK:
+            raise PtykError(rc, self.lib.ptyk_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != PTYK_OK:
+            raise PtykError(rc, self.lib.ptyk_batch_last_error(self.h).decode())
+
+    def dm_iteration(self, flags: int = 0):
+        self._chk(self.lib.ptyk_batch_dm_iteration(self.h, int(flags)))
+
+    def synchronize(self):
+        self._chk(self.lib.ptyk_batch_synchronize(self.h))
+
+    @property
+    def graph_count(self) -> int:
+        return self.lib.ptyk_batch_graph_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ptyk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -57,6 +58,11 @@ struct ptyk_ctx {
   std::vector<int> wave_lo, wave_hi;
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> evpool;
+  // object-error state
+  float2* truth = nullptr;
+  float* cov = nullptr;
+  float* pint = nullptr;
+  double* epart = nullptr;
   // MLs state
   MlsDev mls;
   bool mls_ready = false;
@@ -111,6 +117,8 @@ static void free_mls(ptyk_ctx* c) {
 
 static void free_all(ptyk_ctx* c) {
   free_mls(c);
+  for (void* p : {(void*)c->truth, (void*)c->cov, (void*)c->pint, (void*)c->epart})
+    if (p) hipFree(p);
   void* ptrs[] = {c->tw, c->obj, c->obj_int, c->probe, c->probe_int, c->probe_tmp, c->amp, c->amp_scale,
                   c->amp_sum, c->avg, c->pos_dev, c->roi_dev, c->order_dev, c->scal, c->factors, c->e_psi_old,
                   c->e_psi, c->psi, c->row_ptr, c->row_frames, c->probe_partial, c->obj_partial,
@@ -640,13 +648,9 @@ static int dm_prepare(ptyk_ctx* c) {
   return PTYK_OK;
 }
 
-static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
-  if (!c) return PTYK_EINVAL;
-  if (!c->have_frames || !c->have_pos || !c->have_probe)
-    return fail(c, PTYK_ESTATE, "frames, positions and probe must be set");
-  hipSetDevice(c->cfg.device);
-  int r = dm_prepare(c);
-  if (r) return r;
+// The launches of one DM iteration on the context's stream (no host synchronisation, no
+// allocation: safe inside a stream capture once dm_prepare has run).
+static int dm_launch(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   DmTimer tm;
   if (kernel_ms) {
     if (c->evpool.empty()) {
@@ -722,6 +726,16 @@ static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
     }
   }
   return PTYK_OK;
+}
+
+static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
+  if (!c) return PTYK_EINVAL;
+  if (!c->have_frames || !c->have_pos || !c->have_probe)
+    return fail(c, PTYK_ESTATE, "frames, positions and probe must be set");
+  hipSetDevice(c->cfg.device);
+  int r = dm_prepare(c);
+  if (r) return r;
+  return dm_launch(c, flags, kernel_ms);
 }
 
 int ptyk_dm_iteration(ptyk_ctx* c, uint32_t flags) { return dm_run(c, flags, nullptr); }
@@ -966,6 +980,200 @@ int ptyk_mls_set_state(ptyk_ctx* c, const ptyk_mls_state* st) {
   int r = mls_state_copy(c, st, false);
   if (r == PTYK_OK && st->max_illumination > 0) c->mls.max_illum = (float)st->max_illumination;
   return r;
+}
+
+// ---------------------------------------------------------------- batched scans
+// Several reconstructions advanced together: one DM iteration of every scan is captured once per
+// flag set into a HIP graph with one branch per scan (fork/join through events on a capture
+// stream) and replayed on later calls, so an iteration of the whole batch is one graph launch.
+// Each branch is exactly that scan's own launch sequence, so every scan's result is bitwise the
+// one of its standalone run.
+struct ptyk_batch {
+  std::vector<ptyk_ctx*> ctx;
+  int device = 0;
+  hipStream_t s0 = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  struct Graph {
+    uint32_t flags;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  std::vector<uintptr_t> sig;  // buffers and launch shapes the graphs were captured with
+  std::string err;
+};
+
+static std::vector<uintptr_t> dm_signature(const ptyk_ctx* c) {
+  return {(uintptr_t)c->psi, (uintptr_t)c->row_ptr, (uintptr_t)c->row_frames, (uintptr_t)c->probe_partial,
+          (uintptr_t)c->obj_partial, (uintptr_t)c->probe_partial_sum, (uintptr_t)c->d_obj, (uintptr_t)c->d_probe,
+          (uintptr_t)c->wave_row_ptr, (uintptr_t)c->wave_row_frames, (uintptr_t)c->dm_wave, (uintptr_t)c->dm_waves,
+          (uintptr_t)c->probe_chunks, (uintptr_t)c->frames_per_chunk, (uintptr_t)dm_wave_frames(c)};
+}
+
+static void batch_drop_graphs(ptyk_batch* b) {
+  for (auto& g : b->graphs) hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+}
+
+#define BCHK(b, call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+      return PTYK_EDEVICE;                                                   \
+    }                                                                        \
+  } while (0)
+
+int ptyk_batch_create(ptyk_ctx* const* ctxs, int32_t n, ptyk_batch** out) {
+  if (!ctxs || !out || n <= 0) return fail(nullptr, PTYK_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(nullptr, PTYK_EINVAL, "null context in batch");
+    if (ctxs[i]->cfg.device != ctxs[0]->cfg.device) return fail(nullptr, PTYK_EINVAL, "contexts on different devices");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(nullptr, PTYK_EINVAL, "a context appears twice in the batch");
+  }
+  ptyk_batch* b = new ptyk_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->device = ctxs[0]->cfg.device;
+  hipSetDevice(b->device);
+  hipError_t e = hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  b->join.assign(n, nullptr);
+  for (int i = 0; i < n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    ptyk_batch_destroy(b);
+    return fail(nullptr, PTYK_EDEVICE, std::string("batch setup: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return PTYK_OK;
+}
+
+void ptyk_batch_destroy(ptyk_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->s0) hipStreamSynchronize(b->s0);
+  batch_drop_graphs(b);
+  for (auto e : b->join)
+    if (e) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->done) hipEventDestroy(b->done);
+  if (b->s0) hipStreamDestroy(b->s0);
+  delete b;
+}
+
+const char* ptyk_batch_last_error(const ptyk_batch* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+int ptyk_batch_dm_iteration(ptyk_batch* b, uint32_t flags) {
+  if (!b) return PTYK_EINVAL;
+  hipSetDevice(b->device);
+  // host-side preparation of every scan (allocations, frame lists, first-call factors) runs
+  // eagerly; a scan whose DM state is not started yet runs this iteration eagerly too
+  bool eager = false;
+  for (ptyk_ctx* c : b->ctx) {
+    if (!c->have_frames || !c->have_pos || !c->have_probe) {
+      b->err = "frames, positions and probe must be set in every scan";
+      return PTYK_ESTATE;
+    }
+    if (!c->dm_started || !c->psi) eager = true;
+    int r = dm_prepare(c);
+    if (r) {
+      b->err = c->err;
+      return r;
+    }
+  }
+  if (eager) {
+    for (ptyk_ctx* c : b->ctx) {
+      int r = dm_launch(c, flags, nullptr);
+      if (r) {
+        b->err = c->err;
+        return r;
+      }
+    }
+    return PTYK_OK;
+  }
+  std::vector<uintptr_t> sig;
+  for (ptyk_ctx* c : b->ctx) {
+    auto v = dm_signature(c);
+    sig.insert(sig.end(), v.begin(), v.end());
+  }
+  if (sig != b->sig) {
+    batch_drop_graphs(b);
+    b->sig = sig;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : b->graphs)
+    if (g.flags == flags) exec = g.exec;
+  if (!exec) {
+    // the scans' own streams must be idle before their work moves into the graph
+    for (ptyk_ctx* c : b->ctx) BCHK(b, hipStreamSynchronize(c->stream));
+    BCHK(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeRelaxed));
+    BCHK(b, hipEventRecord(b->fork, b->s0));
+    int rc = PTYK_OK;
+    for (size_t i = 0; i < b->ctx.size() && rc == PTYK_OK; ++i) {
+      ptyk_ctx* c = b->ctx[i];
+      if (hipStreamWaitEvent(c->stream, b->fork, 0) != hipSuccess) rc = PTYK_EDEVICE;
+      if (rc == PTYK_OK) rc = dm_launch(c, flags, nullptr);
+      if (rc == PTYK_OK && hipEventRecord(b->join[i], c->stream) != hipSuccess) rc = PTYK_EDEVICE;
+      if (rc == PTYK_OK && hipStreamWaitEvent(b->s0, b->join[i], 0) != hipSuccess) rc = PTYK_EDEVICE;
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->s0, &graph);
+    if (rc != PTYK_OK || ec != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      b->err = "graph capture failed";
+      return rc != PTYK_OK ? rc : PTYK_EDEVICE;
+    }
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      b->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
+      return PTYK_EDEVICE;
+    }
+    b->graphs.push_back({flags, exec});
+  }
+  BCHK(b, hipGraphLaunch(exec, b->s0));
+  // later calls on any scan's stream see this iteration
+  BCHK(b, hipEventRecord(b->done, b->s0));
+  for (ptyk_ctx* c : b->ctx) BCHK(b, hipStreamWaitEvent(c->stream, b->done, 0));
+  return PTYK_OK;
+}
+
+int ptyk_batch_synchronize(ptyk_batch* b) {
+  if (!b) return PTYK_EINVAL;
+  hipSetDevice(b->device);
+  BCHK(b, hipStreamSynchronize(b->s0));
+  for (ptyk_ctx* c : b->ctx) BCHK(b, hipStreamSynchronize(c->stream));
+  return PTYK_OK;
+}
+
+int ptyk_batch_graph_count(const ptyk_batch* b) { return b ? (int)b->graphs.size() : -1; }
+
+// ---------------------------------------------------------------- object error
+int ptyk_set_true_object(ptyk_ctx* c, const float* truth) {
+  if (!c || !truth) return fail(c, PTYK_EINVAL, "null argument");
+  hipSetDevice(c->cfg.device);
+  const long canvas = (long)c->cfg.obj_rows * c->cfg.obj_cols;
+  if (!c->truth) {
+    HIPCHK(c, dalloc(&c->truth, canvas));
+    HIPCHK(c, dalloc(&c->cov, canvas));
+    HIPCHK(c, dalloc(&c->pint, c->mm));
+    HIPCHK(c, dalloc(&c->epart, 5 * 1024));
+  }
+  return copy_in(c, c->truth, truth, canvas * sizeof(float2));
+}
+
+int ptyk_object_error(ptyk_ctx* c, double frac, double* out) {
+  if (!c || !out) return fail(c, PTYK_EINVAL, "null argument");
+  if (!c->truth) return fail(c, PTYK_ESTATE, "ptyk_set_true_object first");
+  if (!(frac >= 0 && frac < 1)) return fail(c, PTYK_EINVAL, "frac must be in [0, 1)");
+  const int2* roi = c->mls_ready ? c->mls.roi : (c->have_pos ? c->roi_dev : nullptr);
+  if (!roi) return fail(c, PTYK_ESTATE, "no scan positions");
+  hipSetDevice(c->cfg.device);
+  HIPCHK(c, object_error(c->obj, c->truth, c->probe, c->cfg.n_modes, c->cfg.frame_size, roi, c->cfg.n_frames,
+                         c->cfg.obj_rows, c->cfg.obj_cols, frac, c->cov, c->pint, c->epart, 5 * 1024, out, c->stream));
+  return PTYK_OK;
 }
 
 // ---------------------------------------------------------------- utilities

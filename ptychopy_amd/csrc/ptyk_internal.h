@@ -145,6 +145,11 @@ hipError_t mls_init(MlsDev& d, const double* frame_power /* host, n */, hipStrea
 hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s);
 hipError_t mls_roi(MlsDev& d, hipStream_t s);
 
+// ---- normalised object error (ptyk_metrics.hip); out[4] = {e, Re gamma, Im gamma, pixels}.
+hipError_t object_error(const float2* obj, const float2* truth, const float2* probe, int k, int m, const int2* roi,
+                        int n, int rows, int cols, double frac, float* cov, float* pint, double* part, int part_cap,
+                        double* out, hipStream_t s);
+
 // ---- misc kernels (ptyk_misc.hip)
 hipError_t quantize_frames(const float* src_dev, __half* dst, float* scales, long n, int m, hipStream_t s);
 hipError_t dequantize_frames(const __half* src, const float* scales, float* dst, long n, int m, hipStream_t s);
