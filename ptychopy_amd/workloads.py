@@ -8,6 +8,9 @@ mls_c2: MLs with mixed modes -- 51 x 51 raster, 256 x 256 frames, 100 nm steps =
 (dx_s = 100/14 nm), object on the MLs canvas (about 1146 x 1146), five orthogonalised
 Hermite-Gaussian modes (60 px fundamental FWHM) with power fractions 0.6/0.2/0.1/0.06/0.04,
 1e5 photons per frame with Poisson noise, delta_p = 0.1.
+batch_c3: 16 independent DM scans of uneven size -- raster sides drawn from 20..51, detectors
+from {128, 256, 512}^2, 1e5..1e7 photons (log-uniform); object/probe statistics of epie_c0
+scaled to the detector (probe FWHM M/5.33, steps M/8 px with +-1 px jitter).
 epie_c0: plain ePIE -- 12 x 12 raster, 16 px steps with +-1 px jitter, 128 x 128 frames,
 320 x 320 object, 24 px FWHM probe, 1e6 photons per frame.
 """
@@ -78,6 +81,26 @@ def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwh
     amp = _poisson_threaded(inten, photons, np.random.SeedSequence([sim.SEED_BASE + config_id, 1]))
     return dict(name="mls_c2", method="MLs", pos=raw, canvas=canvas, box=box, amp=amp, m=m, n=len(raw), k=k,
                 dx_s=dx_s, obj_true=obj, probe_true=modes, iterations=100, photons=photons)
+
+
+def batch_c3(n_scans: int = 16, config_id: int = 3, device: int = 0, sides=(20, 51), sizes=(128, 256, 512),
+             photons=(1e5, 1e7)) -> list:
+    rng = sim.rng_for(config_id, 0)
+    scans = []
+    for s_i in range(n_scans):
+        side = int(rng.integers(sides[0], sides[1] + 1))
+        m = int(rng.choice(sizes))
+        ph = float(np.exp(rng.uniform(np.log(photons[0]), np.log(photons[1]))))
+        srng = sim.rng_for(config_id, 100 + s_i)
+        pos = sim.raster_positions(side, m / 8.0, srng, jitter=1.0, origin=(1.0, 1.0))
+        canvas = sim.canvas_for(pos, m)
+        obj = sim.random_field_object(canvas, srng, phase_std=0.5)
+        probe = sim.gaussian_probe(m, fwhm=m * 24.0 / 128.0, defocus_edge=2.0)
+        inten = _simulate(obj, probe, pos, m, device)
+        amp = _poisson_threaded(inten, ph, np.random.SeedSequence([sim.SEED_BASE + config_id, 200 + s_i]))
+        scans.append(dict(name=f"batch_c3_{s_i}", method="DM", pos=pos, canvas=canvas, amp=amp, m=m, n=len(pos),
+                          photons=ph, obj_true=obj, probe_true=probe, iterations=50))
+    return scans
 
 
 def epie_c0(m: int = 128, photons: float = 1e6, sim_size: int = 320, config_id: int = 0, device: int = 0) -> dict:

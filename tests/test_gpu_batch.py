@@ -77,3 +77,17 @@ def test_batch_recaptures_after_a_scan_changes(gpu):
         assert np.array_equal(e2.get_object(), ref2)
     e1.close()
     e2.close()
+
+
+def test_autotuner_persists_a_timing_table(gpu, tmp_path):
+    import json
+    from ptychopy_amd import autotune
+    P = make_problem(m=128, n_side=4, seed=16)
+    path = tmp_path / "tune.json"
+    best, times = autotune.tune_dm_wave(P["pos"], 128, P["shape"], iters=1, path=path)
+    assert best in times and set(times) <= set(autotune.DM_WAVE_CANDIDATES) and -1 in times
+    table = json.loads(path.read_text())
+    key = autotune.dm_key(P["n"], 128, P["shape"])
+    assert set(int(c) for c in table[key]) == set(times)
+    # a second call reads the table instead of timing again
+    assert autotune.tune_dm_wave(P["pos"], 128, P["shape"], path=path) == (best, times)
