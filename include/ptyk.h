@@ -222,6 +222,31 @@ int ptyk_batch_graph_count(const ptyk_batch* batch); /* graphs captured so far *
 int ptyk_set_true_object(ptyk_ctx* ctx, const float* truth);
 int ptyk_object_error(ptyk_ctx* ctx, double frac, double* out);
 
+/* ---------------------------------------------------------------- learned warm start
+ * Convolutional encoder-decoder (no reference counterpart): four levels with c, 2c, 4c, 8c
+ * channels (c = base_channels, 32 for the benchmark), 3x3 convolutions with ReLU, 2x2 max
+ * pooling, nearest upsampling with skip concatenation, a final 1x1 convolution to two outputs
+ * (object-patch amplitude and phase).  Convolutions are implicit GEMMs on the matrix cores
+ * (BF16 operands, FP32 accumulation); master weights FP32 with Adam (0.9, 0.999, 1e-8).
+ * Parameters: per layer W[cout][cin][k][k] then b[cout], layers in the order
+ *   c<-1, c<-c, 2c<-c, 2c<-2c, 4c<-2c, 4c<-4c, 8c<-4c, 8c<-8c, 4c<-12c, 4c<-4c, 2c<-6c, 2c<-2c,
+ *   c<-3c, c<-c (3x3), 2<-c (1x1).
+ * Samples: inputs 1 x S x S, targets 2 x S x S (float32 on the host, stored BF16 on the device). */
+typedef struct ptyk_net ptyk_net;
+int ptyk_net_create(int32_t device, int32_t size, int32_t base_channels, int32_t batch, uint64_t seed, ptyk_net** out);
+void ptyk_net_destroy(ptyk_net* net);
+const char* ptyk_net_last_error(const ptyk_net* net);
+int64_t ptyk_net_num_params(const ptyk_net* net);
+int ptyk_net_get_params(ptyk_net* net, float* out);
+int ptyk_net_set_params(ptyk_net* net, const float* in); /* also resets the Adam moments */
+int ptyk_net_set_data(ptyk_net* net, const float* inputs, const float* targets, int64_t n);
+/* Adam steps over stored samples order[0..count) (count a multiple of the batch). */
+int ptyk_net_train(ptyk_net* net, const int64_t* order, int64_t count, float lr, double* mean_loss);
+int ptyk_net_evaluate(ptyk_net* net, const int64_t* order, int64_t count, double* mean_loss);
+int ptyk_net_predict(ptyk_net* net, const float* inputs, int64_t n, float* outputs);
+/* Loss (mean squared error) and its parameter gradient for one batch. */
+int ptyk_net_gradient(ptyk_net* net, const float* inputs, const float* targets, double* loss, float* grad);
+
 /* Batched in-place 2D FFT of n frames of M x M complex64 (host buffers), forward
  * exp(-2 pi i kn/M), inverse unnormalised (CudaFFTPlan::getC2CPlan, CudaFFTPlan.cpp:63-101). */
 int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);

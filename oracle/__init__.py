@@ -43,6 +43,10 @@ def lib():
         _lib.orc_simulate.argtypes = [P, i, i, P, P, i64, i, i, i, P]
         _lib.orc_epie_iteration.argtypes = [P, P, i, i, P, P, i, i, i, P, P, P, P, i64, i, i, i, i, i, P]
         _lib.orc_dm_iteration.argtypes = [P, i, i, P, P, P, P, i64, i, i, i, P, P]
+        _lib.orc_nn_num_params.argtypes = [i]
+        _lib.orc_nn_num_params.restype = i64
+        _lib.orc_nn_forward_backward.argtypes = [P, i, i, i, P, P, i, P, P, P]
+        _lib.orc_nn_conv2d.argtypes = [P, P, P, i, i, i, i, i, i, P]
         _lib.orc_object_error.argtypes = [P, P, P, i, i, P, i64, i, i, d, P]
         _lib.orc_mls_roi.argtypes = [P, i, i, i, i, P, P]
         _lib.orc_mls_init.argtypes = [i, i, i, i, i, P, P, P, P, i, P, P, P, P]
@@ -285,3 +289,33 @@ class MlsState:
                                 _ptr(self.pos_o), _ptr(self.roi), _ptr(self.sub), _ptr(self.beta_obj),
                                 _ptr(self.beta_probe), _ptr(self.amp), _ptr(gf), _ptr(go), len(groups),
                                 _ptr(self.ferr))
+
+
+# ---------------------------------------------------------------------------- warm-start network
+def nn_num_params(c: int) -> int:
+    return int(lib().orc_nn_num_params(c))
+
+
+def nn_forward_backward(params, x, t, c: int, emulate_bf16: bool = False, grad: bool = True):
+    """(out, loss, grad) of the encoder-decoder; x: batch x s x s, t: batch x 2 x s x s."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xa = np.ascontiguousarray(x, dtype=np.float32)
+    ta = np.ascontiguousarray(t, dtype=np.float32)
+    b, s = xa.shape[0], xa.shape[-1]
+    out = np.zeros((b, 2, s, s), np.float64)
+    loss = C.c_double(0.0)
+    g = np.zeros(p.size, np.float64) if grad else None
+    lib().orc_nn_forward_backward(_ptr(p), s, c, b, _ptr(xa), _ptr(ta), int(emulate_bf16), _ptr(out),
+                                  C.byref(loss), None if g is None else _ptr(g))
+    return out, loss.value, g
+
+
+def nn_conv2d(x, w, b, relu: bool = False):
+    xa = _f64(x)
+    wa = _f64(w)
+    ba = _f64(b)
+    n, cin, h, _ = xa.shape
+    cout, _, ks, _ = wa.shape
+    y = np.zeros((n, cout, h, h), np.float64)
+    lib().orc_nn_conv2d(_ptr(xa), _ptr(wa), _ptr(ba), n, cin, cout, ks, h, int(relu), _ptr(y))
+    return y

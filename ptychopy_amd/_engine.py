@@ -37,7 +37,9 @@ EXPORTS = (
     "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
     "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
-    "ptyk_batch_synchronize", "ptyk_batch_graph_count",
+    "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
+    "ptyk_net_last_error", "ptyk_net_num_params", "ptyk_net_get_params", "ptyk_net_set_params", "ptyk_net_set_data",
+    "ptyk_net_train", "ptyk_net_evaluate", "ptyk_net_predict", "ptyk_net_gradient",
 )
 
 TUNE_DM_WAVE = 1
@@ -141,6 +143,20 @@ def load():
     lib.ptyk_batch_synchronize.argtypes = [P]
     lib.ptyk_batch_graph_count.argtypes = [P]
     lib.ptyk_object_error.argtypes = [P, C.c_double, P]
+    lib.ptyk_net_create.argtypes = [i32, i32, i32, i32, C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.ptyk_net_destroy.argtypes = [P]
+    lib.ptyk_net_destroy.restype = None
+    lib.ptyk_net_last_error.argtypes = [P]
+    lib.ptyk_net_last_error.restype = C.c_char_p
+    lib.ptyk_net_num_params.argtypes = [P]
+    lib.ptyk_net_num_params.restype = i64
+    lib.ptyk_net_get_params.argtypes = [P, P]
+    lib.ptyk_net_set_params.argtypes = [P, P]
+    lib.ptyk_net_set_data.argtypes = [P, P, P, i64]
+    lib.ptyk_net_train.argtypes = [P, P, i64, f, C.POINTER(C.c_double)]
+    lib.ptyk_net_evaluate.argtypes = [P, P, i64, C.POINTER(C.c_double)]
+    lib.ptyk_net_predict.argtypes = [P, P, i64, P]
+    lib.ptyk_net_gradient.argtypes = [P, P, P, C.POINTER(C.c_double), P]
     _lib = lib
     return lib
 

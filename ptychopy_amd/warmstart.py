@@ -1,0 +1,159 @@
+"""Learned warm start for MLs: network binding, training data, stitching.
+
+The network (include/ptyk.h, ``ptyk_net_*``) maps a log-scaled diffraction amplitude to the
+amplitude and phase of the object patch under that frame; ``stitch`` averages the predicted
+patches into an initial object, weighting each by the scan's probe intensity.  The plain
+iterative solver then refines it (and remains the fallback when no network is given).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _engine, sim
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Net:
+    """Encoder-decoder on the GPU: size x size inputs, base channel width c, fixed batch."""
+
+    def __init__(self, size: int = 256, c: int = 32, batch: int = 8, seed: int = 0, device: int = 0):
+        self.L = _engine.load()
+        self.size, self.c, self.batch = int(size), int(c), int(batch)
+        h = C.c_void_p()
+        rc = self.L.ptyk_net_create(device, self.size, self.c, self.batch, seed, C.byref(h))
+        if rc != 0:
+            raise _engine.PtykError(rc, self.L.ptyk_net_last_error(None).decode())
+        self.h = h
+        self.n_params = int(self.L.ptyk_net_num_params(h))
+        self.n_data = 0
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise _engine.PtykError(rc, self.L.ptyk_net_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ptyk_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        self._chk(self.L.ptyk_net_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.n_params
+        self._chk(self.L.ptyk_net_set_params(self.h, _ptr(a)))
+
+    def set_data(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        n = x.shape[0]
+        assert x.shape == (n, self.size, self.size) and t.shape == (n, 2, self.size, self.size)
+        self._chk(self.L.ptyk_net_set_data(self.h, _ptr(x), _ptr(t), n))
+        self.n_data = n
+
+    def train(self, order, lr: float) -> float:
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        loss = C.c_double(0.0)
+        self._chk(self.L.ptyk_net_train(self.h, _ptr(o), o.size, float(lr), C.byref(loss)))
+        return loss.value
+
+    def evaluate(self, order) -> float:
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        loss = C.c_double(0.0)
+        self._chk(self.L.ptyk_net_evaluate(self.h, _ptr(o), o.size, C.byref(loss)))
+        return loss.value
+
+    def predict(self, inputs) -> np.ndarray:
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        out = np.empty((x.shape[0], 2, self.size, self.size), np.float32)
+        self._chk(self.L.ptyk_net_predict(self.h, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def gradient(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        assert x.shape[0] == self.batch and t.shape[0] == self.batch
+        loss = C.c_double(0.0)
+        g = np.empty(self.n_params, np.float32)
+        self._chk(self.L.ptyk_net_gradient(self.h, _ptr(x), _ptr(t), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+
+def preprocess(amp) -> np.ndarray:
+    """log(1 + A) scaled per frame to a maximum of 1 (the network's input)."""
+    a = np.log1p(np.maximum(np.asarray(amp, np.float32), 0))
+    mx = a.reshape(a.shape[0], -1).max(axis=1)
+    return (a / np.maximum(mx, 1e-12)[:, None, None]).astype(np.float32)
+
+
+def patch_targets(obj, roi, m: int) -> np.ndarray:
+    """(n, 2, m, m): amplitude and phase of the object under each ROI (integer origins)."""
+    out = np.empty((len(roi), 2, m, m), np.float32)
+    for j, (r0, c0) in enumerate(roi):
+        p = obj[r0:r0 + m, c0:c0 + m]
+        out[j, 0] = np.abs(p)
+        out[j, 1] = np.angle(p)
+    return out
+
+
+def training_set(n_frames: int, modes, photons: float = 1e5, frames_per_object: int = 200, seed: int = 0,
+                 phase_std=(0.2, 1.0), device: int = 0):
+    """Frames the engine's simulator forms from random-field objects (phase std drawn from
+    phase_std) under the given incoherent probe modes, Poisson-sampled; returns
+    (inputs, targets): preprocessed amplitudes and object-patch (amplitude, phase)."""
+    k, m, _ = modes.shape
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    side = 4 * m
+    with _engine.Engine(1, m, (m, m), device=device) as e:
+        done = 0
+        while done < n_frames:
+            cnt = min(frames_per_object, n_frames - done)
+            obj = sim.random_field_object((side, side), rng, phase_std=float(rng.uniform(*phase_std)))
+            roi = rng.integers(0, side - m, size=(cnt, 2))
+            pos = roi.astype(np.float32)
+            inten = sum(e.simulate(obj, modes[i], pos, squared=True) for i in range(k))
+            tot = inten.reshape(cnt, -1).sum(axis=1)
+            counts = rng.poisson(inten * (photons / np.maximum(tot, 1e-30))[:, None, None])
+            xs.append(preprocess(np.sqrt(counts)))
+            ts.append(patch_targets(obj, roi, m))
+            done += cnt
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def stitch(pred, roi, weight, canvas, fill=1.0 + 0j) -> np.ndarray:
+    """Average of the predicted patches amp * exp(i phase) at their ROIs, weighted by `weight`
+    (M x M, e.g. the total probe intensity); pixels with no weight take `fill`."""
+    m = pred.shape[-1]
+    num = np.zeros(canvas, np.complex128)
+    den = np.zeros(canvas, np.float64)
+    w = np.asarray(weight, np.float64)
+    for j, (r0, c0) in enumerate(roi):
+        patch = pred[j, 0].astype(np.float64) * np.exp(1j * pred[j, 1].astype(np.float64))
+        num[r0:r0 + m, c0:c0 + m] += w * patch
+        den[r0:r0 + m, c0:c0 + m] += w
+    thr = 1e-3 * den.max()
+    out = np.full(canvas, fill, np.complex128)
+    ok = den > thr
+    out[ok] = num[ok] / den[ok]
+    return out.astype(np.complex64)
