@@ -1,13 +1,20 @@
 #!/usr/bin/env python
-"""Benchmark: DM reconstruction throughput, diffraction frames/s per iteration.
+"""Benchmark: reconstruction throughput, diffraction frames/s per iteration.
 
-Workload (the reference simulation shape): 30 x 30 raster scan, 512 x 512 detector
-(lambda 0.248 nm, z 1 m, 172 um pixels => 2.82 nm object pixels, 50 nm = 17.76 px steps),
-1088 x 1088 random-field object (1027 x 1027 reconstruction canvas), 110 nm Gaussian probe,
-1e7 photons/frame Poisson, one probe mode.  A step is one full DM iteration over all 900
-frames with inputs resident in HBM.  Prints one JSON line.
+Default workload `dm_c1` (the reference simulation shape): DM on a 30 x 30 raster scan,
+512 x 512 detector (lambda 0.248 nm, z 1 m, 172 um pixels => 2.82 nm object pixels, 50 nm =
+17.76 px steps), 1088 x 1088 random-field object (1027 x 1027 reconstruction canvas), 110 nm
+Gaussian probe, 1e7 photons/frame Poisson, one probe mode.  A step is one full DM iteration
+over all 900 frames with inputs resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-frames S | --no-cpu]
+`--workload mls_c2`: MLs on a 51 x 51 raster (100 nm = 14 px steps), 256 x 256 frames, five
+incoherent Hermite-Gaussian probe modes (power 0.6/0.2/0.1/0.06/0.04), 1e5 photons/frame,
+FP16 amplitudes; a step is one MLs iteration (101 groups of ~26 frames, position refinement
+on from iteration 20).
+
+Prints one JSON line.
+
+  python bench.py [--gpus 1] [--workload dm_c1|mls_c2] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -86,11 +93,102 @@ def cpu_baseline(wl: dict, frames: int) -> dict:
                       f"frames, 512x512, same 1027x1027 canvas; {dt:.1f} s on {cpu_model()}"}
 
 
+def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: int) -> dict:
+    """FP64 oracle MLs iteration `it` restricted to the first two groups of the same workload."""
+    import oracle
+    st = oracle.MlsState(obj0, probe0, vmode, evo1, W["pos"], amp_deq)
+    st.init((amp_deq.astype(np.float64) ** 2).sum(axis=(1, 2)))
+    prm = oracle.mls_params(W["box"], iter=it)
+    sample = groups[:2]
+    frames = int(sum(len(g) for g in sample))
+    t0 = time.perf_counter()
+    st.iteration(prm, sample)
+    dt = time.perf_counter() - t0
+    return {"value": frames / dt, "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"FP64 oracle MLs iteration {it} over its first 2 groups ({frames} of {W['n']} frames; "
+                      f"whole-canvas steps included), 256x256, 5 modes; {dt:.1f} s on {cpu_model()}"}
+
+
+def run_mls(args) -> dict:
+    import torch
+    from ptychopy_amd import workloads, _driver
+    from ptychopy_amd import _engine as E
+
+    t_gen = time.perf_counter()
+    W = workloads.mls_c2()
+    t_gen = time.perf_counter() - t_gen
+    n, m, k = W["n"], W["m"], W["k"]
+    rows, cols = W["canvas"]
+    rng = np.random.default_rng(20261204)
+    e = E.Engine(n, m, (rows, cols), n_modes=k)
+    e.set_frames(W["amp"])
+    amp_deq = e.get_frames()
+    obj0 = np.ones((rows, cols), np.complex64)
+    e.set_object(obj0)
+    e.set_probe(W["probe_true"])
+    vmode = (rng.uniform(0, 1, (m, m)) * np.exp(2j * np.pi * rng.uniform(0, 1, (m, m)))).astype(np.complex64)
+    evo1 = 1e-6 * rng.standard_normal(n)
+    fp = (amp_deq.astype(np.float64) ** 2).sum(axis=(1, 2))
+    e.mls_begin(E.mls_config(W["box"]), W["pos"], vmode, evo1, fp)
+    parts = _driver.mls_partitions(n, 100, 1)
+    n_groups = len(parts[0])
+
+    def groups(i):
+        p = parts[i % len(parts)]
+        return [p[j] for j in rng.permutation(len(p))]
+
+    it = 0
+    for _ in range(args.warmup):
+        it += 1
+        e.mls_iteration(it, groups(it))
+    e.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it += 1
+        e.mls_iteration(it, groups(it))
+    e.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    bpf = workloads.mls_bytes_per_frame(m, rows * cols, n, n_groups)
+    fpf = workloads.mls_flops_per_frame(m, k)
+    gbs = bpf * n / (ms * 1e-3) / 1e9
+    tfs = fpf * n / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "diffraction frames/s per iteration",
+        "value": n / (ms * 1e-3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded random-field object, 5 Hermite-Gaussian modes, Poisson 1e5 photons/frame)",
+        "config": {"workload": "MLs 51x51 raster, 256x256 frames, 5 probe modes, FP16 amplitudes, "
+                               f"{n_groups} groups, position refinement from iteration 20",
+                   "frames": n, "frame": m, "canvas": [rows, cols], "modes": k},
+        "roofline": {"bound": "hbm", "kernel": "iteration", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_frame": bpf},
+        "flop_roofline": {"flops_per_frame": fpf, "achieved": tfs, "peak": 157.3, "unit": "TFLOP/s",
+                          "frac": tfs / 157.3},
+        "setup_s": t_gen,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = mls_cpu_baseline(W, amp_deq, W["probe_true"], obj0, vmode, evo1, groups(it + 1),
+                                               max(it + 1, 21))
+    e.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=("dm_c1", "mls_c2"), default="dm_c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 for dm_c1, 10 for mls_c2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 10 for dm_c1, 20 for mls_c2)")
     ap.add_argument("--cpu-frames", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
@@ -103,6 +201,13 @@ def main() -> int:
               "(the DM frame loop would shard by scan region with one object all-reduce per iteration; "
               "that path is not built)", file=sys.stderr)
         return 2
+    if args.workload == "mls_c2":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 20 if args.warmup is None else args.warmup
+        print(json.dumps(run_mls(args)))
+        return 0
+    args.steps = 20 if args.steps is None else args.steps
+    args.warmup = 10 if args.warmup is None else args.warmup
 
     import torch
     from ptychopy_amd import workloads

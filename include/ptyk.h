@@ -239,7 +239,13 @@ const char* ptyk_net_last_error(const ptyk_net* net);
 int64_t ptyk_net_num_params(const ptyk_net* net);
 int ptyk_net_get_params(ptyk_net* net, float* out);
 int ptyk_net_set_params(ptyk_net* net, const float* in); /* also resets the Adam moments */
+/* Loss = sum_p w(p) (y - t)^2 / (batch * 2 * sum_p w(p)); w is S x S, shared by both output
+ * channels and every sample (NULL: uniform). */
+int ptyk_net_set_loss_weight(ptyk_net* net, const float* weight);
 int ptyk_net_set_data(ptyk_net* net, const float* inputs, const float* targets, int64_t n);
+/* Streaming form of ptyk_net_set_data: room for n samples, then slices [offset, offset + count). */
+int ptyk_net_reserve_data(ptyk_net* net, int64_t n);
+int ptyk_net_put_data(ptyk_net* net, int64_t offset, const float* inputs, const float* targets, int64_t count);
 /* Adam steps over stored samples order[0..count) (count a multiple of the batch). */
 int ptyk_net_train(ptyk_net* net, const int64_t* order, int64_t count, float lr, double* mean_loss);
 int ptyk_net_evaluate(ptyk_net* net, const int64_t* order, int64_t count, double* mean_loss);

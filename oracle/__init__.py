@@ -45,7 +45,7 @@ def lib():
         _lib.orc_dm_iteration.argtypes = [P, i, i, P, P, P, P, i64, i, i, i, P, P]
         _lib.orc_nn_num_params.argtypes = [i]
         _lib.orc_nn_num_params.restype = i64
-        _lib.orc_nn_forward_backward.argtypes = [P, i, i, i, P, P, i, P, P, P]
+        _lib.orc_nn_forward_backward.argtypes = [P, i, i, i, P, P, P, i, P, P, P]
         _lib.orc_nn_conv2d.argtypes = [P, P, P, i, i, i, i, i, i, P]
         _lib.orc_object_error.argtypes = [P, P, P, i, i, P, i64, i, i, d, P]
         _lib.orc_mls_roi.argtypes = [P, i, i, i, i, P, P]
@@ -296,17 +296,19 @@ def nn_num_params(c: int) -> int:
     return int(lib().orc_nn_num_params(c))
 
 
-def nn_forward_backward(params, x, t, c: int, emulate_bf16: bool = False, grad: bool = True):
-    """(out, loss, grad) of the encoder-decoder; x: batch x s x s, t: batch x 2 x s x s."""
+def nn_forward_backward(params, x, t, c: int, emulate_bf16: bool = False, grad: bool = True, weight=None):
+    """(out, loss, grad) of the encoder-decoder; x: batch x s x s, t: batch x 2 x s x s;
+    weight: optional s x s per-pixel loss weight."""
     p = np.ascontiguousarray(params, dtype=np.float32)
+    wa = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
     xa = np.ascontiguousarray(x, dtype=np.float32)
     ta = np.ascontiguousarray(t, dtype=np.float32)
     b, s = xa.shape[0], xa.shape[-1]
     out = np.zeros((b, 2, s, s), np.float64)
     loss = C.c_double(0.0)
     g = np.zeros(p.size, np.float64) if grad else None
-    lib().orc_nn_forward_backward(_ptr(p), s, c, b, _ptr(xa), _ptr(ta), int(emulate_bf16), _ptr(out),
-                                  C.byref(loss), None if g is None else _ptr(g))
+    lib().orc_nn_forward_backward(_ptr(p), s, c, b, _ptr(xa), _ptr(ta), None if wa is None else _ptr(wa),
+                                  int(emulate_bf16), _ptr(out), C.byref(loss), None if g is None else _ptr(g))
     return out, loss.value, g
 
 

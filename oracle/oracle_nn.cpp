@@ -186,9 +186,11 @@ int64_t orc_nn_num_params(int c) {
   return np;
 }
 
-// Forward (out: batch x 2 x s x s), loss, and (grad != NULL) the parameter gradient.
+// Forward (out: batch x 2 x s x s), loss, and (grad != NULL) the parameter gradient.  The loss is
+// the weighted mean squared error sum w(p) (y - t)^2 / (batch * 2 * sum_p w(p)), w an s x s
+// per-pixel weight shared by both output channels and every sample (NULL: w = 1).
 void orc_nn_forward_backward(const float* params, int s, int c, int batch, const float* xin, const float* tgt,
-                             int emulate_bf16, double* out, double* loss, double* grad) {
+                             const float* weight, int emulate_bf16, double* out, double* loss, double* grad) {
   Net net;
   long np;
   net.L = layers_for(c, &np);
@@ -211,17 +213,21 @@ void orc_nn_forward_backward(const float* params, int s, int c, int batch, const
   T4 cat0 = up_cat(d1, e0);
   T4 a6 = conv(net, 12, cat0), d0 = conv(net, 13, a6);
   T4 y = conv(net, 14, d0);
-  const size_t cnt = y.v.size();
+  const size_t cnt = y.v.size(), ss = (size_t)s * s;
+  double wsum = 0;
+  for (size_t p = 0; p < ss; ++p) wsum += weight ? weight[p] : 1.0;
+  const double norm = (double)batch * 2 * wsum;
   double l = 0;
   T4 gy(batch, 2, s);
   for (size_t i = 0; i < cnt; ++i) {
     const double t = net.emu ? bf(tgt[i]) : tgt[i];
+    const double w = weight ? weight[i % ss] : 1.0;
     const double d = y.v[i] - t;
-    l += d * d;
-    gy.v[i] = 2.0 * d / cnt;
+    l += w * d * d;
+    gy.v[i] = 2.0 * w * d / norm;
     if (out) out[i] = y.v[i];
   }
-  *loss = l / cnt;
+  *loss = l / norm;
   if (!grad) return;
   std::vector<double> g(np, 0.0);
   auto split = [](const T4& cat, int c_lo, T4* lo, T4* skip) {

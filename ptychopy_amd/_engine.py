@@ -38,8 +38,9 @@ EXPORTS = (
     "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
     "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
-    "ptyk_net_last_error", "ptyk_net_num_params", "ptyk_net_get_params", "ptyk_net_set_params", "ptyk_net_set_data",
-    "ptyk_net_train", "ptyk_net_evaluate", "ptyk_net_predict", "ptyk_net_gradient",
+    "ptyk_net_last_error", "ptyk_net_num_params", "ptyk_net_get_params", "ptyk_net_set_params",
+    "ptyk_net_set_loss_weight", "ptyk_net_set_data", "ptyk_net_reserve_data", "ptyk_net_put_data", "ptyk_net_train",
+    "ptyk_net_evaluate", "ptyk_net_predict", "ptyk_net_gradient",
 )
 
 TUNE_DM_WAVE = 1
@@ -153,6 +154,9 @@ def load():
     lib.ptyk_net_get_params.argtypes = [P, P]
     lib.ptyk_net_set_params.argtypes = [P, P]
     lib.ptyk_net_set_data.argtypes = [P, P, P, i64]
+    lib.ptyk_net_set_loss_weight.argtypes = [P, P]
+    lib.ptyk_net_reserve_data.argtypes = [P, i64]
+    lib.ptyk_net_put_data.argtypes = [P, i64, P, P, i64]
     lib.ptyk_net_train.argtypes = [P, P, i64, f, C.POINTER(C.c_double)]
     lib.ptyk_net_evaluate.argtypes = [P, P, i64, C.POINTER(C.c_double)]
     lib.ptyk_net_predict.argtypes = [P, P, i64, P]

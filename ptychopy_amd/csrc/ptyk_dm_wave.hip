@@ -28,27 +28,59 @@ struct WaveCfg {
   static constexpr int BPF = R0 * CPB;                      // 64 x 64 blocks per frame
 };
 
+// Global <-> LDS traffic of the row passes moves two adjacent complex values (16 B) per lane, in
+// fully unrolled loops so that every load of a thread is in flight before the first LDS store.
+template <int M>
+struct RowIO {
+  using C = WaveCfg<M>;
+  static constexpr int PAIRS = C::ROWS * M / 2 / PTYK_NT;  // float4 per thread
+  static_assert((C::ROWS * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
+  // element e (even) of the workgroup's tile -> (frame, frame row, column); false past the end
+  __device__ __forceinline__ static bool locate(int e, long f0, long n_groups, long* f, int* r, int* c) {
+    const int lr = e / M;
+    *c = e % M;
+    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
+    *f = f0 + G / C::L;
+    *r = (int)(G % C::L) + C::L * (lr % C::R0);
+    return G < n_groups;
+  }
+};
+
 template <int M>
 __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi, const float2* __restrict__ obj,
                                                         int obj_cols, const float2* __restrict__ probe,
                                                         const int2* __restrict__ roi, long f0, long n_groups,
                                                         const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
+  using IO = RowIO<M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
-  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
-    const int lr = e / M, c = e % M;
-    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
-    float2 v = make_float2(0.f, 0.f);
-    if (G < n_groups) {
-      const long f = f0 + G / C::L;
-      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
+  float4 sv[IO::PAIRS], pv[IO::PAIRS];
+  float2 o0[IO::PAIRS], o1[IO::PAIRS];
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c;
+    if (IO::locate(e, f0, n_groups, &f, &r, &c)) {
       const int2 o = roi[f];
-      const float2 po = cmul(probe[r * M + c], obj[(long)(o.x + r) * obj_cols + (o.y + c)]);
-      const float2 s = psi[(f * M + r) * M + c];
-      v = make_float2(2.f * po.x - s.x, 2.f * po.y - s.y);
+      const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
+      sv[i] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
+      pv[i] = *reinterpret_cast<const float4*>(probe + r * M + c);
+      o0[i] = orow[0];
+      o1[i] = orow[1];
+    } else {
+      sv[i] = pv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      o0[i] = o1[i] = make_float2(0.f, 0.f);
     }
-    lds[e] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const float2 a = cmul(make_float2(pv[i].x, pv[i].y), o0[i]);
+    const float2 b = cmul(make_float2(pv[i].z, pv[i].w), o1[i]);
+    *reinterpret_cast<float4*>(&lds[e]) =
+        make_float4(2.f * a.x - sv[i].x, 2.f * a.y - sv[i].y, 2.f * b.x - sv[i].z, 2.f * b.y - sv[i].w);
   }
   __syncthreads();
   fft_lds<M, false, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
@@ -65,14 +97,13 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
     for (int q = 0; q < C::R0; ++q) lds[(g * C::R0 + q) * M + c] = v[q];
   }
   __syncthreads();
-  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
-    const int lr = e / M, c = e % M;
-    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
-    if (G < n_groups) {
-      const long f = f0 + G / C::L;
-      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
-      psi[(f * M + r) * M + c] = lds[e];
-    }
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c;
+    if (IO::locate(e, f0, n_groups, &f, &r, &c))
+      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) = *reinterpret_cast<const float4*>(&lds[e]);
   }
 }
 
@@ -81,53 +112,83 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_b(float2* __restrict__ psi
                                                         const float* __restrict__ amp_scale, long f0,
                                                         const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
+  constexpr int PAIRS = 64 * 64 / 2 / PTYK_NT;
   __shared__ float2 lds[64 * 64];
   const int tid = threadIdx.x;
   const long f = f0 + blockIdx.x / C::BPF;
   const int rem = blockIdx.x % C::BPF;
   const int k2 = rem / C::CPB, c0 = (rem % C::CPB) * 64;
   float2* base = psi + (f * M + 64 * k2) * M + c0;
-  for (int e = tid; e < 64 * 64; e += PTYK_NT) lds[e] = base[(e >> 6) * M + (e & 63)];
-  __syncthreads();
-  fft_lds<64, false, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      v[i] = *reinterpret_cast<const float4*>(base + (e >> 6) * M + (e & 63));
+    }
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
+  }
+  // amplitudes of this block's frequency rows, fetched before the transform so they are in
+  // flight while it runs
   const float inv_m2 = 1.f / (float)(M * M);
   const float sc = amp_scale[f];
   const __half* a = amp + f * M * M + c0;
-  for (int e = tid; e < 64 * 64; e += PTYK_NT) {
-    const int k1 = e >> 6, c = e & 63;
-    const float A = __half2float(a[(long)(C::R0 * k1 + k2) * M + c]) * sc;
-    const float2 ph = lds[e];
-    const float m2 = cabs2(ph);
-    float2 d;
-    if (m2 > 0.f) {
-      const float k = (A * rsqrtf(m2) - 1.f) * inv_m2;
-      d = make_float2(ph.x * k, ph.y * k);
-    } else {
-      d = make_float2(A * inv_m2, 0.f);
+  __half2 av[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    av[i] = *reinterpret_cast<const __half2*>(a + (long)(C::R0 * (e >> 6) + k2) * M + (e & 63));
+  }
+  __syncthreads();
+  fft_lds<64, false, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const float2 amp2 = __half22float2(av[i]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float A = (h ? amp2.y : amp2.x) * sc;
+      const float2 ph = lds[e + h];
+      const float m2 = cabs2(ph);
+      float2 d;
+      if (m2 > 0.f) {
+        const float k = (A * rsqrtf(m2) - 1.f) * inv_m2;
+        d = make_float2(ph.x * k, ph.y * k);
+      } else {
+        d = make_float2(A * inv_m2, 0.f);
+      }
+      lds[e + h] = d;
     }
-    lds[e] = d;
   }
   __syncthreads();
   fft_lds<64, true, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
-  for (int e = tid; e < 64 * 64; e += PTYK_NT) base[(e >> 6) * M + (e & 63)] = lds[e];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    *reinterpret_cast<float4*>(base + (e >> 6) * M + (e & 63)) = *reinterpret_cast<const float4*>(&lds[e]);
+  }
 }
 
 template <int M>
 __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi, long f0, long n_groups,
                                                         const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
+  using IO = RowIO<M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
-  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
-    const int lr = e / M, c = e % M;
-    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
-    float2 v = make_float2(0.f, 0.f);
-    if (G < n_groups) {
-      const long f = f0 + G / C::L;
-      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
-      v = psi[(f * M + r) * M + c];
+  {
+    float4 v[IO::PAIRS];
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      long f;
+      int r, c;
+      v[i] = IO::locate(e, f0, n_groups, &f, &r, &c) ? *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c)
+                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    lds[e] = v;
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
   }
   __syncthreads();
   for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
@@ -144,14 +205,13 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi
   }
   __syncthreads();
   fft_lds<M, true, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
-  for (int e = tid; e < C::ROWS * M; e += PTYK_NT) {
-    const int lr = e / M, c = e % M;
-    const long G = (long)blockIdx.x * C::GPW + lr / C::R0;
-    if (G < n_groups) {
-      const long f = f0 + G / C::L;
-      const int r = (int)(G % C::L) + C::L * (lr % C::R0);
-      psi[(f * M + r) * M + c] = lds[e];
-    }
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c;
+    if (IO::locate(e, f0, n_groups, &f, &r, &c))
+      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) = *reinterpret_cast<const float4*>(&lds[e]);
   }
 }
 
@@ -206,6 +266,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_probe_accum(double2* __restrict
   double ax[CPT], ay[CPT];
 #pragma unroll
   for (int k = 0; k < CPT; ++k) ax[k] = ay[k] = 0.0;
+#pragma unroll 4
   for (long j = f0; j < f1; ++j) {
     const int2 r = roi[j];
     const float2* prow = psi + (j * M + q) * M;

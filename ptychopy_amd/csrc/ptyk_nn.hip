@@ -263,16 +263,20 @@ __global__ void k_up_bwd(View gout, View gin, int batch) {
   }
 }
 
-// mean squared error over all outputs; g = 2 (y - t) / count; per-block FP64 partial losses
+// weighted mean squared error: sum w(p) (y - t)^2 over all outputs, w an s x s map shared by the
+// channels and samples (NULL: 1); g = 2 w (y - t) / norm with norm = batch * 2 * sum_p w(p);
+// per-block FP64 partial sums of w (y - t)^2
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const bf16* __restrict__ t, long count,
+                                             const float* __restrict__ w, long ss, double norm,
                                              float* __restrict__ g, double* __restrict__ part) {
   __shared__ double red[4];
   double s = 0.0;
-  const float sc = (float)(2.0 / count);
+  const float sc = (float)(2.0 / norm);
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long)gridDim.x * 256) {
     const float d = y[i] - (float)t[i];
-    s += (double)d * d;
-    if (g) g[i] = sc * d;
+    const float wi = w ? w[i % ss] : 1.f;
+    s += (double)wi * d * d;
+    if (g) g[i] = sc * wi * d;
   }
   s = block_sum<256>(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -332,6 +336,8 @@ struct NnNet {
   uint8_t* arg[3] = {nullptr, nullptr, nullptr};
   float* out = nullptr;   // batch x 2 x s x s
   float* gout = nullptr;
+  float* lw = nullptr;    // s x s loss weight (NULL: uniform)
+  double lw_sum = 0.0;    // sum of the weight (s^2 when uniform)
   bf16* target = nullptr; // batch x 2 x s x s (staging)
   hipStream_t stream = nullptr;
 };
@@ -481,8 +487,9 @@ hipError_t nn_forward(NnNet& n) {
 hipError_t nn_loss(NnNet& n, bool with_grad, int* nparts) {
   const long count = (long)n.batch * 2 * n.s * n.s;
   const int nb = (int)std::min<long>((count + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, n.stream, n.out, n.target, count, with_grad ? n.gout : nullptr,
-                     n.lpart);
+  const double norm = (double)n.batch * 2 * n.lw_sum;
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, n.stream, n.out, n.target, count, n.lw, (long)n.s * n.s, norm,
+                     with_grad ? n.gout : nullptr, n.lpart);
   *nparts = nb;
   return hipGetLastError();
 }
@@ -589,6 +596,7 @@ hipError_t nn_create(NnNet& n, int s, int c, int batch) {
   n.s = s;
   n.c = c;
   n.batch = batch;
+  n.lw_sum = (double)s * s;
   nn_layer_table(c, &n.layers, &n.nparam);
   const int ch[B_COUNT] = {1, c, 3 * c, c, 2 * c, 6 * c, 2 * c, 4 * c, 12 * c, 4 * c, 8 * c, 8 * c, 4 * c, 4 * c,
                            2 * c, 2 * c, c, c};
@@ -643,7 +651,7 @@ void nn_free(NnNet& n) {
   for (auto& p : n.arg)
     if (p) hipFree(p);
   for (void* p : {(void*)n.out, (void*)n.gout, (void*)n.target, (void*)n.w, (void*)n.mo, (void*)n.ve, (void*)n.g,
-                  (void*)n.wb, (void*)n.lpart, (void*)n.partial})
+                  (void*)n.wb, (void*)n.lpart, (void*)n.partial, (void*)n.lw})
     if (p) hipFree(p);
 }
 
@@ -668,7 +676,8 @@ struct ptyk_net {
   std::string err;
   bf16* data_x = nullptr;
   bf16* data_y = nullptr;
-  int64_t ndata = 0;
+  int64_t ndata = 0;     // samples [0, ndata) hold data
+  int64_t data_cap = 0;  // samples the device buffers hold room for
   int64_t* idx = nullptr;
   int64_t idx_cap = 0;
 };
@@ -763,41 +772,95 @@ int ptyk_net_set_params(ptyk_net* h, const float* in) {
   return PTYK_OK;
 }
 
-// BF16 device copy of n samples: inputs n x s x s, targets n x 2 x s x s (float, host)
-int ptyk_net_set_data(ptyk_net* h, const float* inputs, const float* targets, int64_t n) {
-  if (!h || !inputs || !targets || n <= 0) return PTYK_EINVAL;
+// Per-pixel loss weight (s x s, non-negative, not all zero), shared by both output channels and
+// every sample; NULL restores the uniform weight.
+int ptyk_net_set_loss_weight(ptyk_net* h, const float* weight) {
+  if (!h) return PTYK_EINVAL;
   hipSetDevice(h->device);
   const long ss = (long)h->n.s * h->n.s;
+  if (!weight) {
+    NCHK(h, hipStreamSynchronize(h->n.stream));
+    if (h->n.lw) hipFree(h->n.lw);
+    h->n.lw = nullptr;
+    h->n.lw_sum = (double)ss;
+    return PTYK_OK;
+  }
+  double sum = 0.0;
+  for (long i = 0; i < ss; ++i) {
+    if (!(weight[i] >= 0.f) || !std::isfinite(weight[i])) {
+      h->err = "loss weights must be finite and non-negative";
+      return PTYK_EINVAL;
+    }
+    sum += weight[i];
+  }
+  if (!(sum > 0.0)) {
+    h->err = "loss weights are all zero";
+    return PTYK_EINVAL;
+  }
+  if (!h->n.lw) NCHK(h, hipMalloc((void**)&h->n.lw, ss * sizeof(float)));
+  NCHK(h, hipMemcpyAsync(h->n.lw, weight, ss * sizeof(float), hipMemcpyHostToDevice, h->n.stream));
+  NCHK(h, hipStreamSynchronize(h->n.stream));
+  h->n.lw_sum = sum;
+  return PTYK_OK;
+}
+
+// Device room for n samples (BF16); previous samples are dropped.
+int ptyk_net_reserve_data(ptyk_net* h, int64_t n) {
+  if (!h || n <= 0) return PTYK_EINVAL;
+  hipSetDevice(h->device);
+  const long ss = (long)h->n.s * h->n.s;
+  NCHK(h, hipStreamSynchronize(h->n.stream));
   if (h->data_x) hipFree(h->data_x);
   if (h->data_y) hipFree(h->data_y);
   h->data_x = h->data_y = nullptr;
-  h->ndata = 0;
+  h->ndata = h->data_cap = 0;
   NCHK(h, hipMalloc((void**)&h->data_x, (size_t)n * ss * sizeof(bf16)));
   NCHK(h, hipMalloc((void**)&h->data_y, (size_t)n * 2 * ss * sizeof(bf16)));
-  // convert in slices through a pinned-free staging buffer
+  h->data_cap = n;
+  return PTYK_OK;
+}
+
+// BF16 device copy of samples [offset, offset + count): inputs count x s x s, targets
+// count x 2 x s x s (float, host); samples up to offset + count become usable.
+int ptyk_net_put_data(ptyk_net* h, int64_t offset, const float* inputs, const float* targets, int64_t count) {
+  if (!h || !inputs || !targets || offset < 0 || count <= 0) return PTYK_EINVAL;
+  if (offset + count > h->data_cap) {
+    h->err = "samples beyond the reserved room";
+    return PTYK_EINVAL;
+  }
+  hipSetDevice(h->device);
+  const long ss = (long)h->n.s * h->n.s;
   const int64_t slice = 256;
   float* stage = nullptr;
-  NCHK(h, hipMalloc((void**)&stage, (size_t)slice * 2 * ss * sizeof(float)));
+  NCHK(h, hipMalloc((void**)&stage, (size_t)std::min(slice, count) * 2 * ss * sizeof(float)));
   hipError_t e = hipSuccess;
-  for (int64_t b = 0; b < n && e == hipSuccess; b += slice) {
-    const int64_t cnt = std::min(slice, n - b);
+  for (int64_t b = 0; b < count && e == hipSuccess; b += slice) {
+    const int64_t cnt = std::min(slice, count - b);
+    bf16* dx = h->data_x + (offset + b) * ss;
+    bf16* dy = h->data_y + (offset + b) * 2 * ss;
     e = hipMemcpyAsync(stage, inputs + b * ss, (size_t)cnt * ss * sizeof(float), hipMemcpyHostToDevice, h->n.stream);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((cnt * ss + 255) / 256)), dim3(256), 0, h->n.stream, stage,
-                         h->data_x + b * ss, (long)(cnt * ss));
+      hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((cnt * ss + 255) / 256)), dim3(256), 0, h->n.stream, stage, dx,
+                         (long)(cnt * ss));
       e = hipMemcpyAsync(stage, targets + b * 2 * ss, (size_t)cnt * 2 * ss * sizeof(float), hipMemcpyHostToDevice,
                          h->n.stream);
     }
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((cnt * 2 * ss + 255) / 256)), dim3(256), 0, h->n.stream, stage,
-                         h->data_y + b * 2 * ss, (long)(cnt * 2 * ss));
-      e = hipStreamSynchronize(h->n.stream);  // the host slices are reused by the caller's memory only
+      hipLaunchKernelGGL(k_to_bf, dim3((unsigned)((cnt * 2 * ss + 255) / 256)), dim3(256), 0, h->n.stream, stage, dy,
+                         (long)(cnt * 2 * ss));
+      e = hipStreamSynchronize(h->n.stream);  // the staging buffer is refilled next slice
     }
   }
   hipFree(stage);
   NCHK(h, e);
-  h->ndata = n;
+  h->ndata = std::max(h->ndata, offset + count);
   return PTYK_OK;
+}
+
+int ptyk_net_set_data(ptyk_net* h, const float* inputs, const float* targets, int64_t n) {
+  if (!h || !inputs || !targets || n <= 0) return PTYK_EINVAL;
+  int r = ptyk_net_reserve_data(h, n);
+  return r ? r : ptyk_net_put_data(h, 0, inputs, targets, n);
 }
 
 static int net_load_batch(ptyk_net* h, const int64_t* order, int64_t off) {
@@ -823,7 +886,7 @@ static int net_loss_value(ptyk_net* h, int nparts, double* loss) {
   NCHK(h, hipStreamSynchronize(h->n.stream));
   double s = 0;
   for (double v : p) s += v;
-  *loss = s / ((double)h->n.batch * 2 * h->n.s * h->n.s);
+  *loss = s / ((double)h->n.batch * 2 * h->n.lw_sum);
   return PTYK_OK;
 }
 

@@ -71,6 +71,28 @@ class Net:
         self._chk(self.L.ptyk_net_set_data(self.h, _ptr(x), _ptr(t), n))
         self.n_data = n
 
+    def set_loss_weight(self, weight=None):
+        """S x S per-pixel loss weight shared by both outputs (None: uniform)."""
+        if weight is None:
+            self._chk(self.L.ptyk_net_set_loss_weight(self.h, None))
+            return
+        w = np.ascontiguousarray(weight, dtype=np.float32)
+        assert w.shape == (self.size, self.size)
+        self._chk(self.L.ptyk_net_set_loss_weight(self.h, _ptr(w)))
+
+    def reserve_data(self, n: int):
+        self._chk(self.L.ptyk_net_reserve_data(self.h, int(n)))
+        self.n_data = 0
+        self.data_cap = int(n)
+
+    def put_data(self, offset: int, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        n = x.shape[0]
+        assert x.shape == (n, self.size, self.size) and t.shape == (n, 2, self.size, self.size)
+        self._chk(self.L.ptyk_net_put_data(self.h, int(offset), _ptr(x), _ptr(t), n))
+        self.n_data = max(self.n_data, int(offset) + n)
+
     def train(self, order, lr: float) -> float:
         o = np.ascontiguousarray(order, dtype=np.int64)
         loss = C.c_double(0.0)
@@ -116,29 +138,53 @@ def patch_targets(obj, roi, m: int) -> np.ndarray:
     return out
 
 
-def training_set(n_frames: int, modes, photons: float = 1e5, frames_per_object: int = 200, seed: int = 0,
-                 phase_std=(0.2, 1.0), device: int = 0):
-    """Frames the engine's simulator forms from random-field objects (phase std drawn from
-    phase_std) under the given incoherent probe modes, Poisson-sampled; returns
-    (inputs, targets): preprocessed amplitudes and object-patch (amplitude, phase)."""
+def _simulated_chunk(e, modes, side, cnt, photons, rng, phase_std, phase_corr=6.0):
     k, m, _ = modes.shape
+    obj = sim.random_field_object((side, side), rng, phase_std=float(rng.uniform(*phase_std)),
+                                  phase_corr=phase_corr)
+    roi = rng.integers(0, side - m + 1, size=(cnt, 2))
+    inten = sum(e.simulate(obj, modes[i], roi.astype(np.float32), squared=True) for i in range(k))
+    amp = sim.poisson_amplitudes(inten, photons, rng)
+    return preprocess(amp), patch_targets(obj, roi, m)
+
+
+def training_set(net: Net, n_frames: int, modes, photons: float = 1e5, frames_per_object: int = 200,
+                 seed: int = 0, phase_std=(0.2, 1.0), device: int = 0, log=None):
+    """Fill the network's sample store with n_frames frames that the engine's simulator forms
+    from random-field objects (phase std drawn from phase_std per object) under the incoherent
+    probe modes (K x M x M), Poisson-sampled at `photons` per frame.  Sample j's input is the
+    preprocessed amplitude, its target the object patch's (amplitude, phase)."""
+    k, m, _ = modes.shape
+    assert m == net.size
     rng = np.random.default_rng(seed)
-    xs, ts = [], []
+    net.reserve_data(n_frames)
     side = 4 * m
     with _engine.Engine(1, m, (m, m), device=device) as e:
         done = 0
         while done < n_frames:
             cnt = min(frames_per_object, n_frames - done)
-            obj = sim.random_field_object((side, side), rng, phase_std=float(rng.uniform(*phase_std)))
-            roi = rng.integers(0, side - m, size=(cnt, 2))
-            pos = roi.astype(np.float32)
-            inten = sum(e.simulate(obj, modes[i], pos, squared=True) for i in range(k))
-            tot = inten.reshape(cnt, -1).sum(axis=1)
-            counts = rng.poisson(inten * (photons / np.maximum(tot, 1e-30))[:, None, None])
-            xs.append(preprocess(np.sqrt(counts)))
-            ts.append(patch_targets(obj, roi, m))
+            x, t = _simulated_chunk(e, modes, side, cnt, photons, rng, phase_std)
+            net.put_data(done, x, t)
             done += cnt
-    return np.concatenate(xs), np.concatenate(ts)
+            if log and (done // frames_per_object) % 25 == 0:
+                log(f"simulated {done}/{n_frames} frames")
+
+
+def fit(net: Net, n_train: int, epochs: int, lr: float = 1e-3, seed: int = 0, held_out: int = 0, log=None):
+    """Adam over samples [0, n_train) for `epochs` shuffled passes; held-out loss over
+    [n_train, n_train + held_out) after each epoch.  Returns [(train_loss, held_out_loss)]."""
+    rng = np.random.default_rng(seed)
+    b = net.batch
+    n_use = n_train // b * b
+    val = np.arange(n_train, n_train + held_out // b * b)
+    hist = []
+    for ep in range(epochs):
+        tl = net.train(rng.permutation(n_train)[:n_use], lr)
+        vl = net.evaluate(val) if len(val) else float("nan")
+        hist.append((tl, vl))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: train {tl:.5f} held-out {vl:.5f}")
+    return hist
 
 
 def stitch(pred, roi, weight, canvas, fill=1.0 + 0j) -> np.ndarray:

@@ -122,3 +122,17 @@ def dm_bytes_per_frame(m: int, k: int, obj_px: int, n: int) -> float:
     """Compulsory HBM bytes per frame-iteration of the psi-resident DM form:
     M^2 (2 [FP16 A] + 16 K [psi read + write]) + (16 obj_px + 16 K M^2) / N."""
     return m * m * (2 + 16 * k) + (16 * obj_px + 16 * k * m * m) / n
+
+
+def mls_bytes_per_frame(m: int, obj_px: int, n: int, groups: int) -> float:
+    """Compulsory HBM bytes per frame-iteration of MLs: the FP16 amplitudes (2 M^2) plus the
+    canvas-sized arrays each group reads and writes (object, its direction and preconditioner,
+    illumination: 36 B per canvas pixel per group), spread over the frames."""
+    return 2.0 * m * m + 36.0 * obj_px * groups / n
+
+
+def mls_flops_per_frame(m: int, k: int) -> float:
+    """FFT work of one MLs frame-iteration, 5 n log2 n per complex n-point transform: forward
+    and inverse per mode (2K), the probe's sub-pixel shift and its inverse (2) and the three
+    transforms of the position solver."""
+    return (2 * k + 2 + 3) * 10.0 * m * m * np.log2(m)

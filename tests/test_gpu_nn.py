@@ -46,12 +46,18 @@ def sample(rng, batch, s):
     return x, t
 
 
-@pytest.mark.parametrize("s,c,batch", [(16, 4, 2), (32, 8, 3), (64, 16, 2)])
-def test_gradient_matches_oracle(s, c, batch):
-    print(f"net gradient s={s} c={c} batch={batch}", flush=True)
+@pytest.mark.parametrize("s,c,batch,weighted", [(16, 4, 2, False), (32, 8, 3, False), (64, 16, 2, False),
+                                                (32, 8, 2, True), (256, 8, 2, True)])
+def test_gradient_matches_oracle(s, c, batch, weighted):
+    print(f"net gradient s={s} c={c} batch={batch} weighted={weighted}", flush=True)
     rng = np.random.default_rng(7 + s)
     x, t = sample(rng, batch, s)
+    w = None
+    if weighted:  # a Gaussian beam profile, as the warm start uses
+        yy, xx = np.mgrid[0:s, 0:s] - s / 2
+        w = np.exp(-(xx ** 2 + yy ** 2) / (2 * (s / 6) ** 2)).astype(np.float32)
     with warmstart.Net(size=s, c=c, batch=batch, seed=3) as net:
+        net.set_loss_weight(w)
         sl, n = layer_slices(c)
         assert net.n_params == n == oracle.nn_num_params(c)
         p = net.get_params()
@@ -61,7 +67,7 @@ def test_gradient_matches_oracle(s, c, batch):
         net.set_params(p)
         loss, g = net.gradient(x, t)
         out = net.predict(x)
-    ref_out, ref_loss, ref_g = oracle.nn_forward_backward(p, x, t, c, emulate_bf16=True)
+    ref_out, ref_loss, ref_g = oracle.nn_forward_backward(p, x, t, c, emulate_bf16=True, weight=w)
     assert rel(out, ref_out) <= 5e-3
     assert abs(loss - ref_loss) <= 5e-3 * ref_loss
     for li, (w, b) in enumerate(sl):
@@ -136,3 +142,23 @@ def test_invalid_arguments_are_rejected():
             net.train(np.arange(3), 1e-3)
         with pytest.raises(Exception, match="out of range"):
             net.train(np.array([0, 5]), 1e-3)
+        with pytest.raises(Exception, match="non-negative"):
+            net.set_loss_weight(-np.ones((16, 16), np.float32))
+        with pytest.raises(Exception, match="all zero"):
+            net.set_loss_weight(np.zeros((16, 16), np.float32))
+        with pytest.raises(Exception, match="beyond the reserved"):
+            net.put_data(1, x, t)
+
+
+def test_streamed_samples_equal_one_shot_samples():
+    rng = np.random.default_rng(4)
+    s, c, batch = 16, 4, 2
+    x, t = sample(rng, 6, s)
+    with warmstart.Net(size=s, c=c, batch=batch, seed=5) as a, warmstart.Net(size=s, c=c, batch=batch, seed=5) as b:
+        a.set_data(x, t)
+        b.reserve_data(6)
+        b.put_data(4, x[4:], t[4:])
+        b.put_data(0, x[:4], t[:4])
+        order = np.array([5, 0, 3, 2, 1, 4])
+        assert a.train(order, 1e-3) == b.train(order, 1e-3)
+        np.testing.assert_array_equal(a.get_params(), b.get_params())
