@@ -192,6 +192,9 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 512 MiB device-to-device once (a kernel of known traffic that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--dm-wave", type=int, default=64,
                     help="frames per DM wave (0 automatic, -1 one pass over all frames)")
     args = ap.parse_args()
@@ -201,6 +204,13 @@ def main() -> int:
               "(the DM frame loop would shard by scan region with one object all-reduce per iteration; "
               "that path is not built)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        import torch
+        src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "mls_c2":
         args.steps = 10 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup

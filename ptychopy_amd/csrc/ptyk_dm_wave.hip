@@ -228,21 +228,37 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_obj_accum(double2* __restrict__
                                                            const int* __restrict__ row_frames,
                                                            const float* __restrict__ factors, int phase_constraint) {
   const int p = row_lo + blockIdx.x;
-  const int cbeg = blockIdx.y * PTYK_OBJ_CHUNK, cend = cbeg + PTYK_OBJ_CHUNK;
-  const int c = cbeg + threadIdx.x;
+  const int c = blockIdx.y * PTYK_OBJ_CHUNK + threadIdx.x;
   const float so = factors[1];
   double ax = 0.0, ay = 0.0;
   const int k0 = row_ptr[p], k1 = row_ptr[p + 1];
-  for (int k = k0; k < k1; ++k) {
-    const int j = row_frames[k];
-    const int2 r = roi[j];
-    if (r.y >= cend || r.y + M <= cbeg) continue;
-    const int q = p - r.x, cc = c - r.y;
-    if (c < obj_cols && cc >= 0 && cc < M) {
-      float2 u = cscale(cmulcj(probe[q * M + cc], psi[((long)j * M + q) * M + cc]), so);
-      if (phase_constraint) u = clamp_phase(u);
-      ax += u.x;
-      ay += u.y;
+  // frames in ascending order, four at a time: all loads of a step are issued before the
+  // (in-order) accumulation, so the sum is the same as one frame at a time
+  constexpr int U = 4;
+  for (int kb = k0; kb < k1; kb += U) {
+    float2 pv[U], sv[U];
+    bool act[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      act[u] = false;
+      if (kb + u < k1) {
+        const int j = row_frames[kb + u];
+        const int2 r = roi[j];
+        const int q = p - r.x, cc = c - r.y;
+        act[u] = c < obj_cols && cc >= 0 && cc < M;
+        if (act[u]) {
+          pv[u] = probe[q * M + cc];
+          sv[u] = psi[((long)j * M + q) * M + cc];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!act[u]) continue;
+      float2 t = cscale(cmulcj(pv[u], sv[u]), so);
+      if (phase_constraint) t = clamp_phase(t);
+      ax += t.x;
+      ay += t.y;
     }
   }
   if (c < obj_cols && k1 > k0) {

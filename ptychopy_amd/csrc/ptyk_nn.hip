@@ -49,89 +49,105 @@ struct GemmArgs {
   const bf16* wt;      // weights Cout x Cin x ks x ks (bf16)
   const float* bias;   // FWD
   float* partial;      // BWD_W: chunks x M x N
-  int batch, cin, cout, ks, relu, out_f32;
+  int batch, cin, cout, relu, out_f32;
   int M, N, K, kchunk;
+  int w_sh, hw_sh;     // log2 of the image width and of the pixels per image (powers of two)
+  int vec_a;           // FWD: weight rows 16-byte aligned and K % 8 == 0 (8 bf16 per load)
 };
 
 __device__ __forceinline__ bf16 to_bf(float v) { return (bf16)v; }
 
-template <int MODE>
+// A(m, k) for BWD_DATA and BWD_W (FWD reads the weight rows directly)
+template <int MODE, int KS>
 __device__ __forceinline__ bf16 load_a(const GemmArgs& a, int m, int k) {
   if (m >= a.M || k >= a.K) return to_bf(0.f);
-  const int kk2 = a.ks * a.ks;
+  constexpr int KK2 = KS * KS;
   if (MODE == GEMM_FWD) return a.wt[(long)m * a.K + k];
   if (MODE == GEMM_BWD_DATA) {  // A(ci, (co, d)) = W[co][ci][d]
-    const int co = k / kk2, d = k - co * kk2;
-    return a.wt[((long)co * a.cin + m) * kk2 + d];
+    const int co = k / KK2, d = k - co * KK2;
+    return a.wt[((long)co * a.cin + m) * KK2 + d];
   }
   // BWD_W: A(co, (b, p)) = dZ[b][co][p]
-  const int hw = a.y.h * a.y.w;
-  const int b = k / hw, p = k - b * hw;
-  return to_bf(((const float*)a.y.p)[((long)b * a.y.ctot + m) * hw + p]);
+  const int b = k >> a.hw_sh, p = k & ((1 << a.hw_sh) - 1);
+  return to_bf(((const float*)a.y.p)[(((long)b * a.y.ctot + m) << a.hw_sh) + p]);
 }
 
-template <int MODE>
-__device__ __forceinline__ bf16 load_b(const GemmArgs& a, int k, int n) {
-  if (k >= a.K || n >= a.N) return to_bf(0.f);
-  const int kk2 = a.ks * a.ks, r = a.ks >> 1;
-  if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
-    // n = (b, y, x); k = (channel, d); FWD reads x(p + d), BWD_DATA reads dZ(p - d)
-    const View& v = a.x;
-    const int hw = v.h * v.w;
-    const int b = n / hw, p = n - b * hw, yy = p / v.w, xx = p - yy * v.w;
-    const int ch = k / kk2, d = k - ch * kk2;
-    int dy = d / a.ks - r, dx = d % a.ks - r;
-    if (MODE == GEMM_BWD_DATA) {
-      dy = -dy;
-      dx = -dx;
-    }
-    const int sy = yy + dy, sx = xx + dx;
-    if (sy < 0 || sy >= v.h || sx < 0 || sx >= v.w) return to_bf(0.f);
-    const long idx = ((long)b * v.ctot + ch) * hw + sy * v.w + sx;
-    if (MODE == GEMM_FWD) return ((const bf16*)v.p)[idx];
-    return to_bf(((const float*)v.p)[idx]);
-  }
-  // BWD_W: B((b, p), (ci, d)) = x[b][ci][p + d]
-  const View& v = a.x;
-  const int hw = v.h * v.w;
-  const int b = k / hw, p = k - b * hw, yy = p / v.w, xx = p - yy * v.w;
-  const int ci = n / kk2, d = n - ci * kk2;
-  const int sy = yy + d / a.ks - r, sx = xx + d % a.ks - r;
-  if (sy < 0 || sy >= v.h || sx < 0 || sx >= v.w) return to_bf(0.f);
-  return ((const bf16*)v.p)[((long)b * v.ctot + ci) * hw + sy * v.w + sx];
-}
-
-template <int MODE>
+// Implicit-GEMM convolution tile: D[m0 .. m0+64) x [n0 .. n0+64) over K in steps of 32.
+template <int MODE, int KS>
 __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
+  constexpr int KK2 = KS * KS, R = KS / 2;
   __shared__ bf16 As[TM * LDP];
   __shared__ bf16 Bt[TN * LDP];  // transposed: Bt[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int kbeg = MODE == GEMM_BWD_W ? blockIdx.z * a.kchunk : 0;
   const int kend = MODE == GEMM_BWD_W ? min(a.K, kbeg + a.kchunk) : a.K;
+  const int wmask = (1 << a.w_sh) - 1, hwmask = (1 << a.hw_sh) - 1;
+  // FWD / BWD_DATA: each thread always loads the same B column n0 + tid % 64
+  const int bn = n0 + (tid & (TN - 1));
+  const bool bn_ok = bn < a.N;
+  const int bb = bn >> a.hw_sh, bp = bn & hwmask, by = bp >> a.w_sh, bx = bp & wmask;
   v4f acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
   for (int k0 = kbeg; k0 < kend; k0 += TK) {
-    // A tile 64 x 32: consecutive threads along k
-    for (int e = tid; e < TM * TK; e += GT) {
-      const int mm = e / TK, kk = e % TK;
-      const int k = k0 + kk;
-      As[mm * LDP + kk] = k < kend ? load_a<MODE>(a, m0 + mm, k) : to_bf(0.f);
-    }
-    // B tile 32 x 64: FWD / BWD_DATA consecutive threads along n (pixels); BWD_W along k
-    for (int e = tid; e < TN * TK; e += GT) {
-      int kk, nn;
-      if (MODE == GEMM_BWD_W) {
-        kk = e % TK;
-        nn = e / TK;
-      } else {
-        nn = e % TN;
-        kk = e / TN;
+    // A tile 64 x 32
+    if (MODE == GEMM_FWD && a.vec_a) {
+      const int mm = tid >> 2, kk = (tid & 3) * 8;
+      const int m = m0 + mm, k = k0 + kk;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (m < a.M && k < a.K) v = *reinterpret_cast<const uint4*>(a.wt + (long)m * a.K + k);
+      *reinterpret_cast<uint4*>(&As[mm * LDP + kk]) = v;
+    } else {
+#pragma unroll
+      for (int i = 0; i < TM * TK / GT; ++i) {
+        const int e = tid + i * GT;
+        const int mm = e / TK, kk = e % TK;
+        const int k = k0 + kk;
+        As[mm * LDP + kk] = k < kend ? load_a<MODE, KS>(a, m0 + mm, k) : to_bf(0.f);
       }
-      const int k = k0 + kk;
-      Bt[nn * LDP + kk] = k < kend ? load_b<MODE>(a, k, n0 + nn) : to_bf(0.f);
+    }
+    // B tile 32 x 64
+    if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
+      const View& v = a.x;
+#pragma unroll
+      for (int i = 0; i < TN * TK / GT; ++i) {
+        const int kk = (tid >> 6) + i * (GT / TN);
+        const int k = k0 + kk;
+        bf16 val = to_bf(0.f);
+        if (bn_ok && k < kend) {
+          const int ch = k / KK2, d = k - ch * KK2;
+          int dy = d / KS - R, dx = d % KS - R;
+          if (MODE == GEMM_BWD_DATA) {
+            dy = -dy;
+            dx = -dx;
+          }
+          const int sy = by + dy, sx = bx + dx;
+          if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w) {
+            const long idx = (((long)bb * v.ctot + ch) << a.hw_sh) + (sy << a.w_sh) + sx;
+            val = MODE == GEMM_FWD ? ((const bf16*)v.p)[idx] : to_bf(((const float*)v.p)[idx]);
+          }
+        }
+        Bt[(tid & (TN - 1)) * LDP + kk] = val;
+      }
+    } else {  // BWD_W: B((b, p), (ci, d)) = x[b][ci][p + d]; consecutive threads along k
+      const View& v = a.x;
+#pragma unroll
+      for (int i = 0; i < TN * TK / GT; ++i) {
+        const int e = tid + i * GT;
+        const int kk = e % TK, nn = e / TK;
+        const int k = k0 + kk, n = n0 + nn;
+        bf16 val = to_bf(0.f);
+        if (k < kend && n < a.N) {
+          const int b = k >> a.hw_sh, p = k & hwmask, yy = p >> a.w_sh, xx = p & wmask;
+          const int ci = n / KK2, d = n - ci * KK2;
+          const int sy = yy + d / KS - R, sx = xx + d % KS - R;
+          if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w)
+            val = ((const bf16*)v.p)[(((long)b * v.ctot + ci) << a.hw_sh) + (sy << a.w_sh) + sx];
+        }
+        Bt[nn * LDP + kk] = val;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -160,9 +176,8 @@ __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
           a.partial[((long)blockIdx.z * a.M + m) * a.N + n] = v;
         } else {
           const View& o = a.y;
-          const int hw = o.h * o.w;
-          const int b = n / hw, p = n - b * hw;
-          const long idx = ((long)b * o.ctot + m) * hw + p;
+          const int b = n >> a.hw_sh, p = n & hwmask;
+          const long idx = (((long)b * o.ctot + m) << a.hw_sh) + p;
           if (MODE == GEMM_FWD) {
             v += a.bias[m];
             if (a.relu) v = fmaxf(v, 0.f);
@@ -173,6 +188,18 @@ __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
           }
         }
       }
+}
+
+template <int MODE>
+static void launch_gemm(const GemmArgs& a, int ks, dim3 g, hipStream_t st) {
+  if (ks == 3) hipLaunchKernelGGL((k_conv_gemm<MODE, 3>), g, dim3(GT), 0, st, a);
+  else hipLaunchKernelGGL((k_conv_gemm<MODE, 1>), g, dim3(GT), 0, st, a);
+}
+
+static int ilog2(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
 }
 
 // dW = sum over chunks (fixed order); db[co] = sum over (b, p) of dZ (block partials, fixed order)
@@ -383,14 +410,16 @@ static hipError_t conv_fwd(NnNet& n, int li, View x, View y, bool out_f32) {
   a.batch = n.batch;
   a.cin = L.cin;
   a.cout = L.cout;
-  a.ks = L.ks;
   a.relu = L.relu;
   a.out_f32 = out_f32 ? 1 : 0;
   a.M = L.cout;
   a.N = n.batch * x.h * x.w;
   a.K = L.cin * L.ks * L.ks;
+  a.w_sh = ilog2(x.w);
+  a.hw_sh = ilog2(x.h * x.w);
+  a.vec_a = (a.K % 8 == 0 && L.woff % 8 == 0 && ((uintptr_t)n.wb & 15) == 0) ? 1 : 0;
   dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, 1);
-  hipLaunchKernelGGL(k_conv_gemm<GEMM_FWD>, g, dim3(GT), 0, n.stream, a);
+  launch_gemm<GEMM_FWD>(a, L.ks, g, n.stream);
   return hipGetLastError();
 }
 
@@ -406,17 +435,18 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     a.batch = n.batch;
     a.cin = L.cin;
     a.cout = L.cout;
-    a.ks = L.ks;
     a.M = L.cout;
     a.N = L.cin * kk2;
     a.K = n.batch * x.h * x.w;
+    a.w_sh = ilog2(x.w);
+    a.hw_sh = ilog2(x.h * x.w);
     int chunks = std::max(1, std::min(256, a.K / 4096));
     a.kchunk = ((a.K + chunks - 1) / chunks + TK - 1) / TK * TK;
     chunks = (a.K + a.kchunk - 1) / a.kchunk;
     if ((long)chunks * a.M * a.N > n.partial_cap) return hipErrorInvalidValue;
     a.partial = n.partial;
     dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, chunks);
-    hipLaunchKernelGGL(k_conv_gemm<GEMM_BWD_W>, g, dim3(GT), 0, n.stream, a);
+    launch_gemm<GEMM_BWD_W>(a, L.ks, g, n.stream);
     const long mn = (long)a.M * a.N;
     hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, n.stream, n.partial, chunks,
                        mn, n.g + L.woff);
@@ -430,12 +460,13 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     a.batch = n.batch;
     a.cin = L.cin;
     a.cout = L.cout;
-    a.ks = L.ks;
     a.M = L.cin;
     a.N = n.batch * dz.h * dz.w;
     a.K = L.cout * kk2;
+    a.w_sh = ilog2(dz.w);
+    a.hw_sh = ilog2(dz.h * dz.w);
     dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, 1);
-    hipLaunchKernelGGL(k_conv_gemm<GEMM_BWD_DATA>, g, dim3(GT), 0, n.stream, a);
+    launch_gemm<GEMM_BWD_DATA>(a, L.ks, g, n.stream);
   }
   return hipGetLastError();
 }

@@ -65,7 +65,7 @@ def dm_c1(n_side: int = 30, m: int = 512, photons: float = 1e7, sim_size: int = 
 
 
 def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwhm: float = 60.0,
-           config_id: int = 2, device: int = 0) -> dict:
+           config_id: int = 2, device: int = 0, phase_corr: float = 6.0) -> dict:
     dx_s = 100e-9 / 14.0
     step = 100e-9
     raw = cartesian_mesh_raw(n_side, n_side, step, step, dx_s, m)
@@ -73,7 +73,7 @@ def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwh
     hi = np.maximum(raw.max(axis=0), 0) if n_side % 32 else raw.max(axis=0)
     canvas, box = mls_canvas(raw, np.array([lo[0], lo[1], hi[0], hi[1]], np.float32), m)
     rng = sim.rng_for(config_id, 0)
-    obj = sim.random_field_object(canvas, rng, phase_std=0.5)
+    obj = sim.random_field_object(canvas, rng, phase_std=0.5, phase_corr=phase_corr)
     modes = sim.hermite_gaussian_modes(m, k, fwhm)
     # frames at the MLs ROI frame: origin round(pos + ceil(Np/2 - M/2)) - 1
     off = np.array([np.ceil(canvas[0] / 2 - m / 2) - 1, np.ceil(canvas[1] / 2 - m / 2) - 1], np.float32)
