@@ -221,6 +221,12 @@ int ptyk_batch_graph_count(const ptyk_batch* batch); /* graphs captured so far *
  * out[4] = {e, Re gamma, Im gamma, pixels used}. */
 int ptyk_set_true_object(ptyk_ctx* ctx, const float* truth);
 int ptyk_object_error(ptyk_ctx* ctx, double frac, double* out);
+/* Learned warm start (no reference counterpart): the object becomes the average of the given
+ * patches (n x 2 x M x M: amplitude, phase; patch i for frame frames[i]) placed at the frames'
+ * ROIs (the MLs ROIs after ptyk_mls_begin), weighted by sum_k |P_k|^2 of the current probe;
+ * pixels whose weight sum is <= rel_threshold x the probe's peak intensity take fill. */
+int ptyk_stitch_patches(ptyk_ctx* ctx, const float* patches, const int32_t* frames, int64_t n, float rel_threshold,
+                        float fill_re, float fill_im);
 
 /* ---------------------------------------------------------------- learned warm start
  * Convolutional encoder-decoder (no reference counterpart): four levels with c, 2c, 4c, 8c
@@ -250,6 +256,8 @@ int ptyk_net_put_data(ptyk_net* net, int64_t offset, const float* inputs, const 
 int ptyk_net_train(ptyk_net* net, const int64_t* order, int64_t count, float lr, double* mean_loss);
 int ptyk_net_evaluate(ptyk_net* net, const int64_t* order, int64_t count, double* mean_loss);
 int ptyk_net_predict(ptyk_net* net, const float* inputs, int64_t n, float* outputs);
+/* Same, from raw diffraction amplitudes: the input log(1 + A) / max is formed on the device. */
+int ptyk_net_predict_amplitudes(ptyk_net* net, const float* amplitudes, int64_t n, float* outputs);
 /* Loss (mean squared error) and its parameter gradient for one batch. */
 int ptyk_net_gradient(ptyk_net* net, const float* inputs, const float* targets, double* loss, float* grad);
 

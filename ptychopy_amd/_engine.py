@@ -40,7 +40,8 @@ EXPORTS = (
     "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
     "ptyk_net_last_error", "ptyk_net_num_params", "ptyk_net_get_params", "ptyk_net_set_params",
     "ptyk_net_set_loss_weight", "ptyk_net_set_data", "ptyk_net_reserve_data", "ptyk_net_put_data", "ptyk_net_train",
-    "ptyk_net_evaluate", "ptyk_net_predict", "ptyk_net_gradient",
+    "ptyk_net_evaluate", "ptyk_net_predict", "ptyk_net_predict_amplitudes", "ptyk_net_gradient",
+    "ptyk_stitch_patches",
 )
 
 TUNE_DM_WAVE = 1
@@ -160,6 +161,8 @@ def load():
     lib.ptyk_net_train.argtypes = [P, P, i64, f, C.POINTER(C.c_double)]
     lib.ptyk_net_evaluate.argtypes = [P, P, i64, C.POINTER(C.c_double)]
     lib.ptyk_net_predict.argtypes = [P, P, i64, P]
+    lib.ptyk_net_predict_amplitudes.argtypes = [P, P, i64, P]
+    lib.ptyk_stitch_patches.argtypes = [P, P, P, i64, f, f, f]
     lib.ptyk_net_gradient.argtypes = [P, P, P, C.POINTER(C.c_double), P]
     _lib = lib
     return lib
@@ -390,6 +393,14 @@ class Engine:
         t = np.ascontiguousarray(truth, dtype=np.complex64)
         assert t.shape == self.obj_shape
         self._chk(self.lib.ptyk_set_true_object(self.h, _ptr(t)))
+
+    def stitch_patches(self, patches, frames, rel_threshold: float = 1e-3, fill=1.0 + 0j):
+        """Object <- probe-intensity-weighted average of (amplitude, phase) patches at the frames' ROIs."""
+        p = np.ascontiguousarray(patches, dtype=np.float32)
+        fr = np.ascontiguousarray(frames, dtype=np.int32)
+        assert p.shape == (fr.size, 2, self.m, self.m)
+        self._chk(self.lib.ptyk_stitch_patches(self.h, _ptr(p), _ptr(fr), fr.size, float(rel_threshold),
+                                               float(np.real(fill)), float(np.imag(fill))))
 
     def object_error(self, frac: float = 0.01):
         """(e, gamma, pixels): min_gamma ||gamma O - T|| / ||T|| over the illuminated region."""

@@ -1158,7 +1158,7 @@ int ptyk_set_true_object(ptyk_ctx* c, const float* truth) {
   if (!c->truth) {
     HIPCHK(c, dalloc(&c->truth, canvas));
     HIPCHK(c, dalloc(&c->cov, canvas));
-    HIPCHK(c, dalloc(&c->pint, c->mm));
+    if (!c->pint) HIPCHK(c, dalloc(&c->pint, c->mm));
     HIPCHK(c, dalloc(&c->epart, 5 * 1024));
   }
   return copy_in(c, c->truth, truth, canvas * sizeof(float2));
@@ -1173,6 +1173,36 @@ int ptyk_object_error(ptyk_ctx* c, double frac, double* out) {
   hipSetDevice(c->cfg.device);
   HIPCHK(c, object_error(c->obj, c->truth, c->probe, c->cfg.n_modes, c->cfg.frame_size, roi, c->cfg.n_frames,
                          c->cfg.obj_rows, c->cfg.obj_cols, frac, c->cov, c->pint, c->epart, 5 * 1024, out, c->stream));
+  return PTYK_OK;
+}
+
+// Learned warm start: the object becomes the probe-intensity-weighted average of the given
+// (amplitude, phase) patches placed at the ROIs of the selected frames (MLs ROIs after
+// ptyk_mls_begin, otherwise the positions' ROIs).
+int ptyk_stitch_patches(ptyk_ctx* c, const float* patches, const int32_t* frames, int64_t n, float rel_threshold,
+                        float fill_re, float fill_im) {
+  if (!c || !patches || !frames || n <= 0 || n > c->cfg.n_frames || !(rel_threshold >= 0.f))
+    return fail(c, PTYK_EINVAL, "bad argument");
+  const int2* roi = c->mls_ready ? c->mls.roi : (c->have_pos ? c->roi_dev : nullptr);
+  if (!roi) return fail(c, PTYK_ESTATE, "no scan positions");
+  for (int64_t i = 0; i < n; ++i)
+    if (frames[i] < 0 || frames[i] >= c->cfg.n_frames) return fail(c, PTYK_EINVAL, "frame index out of range");
+  hipSetDevice(c->cfg.device);
+  if (!c->pint) HIPCHK(c, dalloc(&c->pint, c->mm));
+  float* d_p = nullptr;
+  int* d_sel = nullptr;
+  const size_t pbytes = (size_t)n * 2 * c->mm * sizeof(float);
+  hipError_t e = hipMalloc((void**)&d_p, pbytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_sel, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpyAsync(d_p, patches, pbytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sel, frames, n * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = stitch_patches(c->obj, d_p, c->probe, c->cfg.n_modes, c->cfg.frame_size, roi, d_sel, (int)n, c->cfg.obj_rows,
+                       c->cfg.obj_cols, c->pint, rel_threshold, make_float2(fill_re, fill_im), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_p) hipFree(d_p);
+  if (d_sel) hipFree(d_sel);
+  HIPCHK(c, e);
   return PTYK_OK;
 }
 

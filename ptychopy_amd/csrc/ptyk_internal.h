@@ -145,6 +145,12 @@ hipError_t mls_init(MlsDev& d, const double* frame_power /* host, n */, hipStrea
 hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s);
 hipError_t mls_roi(MlsDev& d, hipStream_t s);
 
+// ---- learned warm start: weighted average of (amplitude, phase) patches at the ROIs of the
+// selected frames sel[0..n) into obj; pixels whose weight sum is <= rel_thr * max_q
+// sum_k |P_k(q)|^2 take `fill` (ptyk_metrics.hip).
+hipError_t stitch_patches(float2* obj, const float* patches, const float2* probe, int k, int m, const int2* roi,
+                          const int* sel, int n, int rows, int cols, float* pint, float rel_thr, float2 fill,
+                          hipStream_t s);
 // ---- normalised object error (ptyk_metrics.hip); out[4] = {e, Re gamma, Im gamma, pixels}.
 hipError_t object_error(const float2* obj, const float2* truth, const float2* probe, int k, int m, const int2* roi,
                         int n, int rows, int cols, double frac, float* cov, float* pint, double* part, int part_cap,

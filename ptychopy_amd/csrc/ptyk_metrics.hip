@@ -100,7 +100,77 @@ __global__ __launch_bounds__(NT) void k_err_sums(const float2* __restrict__ o, c
   }
 }
 
+// Weighted patch average: O[p] = sum_j w(p - r_j) a_j e^{i phi_j} / sum_j w(p - r_j) over the
+// selected frames j covering p (ascending selection order; frame sel[j], ROI origin
+// roi[sel[j]]), w = sum_k |P_k|^2; pixels whose weight sum is at most thr keep `fill`.
+// Patches: n x 2 x M x M (amplitude, phase), patch j for selected frame j.
+__global__ __launch_bounds__(NT) void k_stitch(float2* __restrict__ obj, const float* __restrict__ patches,
+                                               const float* __restrict__ pint, const int2* __restrict__ roi,
+                                               const int* __restrict__ sel, int n, int m, int rows, int cols,
+                                               float thr, float2 fill) {
+  __shared__ int lj[NT];
+  __shared__ int wcnt[NT / 64];
+  const int r0 = blockIdx.y * 4, c0 = blockIdx.x * 64;
+  const int r = r0 + (threadIdx.x >> 6), c = c0 + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long mm = (long)m * m;
+  float nx = 0.f, ny = 0.f, den = 0.f;
+  for (int base = 0; base < n; base += NT) {
+    const int f = base + threadIdx.x;
+    bool hit = false;
+    if (f < n) {
+      const int2 o = roi[sel[f]];
+      hit = !(o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0);
+    }
+    const unsigned long long bal = __ballot(hit);
+    __syncthreads();
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < NT / 64; ++i) {
+      if (i < w) off += wcnt[i];
+      tot += wcnt[i];
+    }
+    if (hit) lj[off + __popcll(bal & ((1ull << lane) - 1ull))] = f;
+    __syncthreads();
+    for (int t = 0; t < tot; ++t) {
+      const int j = lj[t];
+      const int2 o = roi[sel[j]];
+      const int lr = r - o.x, lc = c - o.y;
+      if (lr >= 0 && lr < m && lc >= 0 && lc < m) {
+        const long q = (long)lr * m + lc;
+        const float wt = pint[q];
+        const float a = patches[(2 * (long)j) * mm + q], ph = patches[(2 * (long)j + 1) * mm + q];
+        float sn, cs;
+        sincosf(ph, &sn, &cs);
+        nx += wt * a * cs;
+        ny += wt * a * sn;
+        den += wt;
+      }
+    }
+    __syncthreads();
+  }
+  if (r < rows && c < cols) obj[(long)r * cols + c] = den > thr ? make_float2(nx / den, ny / den) : fill;
+}
+
 }  // namespace
+
+hipError_t stitch_patches(float2* obj, const float* patches, const float2* probe, int k, int m, const int2* roi,
+                          const int* sel, int n, int rows, int cols, float* pint, float rel_thr, float2 fill,
+                          hipStream_t s) {
+  const int mm = m * m;
+  hipLaunchKernelGGL(k_mode_power, dim3((mm + NT - 1) / NT), dim3(NT), 0, s, probe, k, mm, pint);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  std::vector<float> h(mm);
+  if ((e = hipMemcpyAsync(h.data(), pint, mm * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  float mx = 0.f;
+  for (float v : h) mx = std::max(mx, v);
+  hipLaunchKernelGGL(k_stitch, dim3((cols + 63) / 64, (rows + 3) / 4), dim3(NT), 0, s, obj, patches, pint, roi, sel,
+                     n, m, rows, cols, rel_thr * mx, fill);
+  return hipGetLastError();
+}
 
 hipError_t object_error(const float2* obj, const float2* truth, const float2* probe, int k, int m, const int2* roi,
                         int n, int rows, int cols, double frac, float* cov, float* pint, double* part, int part_cap,

@@ -1,9 +1,10 @@
 """Learned warm start for MLs: network binding, training data, stitching.
 
 The network (include/ptyk.h, ``ptyk_net_*``) maps a log-scaled diffraction amplitude to the
-amplitude and phase of the object patch under that frame; ``stitch`` averages the predicted
-patches into an initial object, weighting each by the scan's probe intensity.  The plain
-iterative solver then refines it (and remains the fallback when no network is given).
+amplitude and phase of the object patch under that frame; ``warm_start`` runs it on the GPU
+(preprocessing included) and has the engine average the predicted patches into its object
+(``ptyk_stitch_patches``), weighting each by the probe intensity.  The plain iterative solver
+then refines that object, and remains the fallback when no network is given.
 """
 from __future__ import annotations
 
@@ -111,6 +112,13 @@ class Net:
         self._chk(self.L.ptyk_net_predict(self.h, _ptr(x), x.shape[0], _ptr(out)))
         return out
 
+    def predict_amplitudes(self, amplitudes) -> np.ndarray:
+        """predict(preprocess(amplitudes)) with the preprocessing done on the GPU."""
+        a = np.ascontiguousarray(amplitudes, dtype=np.float32)
+        out = np.empty((a.shape[0], 2, self.size, self.size), np.float32)
+        self._chk(self.L.ptyk_net_predict_amplitudes(self.h, _ptr(a), a.shape[0], _ptr(out)))
+        return out
+
     def gradient(self, inputs, targets):
         x = np.ascontiguousarray(inputs, dtype=np.float32)
         t = np.ascontiguousarray(targets, dtype=np.float32)
@@ -187,19 +195,18 @@ def fit(net: Net, n_train: int, epochs: int, lr: float = 1e-3, seed: int = 0, he
     return hist
 
 
-def stitch(pred, roi, weight, canvas, fill=1.0 + 0j) -> np.ndarray:
-    """Average of the predicted patches amp * exp(i phase) at their ROIs, weighted by `weight`
-    (M x M, e.g. the total probe intensity); pixels with no weight take `fill`."""
-    m = pred.shape[-1]
-    num = np.zeros(canvas, np.complex128)
-    den = np.zeros(canvas, np.float64)
-    w = np.asarray(weight, np.float64)
-    for j, (r0, c0) in enumerate(roi):
-        patch = pred[j, 0].astype(np.float64) * np.exp(1j * pred[j, 1].astype(np.float64))
-        num[r0:r0 + m, c0:c0 + m] += w * patch
-        den[r0:r0 + m, c0:c0 + m] += w
-    thr = 1e-3 * den.max()
-    out = np.full(canvas, fill, np.complex128)
-    ok = den > thr
-    out[ok] = num[ok] / den[ok]
-    return out.astype(np.complex64)
+def lattice_frames(n_rows: int, n_cols: int, stride: int) -> np.ndarray:
+    """Frame ids of a row-major n_rows x n_cols raster on every stride-th row and column: a sparser
+    set of patches that still covers the scan when stride x step is below the beam width."""
+    r, c = np.meshgrid(np.arange(0, n_rows, stride), np.arange(0, n_cols, stride), indexing="ij")
+    return (r * n_cols + c).ravel().astype(np.int32)
+
+
+def warm_start(engine: _engine.Engine, net: Net, amplitudes, frames, rel_threshold: float = 1e-3, fill=1.0 + 0j):
+    """Initial object for the engine's scan: the network's patches for the given frames (from the
+    raw amplitudes, preprocessed on the GPU), stitched on the GPU at the frames' ROIs and weighted
+    by the current probe intensity.  Call after positions (or ptyk_mls_begin) are set."""
+    fr = np.ascontiguousarray(frames, dtype=np.int32)
+    pred = net.predict_amplitudes(np.asarray(amplitudes)[fr])
+    engine.stitch_patches(pred, fr, rel_threshold, fill)
+    return pred
