@@ -14,7 +14,11 @@ on from iteration 20).
 
 Prints one JSON line.
 
-  python bench.py [--gpus 1] [--workload dm_c1|mls_c2] [--steps K] [--warmup W] [--no-cpu]
+`--workload c4`: the learned warm start -- trains the network on simulated frames, then runs
+MLs on the mls_c2 workload from a uniform and from the stitched start to a target normalised
+object error (minutes; not a throughput line).
+
+  python bench.py [--gpus 1] [--workload dm_c1|mls_c2|c4] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -186,14 +190,20 @@ def run_mls(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("dm_c1", "mls_c2"), default="dm_c1")
+    ap.add_argument("--workload", choices=("dm_c1", "mls_c2", "c4"), default="dm_c1")
+    ap.add_argument("--c4-train", type=int, default=50000, help="c4: training frames")
+    ap.add_argument("--c4-held", type=int, default=5000, help="c4: held-out frames")
+    ap.add_argument("--c4-epochs", type=int, default=2, help="c4: training epochs")
+    ap.add_argument("--c4-target", type=float, default=1e-2, help="c4: normalised object error target")
+    ap.add_argument("--c4-iters", type=int, default=100, help="c4: MLs iteration limit per start")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 for dm_c1, 10 for mls_c2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 10 for dm_c1, 20 for mls_c2)")
     ap.add_argument("--cpu-frames", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="first copy 512 MiB device-to-device once (a kernel of known traffic that scales the "
+                    help="first copy 512 MiB device-to-device twice from a cache-cold source, with the runtime's "
+                         "copy kernel and with an elementwise kernel (kernels of known traffic that scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--dm-wave", type=int, default=64,
                     help="frames per DM wave (0 automatic, -1 one pass over all frames)")
@@ -207,10 +217,30 @@ def main() -> int:
     if args.copy_probe:
         import torch
         src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        evict = torch.ones(256 << 20, dtype=torch.float32, device="cuda")  # 1 GiB: src leaves the caches
         dst = torch.empty_like(src)
-        dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        dst.copy_(src)  # the runtime's copy kernel
+        evict.fill_(2.0)
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel with 16-byte loads and stores
+        torch.cuda.synchronize()
+        del src, dst, evict
+    if args.workload == "c4":
+        from ptychopy_amd import warmstart
+        t0 = time.perf_counter()
+        r = warmstart.c4_experiment(n_train=args.c4_train, n_held=args.c4_held, epochs=args.c4_epochs,
+                                    target=args.c4_target, max_iter=args.c4_iters,
+                                    log=lambda s: print(s, file=sys.stderr, flush=True))
+        out = {"metric": f"time-to-{args.c4_target:g} normalised object error, warm vs plain start",
+               "value": r["speedup"], "unit": "x (plain time / warm time)", "n_gpus": 1, "steps": None,
+               "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "bf16 network, c64 solver",
+               "data": "synthetic (random-field training objects, phase std 0.2-1.0 rad; the MLs workload's "
+                       "seeded object and Poisson frames)",
+               "config": {"workload": "learned warm start + MLs, 51x51 raster, 256x256 frames, 5 probe modes"},
+               "c4": r, "wall_s": time.perf_counter() - t0}
+        print(json.dumps(out))
+        return 0
     if args.workload == "mls_c2":
         args.steps = 10 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
@@ -264,10 +294,12 @@ def main() -> int:
     dom = int(np.argmax(kt))
     name = KERNELS[dom]
     ach = kernel_bytes(name, n, m, obj_px) / (kt[dom] * 1e-3) / 1e9
+    # HBM bytes of this kernel kind per iteration, from the committed FETCH_SIZE / WRITE_SIZE counter
+    # runs (scaled by the calibration copy), valid for the default 64-frame waves
     traffic = None
     tfile = ROOT / "profiles" / "dm_c1_traffic.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get(name)
+    if tfile.exists() and args.dm_wave == 64:
+        traffic = json.loads(tfile.read_text()).get("kinds", {}).get(name, {}).get("traffic")
     bpf = workloads.dm_bytes_per_frame(m, 1, obj_px, n)
     it_gbs = bpf * n / (ms * 1e-3) / 1e9
     out = {
@@ -287,7 +319,8 @@ def main() -> int:
                                "1 probe mode, FP16 amplitudes, probe update on",
                    "frames": n, "frame": m, "canvas": [rows, cols], "modes": 1},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": kernel_bytes(name, n, m, obj_px)},
         "iteration_roofline": {"bytes_per_frame": bpf, "achieved": it_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": it_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: float(v) for k, v in zip(KERNELS, kt)},

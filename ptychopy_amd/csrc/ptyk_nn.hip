@@ -31,7 +31,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 namespace {
 
 constexpr int GT = 256;  // threads per GEMM block (4 waves)
-constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int TM = 64, TN = 64, TK = 32;
 constexpr int LDP = TK + 8;  // LDS row pitch (bf16) of the A and transposed-B tiles
 
 // NCHW view: element (b, c, y, x) at p[((b * ctot) + c) * hw + y * w + x]; p already offset to
@@ -72,107 +72,12 @@ __device__ __forceinline__ bf16 load_a(const GemmArgs& a, int m, int k) {
   return to_bf(((const float*)a.y.p)[(((long)b * a.y.ctot + m) << a.hw_sh) + p]);
 }
 
-// Operands of one K step held in registers between the global fetch and the LDS store.
-struct TileRegs {
-  uint4 av[TM * TK / 8 / GT];  // FWD with 16-byte weight rows
-  bf16 a[TM * TK / GT];
-  bf16 b[TN * TK / GT];
-};
-
-template <int MODE, int KS>
-__device__ __forceinline__ void fetch_tile(const GemmArgs& a, TileRegs& r, int tid, int m0, int n0, int k0, int kend,
-                                           bool bn_ok, int bb, int by, int bx) {
-  constexpr int KK2 = KS * KS, R = KS / 2;
-  const int wmask = (1 << a.w_sh) - 1, hwmask = (1 << a.hw_sh) - 1;
-  if (MODE == GEMM_FWD && a.vec_a) {
-#pragma unroll
-    for (int h = 0; h < TM * TK / 8 / GT; ++h) {
-      const int idx = tid + h * GT, mm = idx / (TK / 8), kk = (idx % (TK / 8)) * 8;
-      const int m = m0 + mm, k = k0 + kk;
-      r.av[h] = (m < a.M && k < a.K) ? *reinterpret_cast<const uint4*>(a.wt + (long)m * a.K + k)
-                                     : make_uint4(0u, 0u, 0u, 0u);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < TM * TK / GT; ++i) {
-      const int e = tid + i * GT, mm = e / TK, kk = e % TK;
-      const int k = k0 + kk;
-      r.a[i] = k < kend ? load_a<MODE, KS>(a, m0 + mm, k) : to_bf(0.f);
-    }
-  }
-  const View& v = a.x;
-  if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
-#pragma unroll
-    for (int i = 0; i < TN * TK / GT; ++i) {
-      const int k = k0 + (tid >> 6) + i * (GT / TN);
-      bf16 val = to_bf(0.f);
-      if (bn_ok && k < kend) {
-        const int ch = k / KK2, d = k - ch * KK2;
-        int dy = d / KS - R, dx = d % KS - R;
-        if (MODE == GEMM_BWD_DATA) {
-          dy = -dy;
-          dx = -dx;
-        }
-        const int sy = by + dy, sx = bx + dx;
-        if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w) {
-          const long idx = (((long)bb * v.ctot + ch) << a.hw_sh) + (sy << a.w_sh) + sx;
-          val = MODE == GEMM_FWD ? ((const bf16*)v.p)[idx] : to_bf(((const float*)v.p)[idx]);
-        }
-      }
-      r.b[i] = val;
-    }
-  } else {  // BWD_W: B((b, p), (ci, d)) = x[b][ci][p + d]; consecutive threads along k
-#pragma unroll
-    for (int i = 0; i < TN * TK / GT; ++i) {
-      const int e = tid + i * GT, kk = e % TK, nn = e / TK;
-      const int k = k0 + kk, n = n0 + nn;
-      bf16 val = to_bf(0.f);
-      if (k < kend && n < a.N) {
-        const int b = k >> a.hw_sh, p = k & hwmask, yy = p >> a.w_sh, xx = p & wmask;
-        const int ci = n / KK2, d = n - ci * KK2;
-        const int sy = yy + d / KS - R, sx = xx + d % KS - R;
-        if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w)
-          val = ((const bf16*)v.p)[(((long)b * v.ctot + ci) << a.hw_sh) + (sy << a.w_sh) + sx];
-      }
-      r.b[i] = val;
-    }
-  }
-}
-
-template <int MODE>
-__device__ __forceinline__ void store_tile(const GemmArgs& a, const TileRegs& r, int tid, bf16* As, bf16* Bt) {
-  if (MODE == GEMM_FWD && a.vec_a) {
-#pragma unroll
-    for (int h = 0; h < TM * TK / 8 / GT; ++h) {
-      const int idx = tid + h * GT, mm = idx / (TK / 8), kk = (idx % (TK / 8)) * 8;
-      *reinterpret_cast<uint4*>(&As[mm * LDP + kk]) = r.av[h];
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < TM * TK / GT; ++i) {
-      const int e = tid + i * GT;
-      As[(e / TK) * LDP + e % TK] = r.a[i];
-    }
-  }
-  if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
-#pragma unroll
-    for (int i = 0; i < TN * TK / GT; ++i) Bt[(tid & (TN - 1)) * LDP + (tid >> 6) + i * (GT / TN)] = r.b[i];
-  } else {
-#pragma unroll
-    for (int i = 0; i < TN * TK / GT; ++i) {
-      const int e = tid + i * GT;
-      Bt[(e / TK) * LDP + e % TK] = r.b[i];
-    }
-  }
-}
-
-// Implicit-GEMM convolution tile: D[m0 .. m0+64) x [n0 .. n0+64) over K in steps of 64, the
-// next step's operands fetched into registers while the matrix cores work on the current one
-// (two LDS buffers, one barrier per step).
+// Implicit-GEMM convolution tile: D[m0 .. m0+64) x [n0 .. n0+64) over K in steps of 32.
 template <int MODE, int KS>
 __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
-  __shared__ bf16 As[2][TM * LDP];
-  __shared__ bf16 Bt[2][TN * LDP];  // transposed: Bt[n][k]
+  constexpr int KK2 = KS * KS, R = KS / 2;
+  __shared__ bf16 As[TM * LDP];
+  __shared__ bf16 Bt[TN * LDP];  // transposed: Bt[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int kbeg = MODE == GEMM_BWD_W ? blockIdx.z * a.kchunk : 0;
@@ -186,32 +91,75 @@ __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
-  TileRegs r;
-  int buf = 0;
-  if (kbeg < kend) {
-    fetch_tile<MODE, KS>(a, r, tid, m0, n0, kbeg, kend, bn_ok, bb, by, bx);
-    store_tile<MODE>(a, r, tid, As[0], Bt[0]);
-  }
-  __syncthreads();
   for (int k0 = kbeg; k0 < kend; k0 += TK) {
-    const bool more = k0 + TK < kend;
-    if (more) fetch_tile<MODE, KS>(a, r, tid, m0, n0, k0 + TK, kend, bn_ok, bb, by, bx);
+    // A tile 64 x 32
+    if (MODE == GEMM_FWD && a.vec_a) {
+      const int mm = tid >> 2, kk = (tid & 3) * 8;
+      const int m = m0 + mm, k = k0 + kk;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (m < a.M && k < a.K) v = *reinterpret_cast<const uint4*>(a.wt + (long)m * a.K + k);
+      *reinterpret_cast<uint4*>(&As[mm * LDP + kk]) = v;
+    } else {
 #pragma unroll
-    for (int ks = 0; ks < TK / 32; ++ks)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const v8bf va =
-            *reinterpret_cast<const v8bf*>(&As[buf][(wm + i * 16 + (lane & 15)) * LDP + 32 * ks + 8 * (lane >> 4)]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const v8bf vb =
-              *reinterpret_cast<const v8bf*>(&Bt[buf][(wn + j * 16 + (lane & 15)) * LDP + 32 * ks + 8 * (lane >> 4)]);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[i][j], 0, 0, 0);
-        }
+      for (int i = 0; i < TM * TK / GT; ++i) {
+        const int e = tid + i * GT;
+        const int mm = e / TK, kk = e % TK;
+        const int k = k0 + kk;
+        As[mm * LDP + kk] = k < kend ? load_a<MODE, KS>(a, m0 + mm, k) : to_bf(0.f);
       }
-    if (more) store_tile<MODE>(a, r, tid, As[buf ^ 1], Bt[buf ^ 1]);
+    }
+    // B tile 32 x 64
+    if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
+      const View& v = a.x;
+#pragma unroll
+      for (int i = 0; i < TN * TK / GT; ++i) {
+        const int kk = (tid >> 6) + i * (GT / TN);
+        const int k = k0 + kk;
+        bf16 val = to_bf(0.f);
+        if (bn_ok && k < kend) {
+          const int ch = k / KK2, d = k - ch * KK2;
+          int dy = d / KS - R, dx = d % KS - R;
+          if (MODE == GEMM_BWD_DATA) {
+            dy = -dy;
+            dx = -dx;
+          }
+          const int sy = by + dy, sx = bx + dx;
+          if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w) {
+            const long idx = (((long)bb * v.ctot + ch) << a.hw_sh) + (sy << a.w_sh) + sx;
+            val = MODE == GEMM_FWD ? ((const bf16*)v.p)[idx] : to_bf(((const float*)v.p)[idx]);
+          }
+        }
+        Bt[(tid & (TN - 1)) * LDP + kk] = val;
+      }
+    } else {  // BWD_W: B((b, p), (ci, d)) = x[b][ci][p + d]; consecutive threads along k
+      const View& v = a.x;
+#pragma unroll
+      for (int i = 0; i < TN * TK / GT; ++i) {
+        const int e = tid + i * GT;
+        const int kk = e % TK, nn = e / TK;
+        const int k = k0 + kk, n = n0 + nn;
+        bf16 val = to_bf(0.f);
+        if (k < kend && n < a.N) {
+          const int b = k >> a.hw_sh, p = k & hwmask, yy = p >> a.w_sh, xx = p & wmask;
+          const int ci = n / KK2, d = n - ci * KK2;
+          const int sy = yy + d / KS - R, sx = xx + d % KS - R;
+          if (sy >= 0 && sy < v.h && sx >= 0 && sx < v.w)
+            val = ((const bf16*)v.p)[(((long)b * v.ctot + ci) << a.hw_sh) + (sy << a.w_sh) + sx];
+        }
+        Bt[nn * LDP + kk] = val;
+      }
+    }
     __syncthreads();
-    buf ^= 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const v8bf va = *reinterpret_cast<const v8bf*>(&As[(wm + i * 16 + (lane & 15)) * LDP + 8 * (lane >> 4)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const v8bf vb = *reinterpret_cast<const v8bf*>(&Bt[(wn + j * 16 + (lane & 15)) * LDP + 8 * (lane >> 4)]);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
   }
   // epilogue: lane holds D[4 (lane / 16) + r][lane % 16] of each 16 x 16 sub-tile
 #pragma unroll
@@ -219,11 +167,11 @@ __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q;
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
         const int n = n0 + wn + j * 16 + (lane & 15);
         if (m >= a.M || n >= a.N) continue;
-        float v = acc[i][j][q];
+        float v = acc[i][j][r];
         if (MODE == GEMM_BWD_W) {
           a.partial[((long)blockIdx.z * a.M + m) * a.N + n] = v;
         } else {

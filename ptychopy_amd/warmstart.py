@@ -210,3 +210,105 @@ def warm_start(engine: _engine.Engine, net: Net, amplitudes, frames, rel_thresho
     pred = net.predict_amplitudes(np.asarray(amplitudes)[fr])
     engine.stitch_patches(pred, fr, rel_threshold, fill)
     return pred
+
+
+def _mls_to_error(e: _engine.Engine, W: dict, vmode, evo1, fp, target: float, max_iter: int, seed: int,
+                  warm=None):
+    """MLs from the engine's current object until e <= target or max_iter; wall time of the
+    solver (and, for a warm start, of inference and stitching, run inside the clock)."""
+    import time
+    n = W["n"]
+    e.set_probe(W["probe_true"])
+    e.mls_begin(_engine.mls_config(W["box"]), W["pos"], vmode, evo1, fp)
+    t = 0.0
+    if warm is not None:
+        net, amp, frames = warm
+        t0 = time.perf_counter()
+        warm_start(e, net, amp, frames)
+        e.set_probe(W["probe_true"])
+        e.mls_begin(_engine.mls_config(W["box"]), W["pos"], vmode, evo1, fp)
+        e.synchronize()
+        t += time.perf_counter() - t0
+    from . import _driver
+    parts = _driver.mls_partitions(n, max_iter, seed)
+    rng = np.random.default_rng(seed)
+    hist = [(0, 0.0, e.object_error()[0])]
+    reached = None
+    for it in range(1, max_iter + 1):
+        p = parts[it % len(parts)]
+        t0 = time.perf_counter()
+        e.mls_iteration(it, [p[j] for j in rng.permutation(len(p))])
+        e.synchronize()
+        t += time.perf_counter() - t0
+        err = e.object_error()[0]
+        hist.append((it, t, err))
+        if reached is None and err <= target:
+            reached = (it, t)
+            break
+    return reached, hist
+
+
+def c4_experiment(n_train: int = 50000, n_held: int = 5000, epochs: int = 2, lr: float = 3e-4, batch: int = 16,
+                  c: int = 32, target: float = 1e-2, max_iter: int = 100, stride: int = 2, seed: int = 0,
+                  log=None) -> dict:
+    """Learned warm start against a plain uniform start on the 51 x 51 / 256^2 / 5-mode MLs
+    workload: train on simulated random-field frames (phase std 0.2-1.0 rad) under the workload's
+    probe modes with a beam-weighted loss, report the held-out loss beside the constant
+    predictor's, then run MLs from both starts to the target normalised object error."""
+    from . import workloads
+    W = workloads.mls_c2()
+    n, m, k = W["n"], W["m"], W["k"]
+    modes = W["probe_true"]
+    pint = (np.abs(modes.astype(np.complex128)) ** 2).sum(axis=0)
+    wgt = (pint / pint.max()).astype(np.float32)
+    net = Net(size=m, c=c, batch=batch, seed=seed + 1)
+    net.set_loss_weight(wgt)
+    # training and held-out frames, streamed into the network's sample store; the held-out
+    # targets' weighted moments give the constant predictor's loss
+    rng = np.random.default_rng(seed + 5)
+    net.reserve_data(n_train + n_held)
+    s1 = np.zeros(2)
+    s2 = np.zeros(2)
+    swt = 0.0
+    with _engine.Engine(1, m, (m, m)) as es:
+        done = 0
+        while done < n_train + n_held:
+            cnt = min(200, n_train + n_held - done)
+            x, t = _simulated_chunk(es, modes, 4 * m, cnt, W["photons"], rng, (0.2, 1.0))
+            net.put_data(done, x, t)
+            lo = max(done, n_train) - done
+            if lo < cnt:
+                tt = t[lo:].astype(np.float64)
+                s1 += (tt * wgt).sum(axis=(0, 2, 3))
+                s2 += (tt * tt * wgt).sum(axis=(0, 2, 3))
+                swt += float(wgt.sum()) * (cnt - lo)
+            done += cnt
+            if log and done % 5000 < cnt:
+                log(f"simulated {done}/{n_train + n_held}")
+    const = float(((s2 - s1 * s1 / swt) / (2 * swt)).sum())
+    hist = fit(net, n_train, epochs, lr, seed=seed, held_out=n_held, log=log)
+    e = _engine.Engine(n, m, W["canvas"], n_modes=k)
+    e.set_frames(W["amp"])
+    amp = e.get_frames()
+    e.set_true_object(W["obj_true"])
+    r = np.random.default_rng(seed)
+    vmode = (r.uniform(0, 1, (m, m)) * np.exp(2j * np.pi * r.uniform(0, 1, (m, m)))).astype(np.complex64)
+    evo1 = 1e-6 * r.standard_normal(n)
+    fp = (amp.astype(np.float64) ** 2).sum(axis=(1, 2))
+    side = int(round(np.sqrt(n)))
+    frames = lattice_frames(side, side, stride)
+    e.set_object(np.ones(W["canvas"], np.complex64))
+    plain, plain_hist = _mls_to_error(e, W, vmode, evo1, fp, target, max_iter, seed)
+    warm, warm_hist = _mls_to_error(e, W, vmode, evo1, fp, target, max_iter, seed, warm=(net, amp, frames))
+    e.close()
+    net.close()
+    best = lambda h: min(v for _, _, v in h)
+    return {
+        "train_frames": n_train, "held_out_frames": n_held, "epochs": epochs,
+        "held_out_loss": hist[-1][1], "constant_predictor_loss": const, "train_loss": hist[-1][0],
+        "target_error": target, "max_iterations": max_iter, "inference_frames": int(len(frames)),
+        "plain": {"reached": plain, "e0": plain_hist[0][2], "best": best(plain_hist),
+                  "trace": plain_hist[::5]},
+        "warm": {"reached": warm, "e0": warm_hist[0][2], "best": best(warm_hist), "trace": warm_hist[::5]},
+        "speedup": (plain[1] / warm[1]) if (plain and warm) else None,
+    }
