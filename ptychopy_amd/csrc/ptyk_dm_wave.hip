@@ -55,32 +55,37 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
   using IO = RowIO<M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
-  float4 sv[IO::PAIRS], pv[IO::PAIRS];
-  float2 o0[IO::PAIRS], o1[IO::PAIRS];
+  // two halves of the tile, each with all of its loads in flight before its LDS stores
+  constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
 #pragma unroll
-  for (int i = 0; i < IO::PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    long f;
-    int r, c;
-    if (IO::locate(e, f0, n_groups, &f, &r, &c)) {
-      const int2 o = roi[f];
-      const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
-      sv[i] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
-      pv[i] = *reinterpret_cast<const float4*>(probe + r * M + c);
-      o0[i] = orow[0];
-      o1[i] = orow[1];
-    } else {
-      sv[i] = pv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      o0[i] = o1[i] = make_float2(0.f, 0.f);
+  for (int h0 = 0; h0 < IO::PAIRS; h0 += HALF) {
+    float4 sv[HALF], pv[HALF];
+    float2 o0[HALF], o1[HALF];
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      long f;
+      int r, c;
+      if (IO::locate(e, f0, n_groups, &f, &r, &c)) {
+        const int2 o = roi[f];
+        const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
+        sv[u] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
+        pv[u] = *reinterpret_cast<const float4*>(probe + r * M + c);
+        o0[u] = orow[0];
+        o1[u] = orow[1];
+      } else {
+        sv[u] = pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        o0[u] = o1[u] = make_float2(0.f, 0.f);
+      }
     }
-  }
 #pragma unroll
-  for (int i = 0; i < IO::PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const float2 a = cmul(make_float2(pv[i].x, pv[i].y), o0[i]);
-    const float2 b = cmul(make_float2(pv[i].z, pv[i].w), o1[i]);
-    *reinterpret_cast<float4*>(&lds[e]) =
-        make_float4(2.f * a.x - sv[i].x, 2.f * a.y - sv[i].y, 2.f * b.x - sv[i].z, 2.f * b.y - sv[i].w);
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      const float2 a = cmul(make_float2(pv[u].x, pv[u].y), o0[u]);
+      const float2 b = cmul(make_float2(pv[u].z, pv[u].w), o1[u]);
+      *reinterpret_cast<float4*>(&lds[e]) =
+          make_float4(2.f * a.x - sv[u].x, 2.f * a.y - sv[u].y, 2.f * b.x - sv[u].z, 2.f * b.y - sv[u].w);
+    }
   }
   __syncthreads();
   fft_lds<M, false, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
