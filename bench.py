@@ -242,6 +242,9 @@ def main() -> int:
                "data": "synthetic (random-field training objects, phase std 0.2-1.0 rad; the MLs workload's "
                        "seeded object and Poisson frames)",
                "config": {"workload": "learned warm start + MLs, 51x51 raster, 256x256 frames, 5 probe modes"},
+               "roofline": {"bound": "mfma", "kernel": "network training step (all convolutions)",
+                            "achieved": r["train_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                            "frac": r["train_tflops"] / 2500.0, "traffic": None},
                "c4": r, "wall_s": time.perf_counter() - t0}
         print(json.dumps(out))
         return 0

@@ -190,10 +190,208 @@ __global__ __launch_bounds__(GT) void k_conv_gemm(GemmArgs a) {
       }
 }
 
+// ---- vector-operand form (image width >= 8): each thread moves 8 consecutive pixels of one
+// image row per K step -- one aligned 16-byte (or two for FP32) load plus the neighbour element
+// a +-1 tap shift needs -- instead of 8 gathered elements.
+typedef unsigned short u16;
+
+__device__ __forceinline__ u16 bf_bits(float v) { return __builtin_bit_cast(u16, to_bf(v)); }
+
+// row[x0 + dx .. x0 + dx + 7] (bf16 bits), zero outside [0, w); x0 % 8 == 0, |dx| <= 1
+__device__ __forceinline__ void row8_bf16(const u16* __restrict__ row, int x0, int dx, int w, u16 out[8]) {
+  union {
+    uint4 v;
+    u16 h[8];
+  } u;
+  u.v = *reinterpret_cast<const uint4*>(row + x0);
+  if (dx == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = u.h[i];
+  } else if (dx < 0) {
+    out[0] = x0 > 0 ? row[x0 - 1] : (u16)0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) out[i] = u.h[i - 1];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) out[i] = u.h[i + 1];
+    out[7] = x0 + 8 < w ? row[x0 + 8] : (u16)0;
+  }
+}
+
+// the same for an FP32 row, rounded to bf16
+__device__ __forceinline__ void row8_f32(const float* __restrict__ row, int x0, int dx, int w, u16 out[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(row + x0);
+  const float4 b = *reinterpret_cast<const float4*>(row + x0 + 4);
+  const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  if (dx == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = bf_bits(v[i]);
+  } else if (dx < 0) {
+    out[0] = x0 > 0 ? bf_bits(row[x0 - 1]) : (u16)0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) out[i] = bf_bits(v[i - 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) out[i] = bf_bits(v[i + 1]);
+    out[7] = x0 + 8 < w ? bf_bits(row[x0 + 8]) : (u16)0;
+  }
+}
+
+// Same tile and MFMA schedule as k_conv_gemm (so the same sums in the same order). a.wt holds
+// the weights for FWD and their [C_in][C_out x taps] transpose for BWD_DATA.
+template <int MODE, int KS>
+__global__ __launch_bounds__(GT) void k_conv_gemm_vec(GemmArgs a) {
+  constexpr int KK2 = KS * KS, R = KS / 2;
+  __shared__ __align__(16) u16 As[TM * LDP];
+  __shared__ __align__(16) u16 Bt[TN * LDP];  // transposed: Bt[n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int kbeg = MODE == GEMM_BWD_W ? blockIdx.z * a.kchunk : 0;
+  const int kend = MODE == GEMM_BWD_W ? min(a.K, kbeg + a.kchunk) : a.K;
+  const int w = 1 << a.w_sh, wmask = w - 1, hwmask = (1 << a.hw_sh) - 1;
+  const View& v = a.x;
+  // FWD / BWD_DATA B roles: k row tid / 8 of the step, pixels nb .. nb + 7 of one image row
+  const int nb = n0 + ((tid & 7) << 3);
+  const bool nb_ok = nb < a.N;
+  const int bb = nb >> a.hw_sh, bp = nb & hwmask, by = bp >> a.w_sh, bx0 = bp & wmask;
+  v4f acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
+    // ---- A tile 64 x 32
+    if (MODE != GEMM_BWD_W) {
+      const u16* wt = reinterpret_cast<const u16*>(a.wt);
+      if (a.vec_a) {
+        const int mm = tid >> 2, kk = (tid & 3) * 8;
+        const int m = m0 + mm, k = k0 + kk;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (m < a.M && k < a.K) val = *reinterpret_cast<const uint4*>(wt + (long)m * a.K + k);
+        *reinterpret_cast<uint4*>(&As[mm * LDP + kk]) = val;
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM * TK / GT; ++i) {
+          const int e = tid + i * GT;
+          const int mm = e / TK, kk = e % TK;
+          const int m = m0 + mm, k = k0 + kk;
+          As[mm * LDP + kk] = (m < a.M && k < a.K) ? wt[(long)m * a.K + k] : (u16)0;
+        }
+      }
+    } else {  // A(co, (b, p)) = dZ[b][co][p]: 8 consecutive pixels of one image
+      const int mm = tid >> 2, kg = (tid & 3) * 8;
+      const int m = m0 + mm, k = k0 + kg;
+      u16 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (m < a.M && k < kend) {
+        const int b = k >> a.hw_sh, p = k & hwmask;
+        row8_f32((const float*)a.y.p + (((long)b * a.y.ctot + m) << a.hw_sh), p, 0, 1 << a.hw_sh, h);
+      }
+      uint4 val;
+      val.x = h[0] | ((unsigned)h[1] << 16);
+      val.y = h[2] | ((unsigned)h[3] << 16);
+      val.z = h[4] | ((unsigned)h[5] << 16);
+      val.w = h[6] | ((unsigned)h[7] << 16);
+      *reinterpret_cast<uint4*>(&As[mm * LDP + kg]) = val;
+    }
+    // ---- B tile 32 x 64 (stored transposed)
+    if (MODE == GEMM_FWD || MODE == GEMM_BWD_DATA) {
+      const int kk = tid >> 3, k = k0 + kk;
+      u16 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (nb_ok && k < kend) {
+        const int ch = k / KK2, d = k - ch * KK2;
+        int dy = d / KS - R, dx = d % KS - R;
+        if (MODE == GEMM_BWD_DATA) {
+          dy = -dy;
+          dx = -dx;
+        }
+        const int sy = by + dy;
+        if (sy >= 0 && sy < v.h) {
+          const long off = (((long)bb * v.ctot + ch) << a.hw_sh) + ((long)sy << a.w_sh);
+          if (MODE == GEMM_FWD) row8_bf16((const u16*)v.p + off, bx0, dx, w, h);
+          else row8_f32((const float*)v.p + off, bx0, dx, w, h);
+        }
+      }
+      const int ng = (tid & 7) << 3;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bt[(ng + i) * LDP + kk] = h[i];
+    } else {  // B((b, p), (ci, d)) = x[b][ci][p + d]: 8 consecutive pixels of one row
+      const int nn = tid >> 2, kg = (tid & 3) * 8;
+      const int n = n0 + nn, k = k0 + kg;
+      u16 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (n < a.N && k < kend) {
+        const int b = k >> a.hw_sh, p = k & hwmask, yy = p >> a.w_sh, x0 = p & wmask;
+        const int ci = n / KK2, d = n - ci * KK2;
+        const int sy = yy + d / KS - R, dx = d % KS - R;
+        if (sy >= 0 && sy < v.h)
+          row8_bf16((const u16*)v.p + (((long)b * v.ctot + ci) << a.hw_sh) + ((long)sy << a.w_sh), x0, dx, w, h);
+      }
+      uint4 val;
+      val.x = h[0] | ((unsigned)h[1] << 16);
+      val.y = h[2] | ((unsigned)h[3] << 16);
+      val.z = h[4] | ((unsigned)h[5] << 16);
+      val.w = h[6] | ((unsigned)h[7] << 16);
+      *reinterpret_cast<uint4*>(&Bt[nn * LDP + kg]) = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const v8bf va = *reinterpret_cast<const v8bf*>(&As[(wm + i * 16 + (lane & 15)) * LDP + 8 * (lane >> 4)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const v8bf vb = *reinterpret_cast<const v8bf*>(&Bt[(wn + j * 16 + (lane & 15)) * LDP + 8 * (lane >> 4)]);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q;
+        const int n = n0 + wn + j * 16 + (lane & 15);
+        if (m >= a.M || n >= a.N) continue;
+        float val = acc[i][j][q];
+        if (MODE == GEMM_BWD_W) {
+          a.partial[((long)blockIdx.z * a.M + m) * a.N + n] = val;
+        } else {
+          const View& o = a.y;
+          const int b = n >> a.hw_sh, p = n & hwmask;
+          const long idx = (((long)b * o.ctot + m) << a.hw_sh) + p;
+          if (MODE == GEMM_FWD) {
+            val += a.bias[m];
+            if (a.relu) val = fmaxf(val, 0.f);
+            if (a.out_f32) ((float*)o.p)[idx] = val;
+            else ((bf16*)o.p)[idx] = to_bf(val);
+          } else {
+            ((float*)o.p)[idx] = val;
+          }
+        }
+      }
+}
+
+// WT[ci][co * taps + d] = W[co][ci][d] (bf16), the data-gradient GEMM's weight rows
+__global__ void k_transpose_w(const bf16* __restrict__ w, bf16* __restrict__ wt, int cout, int cin, int kk2) {
+  const long total = (long)cout * cin * kk2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int d = (int)(i % kk2);
+    const long r = i / kk2;
+    const int ci = (int)(r % cin), co = (int)(r / cin);
+    wt[((long)ci * cout + co) * kk2 + d] = w[i];
+  }
+}
+
+// vec: the vector-operand form (image width >= 8; for BWD_DATA a.wt is the transposed copy)
 template <int MODE>
-static void launch_gemm(const GemmArgs& a, int ks, dim3 g, hipStream_t st) {
-  if (ks == 3) hipLaunchKernelGGL((k_conv_gemm<MODE, 3>), g, dim3(GT), 0, st, a);
-  else hipLaunchKernelGGL((k_conv_gemm<MODE, 1>), g, dim3(GT), 0, st, a);
+static void launch_gemm(const GemmArgs& a, int ks, dim3 g, hipStream_t st, bool vec) {
+  if (vec) {
+    if (ks == 3) hipLaunchKernelGGL((k_conv_gemm_vec<MODE, 3>), g, dim3(GT), 0, st, a);
+    else hipLaunchKernelGGL((k_conv_gemm_vec<MODE, 1>), g, dim3(GT), 0, st, a);
+  } else {
+    if (ks == 3) hipLaunchKernelGGL((k_conv_gemm<MODE, 3>), g, dim3(GT), 0, st, a);
+    else hipLaunchKernelGGL((k_conv_gemm<MODE, 1>), g, dim3(GT), 0, st, a);
+  }
 }
 
 static int ilog2(int v) {
@@ -349,6 +547,7 @@ struct NnNet {
   long nparam = 0;
   float *w = nullptr, *mo = nullptr, *ve = nullptr, *g = nullptr;
   bf16* wb = nullptr;
+  bf16* wbt = nullptr;  // per layer [C_in][C_out x taps] transpose of wb (data-gradient weights)
   float* partial = nullptr;
   long partial_cap = 0;
   double* lpart = nullptr;
@@ -419,7 +618,7 @@ static hipError_t conv_fwd(NnNet& n, int li, View x, View y, bool out_f32) {
   a.hw_sh = ilog2(x.h * x.w);
   a.vec_a = (a.K % 8 == 0 && L.woff % 8 == 0 && ((uintptr_t)n.wb & 15) == 0) ? 1 : 0;
   dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, 1);
-  launch_gemm<GEMM_FWD>(a, L.ks, g, n.stream);
+  launch_gemm<GEMM_FWD>(a, L.ks, g, n.stream, x.w >= 8);
   return hipGetLastError();
 }
 
@@ -446,7 +645,7 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     if ((long)chunks * a.M * a.N > n.partial_cap) return hipErrorInvalidValue;
     a.partial = n.partial;
     dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, chunks);
-    launch_gemm<GEMM_BWD_W>(a, L.ks, g, n.stream);
+    launch_gemm<GEMM_BWD_W>(a, L.ks, g, n.stream, x.w >= 8);
     const long mn = (long)a.M * a.N;
     hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, n.stream, n.partial, chunks,
                        mn, n.g + L.woff);
@@ -456,7 +655,8 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     GemmArgs a{};
     a.x = dz;
     a.y = *dx;
-    a.wt = n.wb + L.woff;
+    const bool vec = dz.w >= 8;
+    a.wt = (vec ? n.wbt : n.wb) + L.woff;
     a.batch = n.batch;
     a.cin = L.cin;
     a.cout = L.cout;
@@ -465,8 +665,9 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     a.K = L.cout * kk2;
     a.w_sh = ilog2(dz.w);
     a.hw_sh = ilog2(dz.h * dz.w);
+    a.vec_a = (a.K % 8 == 0 && L.woff % 8 == 0 && ((uintptr_t)n.wbt & 15) == 0) ? 1 : 0;
     dim3 g((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, 1);
-    launch_gemm<GEMM_BWD_DATA>(a, L.ks, g, n.stream);
+    launch_gemm<GEMM_BWD_DATA>(a, L.ks, g, n.stream, vec);
   }
   return hipGetLastError();
 }
@@ -528,6 +729,11 @@ hipError_t nn_loss(NnNet& n, bool with_grad, int* nparts) {
 hipError_t nn_backward(NnNet& n) {
   const int c = n.c;
   hipError_t e;
+  for (const NnLayer& L : n.layers) {
+    const long cnt = (long)L.cout * L.cin * L.ks * L.ks;
+    hipLaunchKernelGGL(k_transpose_w, grid_of(cnt), dim3(256), 0, n.stream, n.wb + L.woff, n.wbt + L.woff, L.cout,
+                       L.cin, L.ks * L.ks);
+  }
   auto relu_g = [&](View g, View a) {
     hipLaunchKernelGGL(k_relu_grad, grid_of(view_count(g, n.batch)), dim3(256), 0, n.stream, g, a, n.batch);
   };
@@ -654,6 +860,7 @@ hipError_t nn_create(NnNet& n, int s, int c, int batch) {
   if (e == hipSuccess) e = nn_alloc1((void**)&n.ve, n.nparam * sizeof(float));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.g, n.nparam * sizeof(float));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.wb, n.nparam * sizeof(bf16));
+  if (e == hipSuccess) e = nn_alloc1((void**)&n.wbt, n.nparam * sizeof(bf16));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.lpart, 1024 * sizeof(double));
   // weight-gradient partial tiles: the largest chunks x Cout x Cin ks^2 over the layers
   static const int level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
@@ -682,7 +889,7 @@ void nn_free(NnNet& n) {
   for (auto& p : n.arg)
     if (p) hipFree(p);
   for (void* p : {(void*)n.out, (void*)n.gout, (void*)n.target, (void*)n.w, (void*)n.mo, (void*)n.ve, (void*)n.g,
-                  (void*)n.wb, (void*)n.lpart, (void*)n.partial, (void*)n.lw})
+                  (void*)n.wb, (void*)n.wbt, (void*)n.lpart, (void*)n.partial, (void*)n.lw})
     if (p) hipFree(p);
 }
 

@@ -129,6 +129,16 @@ class Net:
         return loss.value, g
 
 
+def net_forward_flops(s: int, c: int) -> float:
+    """Multiply-add FLOPs of one forward pass (2 x C_in x C_out x taps per output pixel, summed over
+    the 15 convolutions); a training step costs about three times this."""
+    tab = [(1, c, 3, 0), (c, c, 3, 0), (c, 2 * c, 3, 1), (2 * c, 2 * c, 3, 1), (2 * c, 4 * c, 3, 2),
+           (4 * c, 4 * c, 3, 2), (4 * c, 8 * c, 3, 3), (8 * c, 8 * c, 3, 3), (12 * c, 4 * c, 3, 2),
+           (4 * c, 4 * c, 3, 2), (6 * c, 2 * c, 3, 1), (2 * c, 2 * c, 3, 1), (3 * c, c, 3, 0), (c, c, 3, 0),
+           (c, 2, 1, 0)]
+    return float(sum(2 * ci * co * k * k * (s >> lv) ** 2 for ci, co, k, lv in tab))
+
+
 def preprocess(amp) -> np.ndarray:
     """log(1 + A) scaled per frame to a maximum of 1 (the network's input)."""
     a = np.log1p(np.maximum(np.asarray(amp, np.float32), 0))
@@ -286,7 +296,10 @@ def c4_experiment(n_train: int = 50000, n_held: int = 5000, epochs: int = 2, lr:
             if log and done % 5000 < cnt:
                 log(f"simulated {done}/{n_train + n_held}")
     const = float(((s2 - s1 * s1 / swt) / (2 * swt)).sum())
+    import time
+    t0 = time.perf_counter()
     hist = fit(net, n_train, epochs, lr, seed=seed, held_out=n_held, log=log)
+    t_fit = time.perf_counter() - t0
     e = _engine.Engine(n, m, W["canvas"], n_modes=k)
     e.set_frames(W["amp"])
     amp = e.get_frames()
@@ -303,8 +316,12 @@ def c4_experiment(n_train: int = 50000, n_held: int = 5000, epochs: int = 2, lr:
     e.close()
     net.close()
     best = lambda h: min(v for _, _, v in h)
+    gf = net_forward_flops(m, c)
+    train_rate = epochs * (n_train // batch * batch) / t_fit  # includes the held-out passes
     return {
         "train_frames": n_train, "held_out_frames": n_held, "epochs": epochs,
+        "train_samples_per_s": train_rate, "train_tflops": 3 * gf * train_rate / 1e12,
+        "forward_gflop_per_sample": gf / 1e9,
         "held_out_loss": hist[-1][1], "constant_predictor_loss": const, "train_loss": hist[-1][0],
         "target_error": target, "max_iterations": max_iter, "inference_frames": int(len(frames)),
         "plain": {"reached": plain, "e0": plain_hist[0][2], "best": best(plain_hist),
