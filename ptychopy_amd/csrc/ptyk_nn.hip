@@ -33,6 +33,7 @@ namespace {
 constexpr int GT = 256;  // threads per GEMM block (4 waves)
 constexpr int TM = 64, TN = 64, TK = 32;
 constexpr int LDP = TK + 8;  // LDS row pitch (bf16) of the A and transposed-B tiles
+constexpr int BIAS_PARTS = 256;  // slices of the bias-gradient reduction per channel
 
 // NCHW view: element (b, c, y, x) at p[((b * ctot) + c) * hw + y * w + x]; p already offset to
 // the view's first channel.
@@ -409,36 +410,51 @@ __global__ void k_sum_chunks(const float* __restrict__ part, int chunks, long mn
   out[i] = s;
 }
 
-__global__ __launch_bounds__(256) void k_bias_grad(View dz, int batch, float* __restrict__ db) {
+// db[co] = sum over (b, p) of dZ: slice sums (grid cout x parts), then one fixed-order sum per
+// channel -- deterministic, and enough blocks to fill the GPU at full resolution
+__global__ __launch_bounds__(256) void k_bias_partial(View dz, int batch, int hw_sh, long slice,
+                                                      double* __restrict__ part) {
   __shared__ double red[4];
-  const int co = blockIdx.x;
-  const int hw = dz.h * dz.w;
+  const int co = blockIdx.x, parts = gridDim.y;
+  const long total = (long)batch << hw_sh, hwm = (1L << hw_sh) - 1;
+  const long lo = blockIdx.y * slice, hi = min(total, lo + slice);
   double s = 0.0;
-  for (int b = 0; b < batch; ++b)
-    for (int p = threadIdx.x; p < hw; p += 256) s += ((const float*)dz.p)[((long)b * dz.ctot + co) * hw + p];
+  for (long e = lo + threadIdx.x; e < hi; e += 256)
+    s += ((const float*)dz.p)[(((e >> hw_sh) * dz.ctot + co) << hw_sh) + (e & hwm)];
   s = block_sum<256>(s, red);
-  if (threadIdx.x == 0) db[co] = (float)s;
+  if (threadIdx.x == 0) part[(long)co * parts + blockIdx.y] = s;
 }
 
+__global__ __launch_bounds__(64) void k_bias_final(const double* __restrict__ part, int parts,
+                                                   float* __restrict__ db) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < parts; i += 64) s += part[(long)blockIdx.x * parts + i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) db[blockIdx.x] = (float)s;
+}
+
+// Elementwise layers on a (pixel blocks x planes) grid: plane = b * C + c (one block row per
+// sample and channel), pixels of the plane strided over blockIdx.x; widths are powers of two.
+__device__ __forceinline__ int lg2(int v) { return __ffs(v) - 1; }
+
 // g <- g * (a > 0): gradient through the ReLU (a is the post-activation, bf16)
-__global__ void k_relu_grad(View g, View a, int batch) {
-  const long hw = (long)g.h * g.w, per = (long)g.c * hw, total = per * batch;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long b = i / per, r = i - b * per, c = r / hw, p = r - c * hw;
-    float& gv = ((float*)g.p)[((long)b * g.ctot + c) * hw + p];
-    const float av = (float)((const bf16*)a.p)[((long)b * a.ctot + c) * hw + p];
-    if (!(av > 0.f)) gv = 0.f;
-  }
+__global__ void k_relu_grad(View g, View a) {
+  const int plane = blockIdx.y, b = plane / g.c, c = plane - b * g.c;
+  const int hw = g.h * g.w;
+  float* gp = (float*)g.p + ((long)b * g.ctot + c) * hw;
+  const bf16* ap = (const bf16*)a.p + ((long)b * a.ctot + c) * hw;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < hw; p += gridDim.x * blockDim.x)
+    if (!((float)ap[p] > 0.f)) gp[p] = 0.f;
 }
 
 // 2 x 2 max pooling; arg holds the winning offset (0..3) for the backward pass
-__global__ void k_pool_fwd(View in, View out, uint8_t* __restrict__ arg, int batch) {
-  const int oh = out.h, ow = out.w;
-  const long ohw = (long)oh * ow, per = (long)out.c * ohw, total = per * batch;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long b = i / per, r = i - b * per, c = r / ohw, p = r - c * ohw;
-    const int y = (int)(p / ow), x = (int)(p - (long)y * ow);
-    const bf16* src = (const bf16*)in.p + ((long)b * in.ctot + c) * in.h * in.w;
+__global__ void k_pool_fwd(View in, View out, uint8_t* __restrict__ arg) {
+  const int plane = blockIdx.y, b = plane / out.c, c = plane - b * out.c;
+  const int ohw = out.h * out.w, osh = lg2(out.w);
+  const bf16* src = (const bf16*)in.p + ((long)b * in.ctot + c) * in.h * in.w;
+  bf16* dst = (bf16*)out.p + ((long)b * out.ctot + c) * ohw;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ohw; p += gridDim.x * blockDim.x) {
+    const int y = p >> osh, x = p & (out.w - 1);
     float best = -INFINITY;
     int bi = 0;
     for (int q = 0; q < 4; ++q) {
@@ -448,43 +464,45 @@ __global__ void k_pool_fwd(View in, View out, uint8_t* __restrict__ arg, int bat
         bi = q;
       }
     }
-    ((bf16*)out.p)[((long)b * out.ctot + c) * ohw + p] = to_bf(best);
-    arg[i] = (uint8_t)bi;
+    dst[p] = to_bf(best);
+    arg[(long)plane * ohw + p] = (uint8_t)bi;
   }
 }
 
 // gin (f32, the skip gradient already there) += scatter of gout to the winning offsets
-__global__ void k_pool_bwd(View gout, View gin, const uint8_t* __restrict__ arg, int batch) {
-  const int ow = gout.w;
-  const long ohw = (long)gout.h * ow, per = (long)gout.c * ohw, total = per * batch;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long b = i / per, r = i - b * per, c = r / ohw, p = r - c * ohw;
-    const int y = (int)(p / ow), x = (int)(p - (long)y * ow);
-    const int q = arg[i];
-    float* dst = (float*)gin.p + ((long)b * gin.ctot + c) * gin.h * gin.w;
-    dst[(2 * y + (q >> 1)) * gin.w + 2 * x + (q & 1)] += ((const float*)gout.p)[((long)b * gout.ctot + c) * ohw + p];
+__global__ void k_pool_bwd(View gout, View gin, const uint8_t* __restrict__ arg) {
+  const int plane = blockIdx.y, b = plane / gout.c, c = plane - b * gout.c;
+  const int ohw = gout.h * gout.w, osh = lg2(gout.w);
+  const float* src = (const float*)gout.p + ((long)b * gout.ctot + c) * ohw;
+  float* dst = (float*)gin.p + ((long)b * gin.ctot + c) * gin.h * gin.w;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ohw; p += gridDim.x * blockDim.x) {
+    const int y = p >> osh, x = p & (gout.w - 1);
+    const int q = arg[(long)plane * ohw + p];
+    dst[(2 * y + (q >> 1)) * gin.w + 2 * x + (q & 1)] += src[p];
   }
 }
 
-// nearest x2 upsampling (bf16) and its gradient (sum of the four children)
-__global__ void k_up_fwd(View in, View out, int batch) {
-  const long ohw = (long)out.h * out.w, per = (long)in.c * ohw, total = per * batch;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long b = i / per, r = i - b * per, c = r / ohw, p = r - c * ohw;
-    const int y = (int)(p / out.w), x = (int)(p - (long)y * out.w);
-    ((bf16*)out.p)[((long)b * out.ctot + c) * ohw + p] =
-        ((const bf16*)in.p)[((long)b * in.ctot + c) * in.h * in.w + (y >> 1) * in.w + (x >> 1)];
+// nearest x2 upsampling (bf16) of in's channels into out's first channels, and its gradient
+// (sum of the four children)
+__global__ void k_up_fwd(View in, View out) {
+  const int plane = blockIdx.y, b = plane / in.c, c = plane - b * in.c;
+  const int ohw = out.h * out.w, osh = lg2(out.w);
+  const bf16* src = (const bf16*)in.p + ((long)b * in.ctot + c) * in.h * in.w;
+  bf16* dst = (bf16*)out.p + ((long)b * out.ctot + c) * ohw;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ohw; p += gridDim.x * blockDim.x) {
+    const int y = p >> osh, x = p & (out.w - 1);
+    dst[p] = src[(y >> 1) * in.w + (x >> 1)];
   }
 }
-__global__ void k_up_bwd(View gout, View gin, int batch) {
-  const long ihw = (long)gin.h * gin.w, per = (long)gin.c * ihw, total = per * batch;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long b = i / per, r = i - b * per, c = r / ihw, p = r - c * ihw;
-    const int y = (int)(p / gin.w), x = (int)(p - (long)y * gin.w);
-    const float* src = (const float*)gout.p + ((long)b * gout.ctot + c) * gout.h * gout.w;
-    const float s = src[(2 * y) * gout.w + 2 * x] + src[(2 * y) * gout.w + 2 * x + 1] +
-                    src[(2 * y + 1) * gout.w + 2 * x] + src[(2 * y + 1) * gout.w + 2 * x + 1];
-    ((float*)gin.p)[((long)b * gin.ctot + c) * ihw + p] = s;
+__global__ void k_up_bwd(View gout, View gin) {
+  const int plane = blockIdx.y, b = plane / gin.c, c = plane - b * gin.c;
+  const int ihw = gin.h * gin.w, ish = lg2(gin.w);
+  const float* src = (const float*)gout.p + ((long)b * gout.ctot + c) * gout.h * gout.w;
+  float* dst = (float*)gin.p + ((long)b * gin.ctot + c) * ihw;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ihw; p += gridDim.x * blockDim.x) {
+    const int y = p >> ish, x = p & (gin.w - 1);
+    dst[p] = src[(2 * y) * gout.w + 2 * x] + src[(2 * y) * gout.w + 2 * x + 1] + src[(2 * y + 1) * gout.w + 2 * x] +
+             src[(2 * y + 1) * gout.w + 2 * x + 1];
   }
 }
 
@@ -528,6 +546,12 @@ __global__ void k_to_bf(const float* __restrict__ s, bf16* __restrict__ d, long 
   if (i < n) d[i] = to_bf(s[i]);
 }
 
+// (pixel blocks, planes) grid of an elementwise layer over `batch` samples of view v
+inline dim3 plane_grid(const View& v, int batch) {
+  const long hw = (long)v.h * v.w;
+  return dim3((unsigned)std::min<long>((hw + 255) / 256, 64), (unsigned)(batch * v.c));
+}
+
 inline dim3 grid_of(long n) {
   long b = (n + 255) / 256;
   return dim3((unsigned)std::min<long>(b, 65535L * 4));
@@ -548,6 +572,7 @@ struct NnNet {
   float *w = nullptr, *mo = nullptr, *ve = nullptr, *g = nullptr;
   bf16* wb = nullptr;
   bf16* wbt = nullptr;  // per layer [C_in][C_out x taps] transpose of wb (data-gradient weights)
+  double* bpart = nullptr;  // bias-gradient slice sums, 8c x BIAS_PARTS
   float* partial = nullptr;
   long partial_cap = 0;
   double* lpart = nullptr;
@@ -649,7 +674,15 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
     const long mn = (long)a.M * a.N;
     hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, n.stream, n.partial, chunks,
                        mn, n.g + L.woff);
-    hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, n.stream, dz, n.batch, n.g + L.boff);
+    {
+      const int hw_sh = ilog2(dz.h * dz.w);
+      const long total = (long)n.batch << hw_sh;
+      const int parts = (int)std::max<long>(1, std::min<long>(BIAS_PARTS, total / 4096));
+      const long slice = (total + parts - 1) / parts;
+      hipLaunchKernelGGL(k_bias_partial, dim3(L.cout, parts), dim3(256), 0, n.stream, dz, n.batch, hw_sh, slice,
+                         n.bpart);
+      hipLaunchKernelGGL(k_bias_final, dim3(L.cout), dim3(64), 0, n.stream, n.bpart, parts, n.g + L.boff);
+    }
   }
   if (dx) {
     GemmArgs a{};
@@ -681,33 +714,33 @@ hipError_t nn_forward(NnNet& n) {
   if ((e = fwd(0, vw(n, B_X, 0, 1, false), vw(n, B_A0, 0, c, false)))) return e;
   if ((e = fwd(1, vw(n, B_A0, 0, c, false), vw(n, B_CAT0, 2 * c, c, false)))) return e;
   View p0 = vw(n, B_P0, 0, c, false);
-  hipLaunchKernelGGL(k_pool_fwd, grid_of(view_count(p0, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT0, 2 * c, c, false), p0, n.arg[0], n.batch);
+  hipLaunchKernelGGL(k_pool_fwd, plane_grid(p0, n.batch), dim3(256), 0, n.stream, vw(n, B_CAT0, 2 * c, c, false), p0,
+                     n.arg[0]);
   if ((e = fwd(2, p0, vw(n, B_A1, 0, 2 * c, false)))) return e;
   if ((e = fwd(3, vw(n, B_A1, 0, 2 * c, false), vw(n, B_CAT1, 4 * c, 2 * c, false)))) return e;
   View p1 = vw(n, B_P1, 0, 2 * c, false);
-  hipLaunchKernelGGL(k_pool_fwd, grid_of(view_count(p1, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT1, 4 * c, 2 * c, false), p1, n.arg[1], n.batch);
+  hipLaunchKernelGGL(k_pool_fwd, plane_grid(p1, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_CAT1, 4 * c, 2 * c, false), p1, n.arg[1]);
   if ((e = fwd(4, p1, vw(n, B_A2, 0, 4 * c, false)))) return e;
   if ((e = fwd(5, vw(n, B_A2, 0, 4 * c, false), vw(n, B_CAT2, 8 * c, 4 * c, false)))) return e;
   View p2 = vw(n, B_P2, 0, 4 * c, false);
-  hipLaunchKernelGGL(k_pool_fwd, grid_of(view_count(p2, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT2, 8 * c, 4 * c, false), p2, n.arg[2], n.batch);
+  hipLaunchKernelGGL(k_pool_fwd, plane_grid(p2, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_CAT2, 8 * c, 4 * c, false), p2, n.arg[2]);
   if ((e = fwd(6, p2, vw(n, B_A3, 0, 8 * c, false)))) return e;
   if ((e = fwd(7, vw(n, B_A3, 0, 8 * c, false), vw(n, B_BN, 0, 8 * c, false)))) return e;
   View up2 = vw(n, B_CAT2, 0, 8 * c, false);
-  hipLaunchKernelGGL(k_up_fwd, grid_of(view_count(up2, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_BN, 0, 8 * c, false), up2, n.batch);
+  hipLaunchKernelGGL(k_up_fwd, plane_grid(up2, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_BN, 0, 8 * c, false), up2);
   if ((e = fwd(8, vw(n, B_CAT2, 0, 12 * c, false), vw(n, B_A4, 0, 4 * c, false)))) return e;
   if ((e = fwd(9, vw(n, B_A4, 0, 4 * c, false), vw(n, B_D2, 0, 4 * c, false)))) return e;
   View up1 = vw(n, B_CAT1, 0, 4 * c, false);
-  hipLaunchKernelGGL(k_up_fwd, grid_of(view_count(up1, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_D2, 0, 4 * c, false), up1, n.batch);
+  hipLaunchKernelGGL(k_up_fwd, plane_grid(up1, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_D2, 0, 4 * c, false), up1);
   if ((e = fwd(10, vw(n, B_CAT1, 0, 6 * c, false), vw(n, B_A5, 0, 2 * c, false)))) return e;
   if ((e = fwd(11, vw(n, B_A5, 0, 2 * c, false), vw(n, B_D1, 0, 2 * c, false)))) return e;
   View up0 = vw(n, B_CAT0, 0, 2 * c, false);
-  hipLaunchKernelGGL(k_up_fwd, grid_of(view_count(up0, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_D1, 0, 2 * c, false), up0, n.batch);
+  hipLaunchKernelGGL(k_up_fwd, plane_grid(up0, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_D1, 0, 2 * c, false), up0);
   if ((e = fwd(12, vw(n, B_CAT0, 0, 3 * c, false), vw(n, B_A6, 0, c, false)))) return e;
   if ((e = fwd(13, vw(n, B_A6, 0, c, false), vw(n, B_D0, 0, c, false)))) return e;
   View out{n.out, 2, 2, n.s, n.s};
@@ -735,7 +768,7 @@ hipError_t nn_backward(NnNet& n) {
                        L.cin, L.ks * L.ks);
   }
   auto relu_g = [&](View g, View a) {
-    hipLaunchKernelGGL(k_relu_grad, grid_of(view_count(g, n.batch)), dim3(256), 0, n.stream, g, a, n.batch);
+    hipLaunchKernelGGL(k_relu_grad, plane_grid(g, n.batch), dim3(256), 0, n.stream, g, a);
   };
   View gout{n.gout, 2, 2, n.s, n.s};
   View gd0 = vw(n, B_D0, 0, c, true);
@@ -748,8 +781,8 @@ hipError_t nn_backward(NnNet& n) {
   View gcat0 = vw(n, B_CAT0, 0, 3 * c, true);
   if ((e = conv_bwd(n, 12, vw(n, B_CAT0, 0, 3 * c, false), ga6, &gcat0))) return e;
   View gd1 = vw(n, B_D1, 0, 2 * c, true);
-  hipLaunchKernelGGL(k_up_bwd, grid_of(view_count(gd1, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT0, 0, 2 * c, true), gd1, n.batch);
+  hipLaunchKernelGGL(k_up_bwd, plane_grid(gd1, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_CAT0, 0, 2 * c, true), gd1);
   // level 1 decoder
   relu_g(gd1, vw(n, B_D1, 0, 2 * c, false));
   View ga5 = vw(n, B_A5, 0, 2 * c, true);
@@ -758,8 +791,8 @@ hipError_t nn_backward(NnNet& n) {
   View gcat1 = vw(n, B_CAT1, 0, 6 * c, true);
   if ((e = conv_bwd(n, 10, vw(n, B_CAT1, 0, 6 * c, false), ga5, &gcat1))) return e;
   View gd2 = vw(n, B_D2, 0, 4 * c, true);
-  hipLaunchKernelGGL(k_up_bwd, grid_of(view_count(gd2, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT1, 0, 4 * c, true), gd2, n.batch);
+  hipLaunchKernelGGL(k_up_bwd, plane_grid(gd2, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_CAT1, 0, 4 * c, true), gd2);
   // level 2 decoder
   relu_g(gd2, vw(n, B_D2, 0, 4 * c, false));
   View ga4 = vw(n, B_A4, 0, 4 * c, true);
@@ -768,8 +801,8 @@ hipError_t nn_backward(NnNet& n) {
   View gcat2 = vw(n, B_CAT2, 0, 12 * c, true);
   if ((e = conv_bwd(n, 8, vw(n, B_CAT2, 0, 12 * c, false), ga4, &gcat2))) return e;
   View gbn = vw(n, B_BN, 0, 8 * c, true);
-  hipLaunchKernelGGL(k_up_bwd, grid_of(view_count(gbn, n.batch)), dim3(256), 0, n.stream,
-                     vw(n, B_CAT2, 0, 8 * c, true), gbn, n.batch);
+  hipLaunchKernelGGL(k_up_bwd, plane_grid(gbn, n.batch), dim3(256), 0, n.stream,
+                     vw(n, B_CAT2, 0, 8 * c, true), gbn);
   // bottleneck
   relu_g(gbn, vw(n, B_BN, 0, 8 * c, false));
   View ga3 = vw(n, B_A3, 0, 8 * c, true);
@@ -778,8 +811,7 @@ hipError_t nn_backward(NnNet& n) {
   View gp2 = vw(n, B_P2, 0, 4 * c, true);
   if ((e = conv_bwd(n, 6, vw(n, B_P2, 0, 4 * c, false), ga3, &gp2))) return e;
   View ge2 = vw(n, B_CAT2, 8 * c, 4 * c, true);
-  hipLaunchKernelGGL(k_pool_bwd, grid_of(view_count(gp2, n.batch)), dim3(256), 0, n.stream, gp2, ge2, n.arg[2],
-                     n.batch);
+  hipLaunchKernelGGL(k_pool_bwd, plane_grid(gp2, n.batch), dim3(256), 0, n.stream, gp2, ge2, n.arg[2]);
   // level 2 encoder
   relu_g(ge2, vw(n, B_CAT2, 8 * c, 4 * c, false));
   View ga2 = vw(n, B_A2, 0, 4 * c, true);
@@ -788,8 +820,7 @@ hipError_t nn_backward(NnNet& n) {
   View gp1 = vw(n, B_P1, 0, 2 * c, true);
   if ((e = conv_bwd(n, 4, vw(n, B_P1, 0, 2 * c, false), ga2, &gp1))) return e;
   View ge1 = vw(n, B_CAT1, 4 * c, 2 * c, true);
-  hipLaunchKernelGGL(k_pool_bwd, grid_of(view_count(gp1, n.batch)), dim3(256), 0, n.stream, gp1, ge1, n.arg[1],
-                     n.batch);
+  hipLaunchKernelGGL(k_pool_bwd, plane_grid(gp1, n.batch), dim3(256), 0, n.stream, gp1, ge1, n.arg[1]);
   // level 1 encoder
   relu_g(ge1, vw(n, B_CAT1, 4 * c, 2 * c, false));
   View ga1 = vw(n, B_A1, 0, 2 * c, true);
@@ -798,8 +829,7 @@ hipError_t nn_backward(NnNet& n) {
   View gp0 = vw(n, B_P0, 0, c, true);
   if ((e = conv_bwd(n, 2, vw(n, B_P0, 0, c, false), ga1, &gp0))) return e;
   View ge0 = vw(n, B_CAT0, 2 * c, c, true);
-  hipLaunchKernelGGL(k_pool_bwd, grid_of(view_count(gp0, n.batch)), dim3(256), 0, n.stream, gp0, ge0, n.arg[0],
-                     n.batch);
+  hipLaunchKernelGGL(k_pool_bwd, plane_grid(gp0, n.batch), dim3(256), 0, n.stream, gp0, ge0, n.arg[0]);
   // level 0 encoder
   relu_g(ge0, vw(n, B_CAT0, 2 * c, c, false));
   View ga0 = vw(n, B_A0, 0, c, true);
@@ -861,6 +891,7 @@ hipError_t nn_create(NnNet& n, int s, int c, int batch) {
   if (e == hipSuccess) e = nn_alloc1((void**)&n.g, n.nparam * sizeof(float));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.wb, n.nparam * sizeof(bf16));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.wbt, n.nparam * sizeof(bf16));
+  if (e == hipSuccess) e = nn_alloc1((void**)&n.bpart, (size_t)8 * c * BIAS_PARTS * sizeof(double));
   if (e == hipSuccess) e = nn_alloc1((void**)&n.lpart, 1024 * sizeof(double));
   // weight-gradient partial tiles: the largest chunks x Cout x Cin ks^2 over the layers
   static const int level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
@@ -889,7 +920,7 @@ void nn_free(NnNet& n) {
   for (auto& p : n.arg)
     if (p) hipFree(p);
   for (void* p : {(void*)n.out, (void*)n.gout, (void*)n.target, (void*)n.w, (void*)n.mo, (void*)n.ve, (void*)n.g,
-                  (void*)n.wb, (void*)n.wbt, (void*)n.lpart, (void*)n.partial, (void*)n.lw})
+                  (void*)n.wb, (void*)n.wbt, (void*)n.bpart, (void*)n.lpart, (void*)n.partial, (void*)n.lw})
     if (p) hipFree(p);
 }
 
