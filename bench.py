@@ -14,11 +14,14 @@ on from iteration 20).
 
 Prints one JSON line.
 
+`--workload epie_c0`: ePIE, 12 x 12 raster, 128 x 128 frames (positions/s; a dependent chain).
+`--workload batch_c3`: 16 DM scans of uneven size advanced in one captured HIP graph per
+iteration (and, beside it, one after another).
 `--workload c4`: the learned warm start -- trains the network on simulated frames, then runs
 MLs on the mls_c2 workload from a uniform and from the stitched start to a target normalised
 object error (minutes; not a throughput line).
 
-  python bench.py [--gpus 1] [--workload dm_c1|mls_c2|c4] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--gpus 1] [--workload dm_c1|mls_c2|epie_c0|batch_c3|c4] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -117,6 +120,142 @@ def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: in
                       f"whole-canvas steps included), 256x256, 5 modes; {dt:.1f} s on {cpu_model()}"}
 
 
+def _base_line(args, metric, value, unit, ms, dtype, data, config) -> dict:
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": data, "config": config}
+
+
+def run_epie(args) -> dict:
+    """ePIE at 12 x 12 / 128^2: positions are a dependent chain, so the figure is positions/s and
+    the per-position latency (no roofline fraction binds a latency-bound chain)."""
+    import torch
+    from ptychopy_amd import workloads
+    from ptychopy_amd import _engine as E
+    W = workloads.epie_c0()
+    n, m = W["n"], W["m"]
+    e = E.Engine(n, m, W["canvas"])
+    e.set_frames(W["amp"])
+    e.set_positions(W["pos"])
+    e.clear_object()
+    e.init_probe(W["beam"], W["dx_s"])
+    amp_deq = e.get_frames()
+    rng = np.random.default_rng(20261200)
+
+    def flags(i):
+        return (E.FLAG_PHASE_CONSTRAINT if i <= 1 else 0) | (E.FLAG_UPDATE_PROBE if i >= 10 else 0)
+
+    it = 0
+    for _ in range(args.warmup):
+        it += 1
+        e.epie_iteration(rng.permutation(n), flags(it))
+    e.synchronize()
+    torch.cuda.synchronize()
+    orders = [rng.permutation(n) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for o in orders:
+        it += 1
+        e.epie_iteration(o, flags(it))
+    e.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    out = _base_line(args, "positions/s per ePIE iteration", n / (ms * 1e-3), "positions/s", ms, "c64",
+                     "synthetic (seeded random-field object, Gaussian probe, Poisson 1e6 photons/frame)",
+                     {"workload": "ePIE 12x12 raster, 128x128 frames, 320x320 object, 1 probe mode, probe update on",
+                      "frames": n, "frame": m})
+    out["us_per_position"] = ms * 1e3 / n
+    out["roofline"] = None
+    if not args.no_cpu:
+        import oracle
+        sc = e.get_scalars()
+        st = oracle.EpieState(e.get_object(), e.get_probe(), sc[E.SCAL_OBJ_MAX], sc[E.SCAL_PROBE_MAX])
+        a64 = amp_deq.astype(np.float64)
+        iters, t1 = 0, time.perf_counter()
+        while True:
+            oracle.epie_iteration(st, a64, W["pos"], rng.permutation(n), update_probe=True)
+            iters += 1
+            dt = time.perf_counter() - t1
+            if dt >= 10.0 and iters >= 2:
+                break
+        out["cpu_baseline"] = {"value": n * iters / dt, "unit": "positions/s", "cores": oracle.num_threads(),
+                               "kind": "port", "sample": f"{iters} FP64 oracle ePIE iterations over all {n} positions; "
+                                                         f"{dt:.1f} s on {cpu_model()}"}
+    e.close()
+    return out
+
+
+def run_batch(args) -> dict:
+    """16 independent DM scans of uneven size advanced together: one captured HIP graph per flag
+    set, replayed per iteration; the same scans run one after another beside it."""
+    import torch
+    from ptychopy_amd import workloads, autotune
+    from ptychopy_amd import _engine as E
+    scans = workloads.batch_c3()
+    tot = sum(s["n"] for s in scans)
+    waves = [autotune.tune_dm_wave(s["pos"], s["m"], s["canvas"], iters=2)[0] for s in scans]
+
+    def make():
+        es = []
+        for s, w in zip(scans, waves):
+            e = E.Engine(s["n"], s["m"], s["canvas"])
+            e.set_frames(s["amp"])
+            e.set_positions(s["pos"])
+            e.clear_object()
+            e.init_probe(s["m"] * 24.0 / 128.0 * 1e-8, 1e-8)
+            e.set_tuning(E.TUNE_DM_WAVE, w)
+            es.append(e)
+        return es
+
+    def flags(i):
+        return (E.FLAG_PHASE_CONSTRAINT if i <= 1 else 0) | (E.FLAG_UPDATE_PROBE if i >= 10 else 0)
+
+    es = make()
+    with E.Batch(es) as b:
+        it = 0
+        for _ in range(max(args.warmup, 10)):
+            it += 1
+            b.dm_iteration(flags(it))
+        b.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            it += 1
+            b.dm_iteration(flags(it))
+        b.synchronize()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        graphs = b.graph_count
+    for e in es:
+        e.close()
+    es = make()
+    it = 0
+    for _ in range(max(args.warmup, 10)):
+        it += 1
+        for e in es:
+            e.dm_iteration(flags(it))
+    for e in es:
+        e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it += 1
+        for e in es:
+            e.dm_iteration(flags(it))
+    for e in es:
+        e.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3 / args.steps
+    for e in es:
+        e.close()
+    out = _base_line(args, "diffraction frames/s per iteration", tot / (ms * 1e-3), "frames/s", ms, "c64",
+                     "synthetic (16 seeded scans: random-field objects, Gaussian probes, Poisson 1e5-1e7 photons)",
+                     {"workload": "batched DM, 16 scans of 20x20..51x51 positions, frames 128/256/512, "
+                                  "one captured HIP graph per iteration", "frames": tot,
+                      "scans": [[s["n"], s["m"]] for s in scans], "frames_per_wave": waves})
+    out["sequential_ms_per_step"] = ms_seq
+    out["graphs"] = graphs
+    out["roofline"] = None
+    return out
+
+
 def run_mls(args) -> dict:
     import torch
     from ptychopy_amd import workloads, _driver
@@ -194,7 +333,7 @@ def run_mls(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("dm_c1", "mls_c2", "c4"), default="dm_c1")
+    ap.add_argument("--workload", choices=("dm_c1", "mls_c2", "epie_c0", "batch_c3", "c4"), default="dm_c1")
     ap.add_argument("--c4-train", type=int, default=50000, help="c4: training frames")
     ap.add_argument("--c4-held", type=int, default=5000, help="c4: held-out frames")
     ap.add_argument("--c4-epochs", type=int, default=2, help="c4: training epochs")
@@ -247,6 +386,11 @@ def main() -> int:
                             "frac": r["train_tflops"] / 2500.0, "traffic": None},
                "c4": r, "wall_s": time.perf_counter() - t0}
         print(json.dumps(out))
+        return 0
+    if args.workload in ("epie_c0", "batch_c3"):
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 10 if args.warmup is None else args.warmup
+        print(json.dumps(run_epie(args) if args.workload == "epie_c0" else run_batch(args)))
         return 0
     if args.workload == "mls_c2":
         args.steps = 10 if args.steps is None else args.steps
