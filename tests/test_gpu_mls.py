@@ -71,6 +71,7 @@ CONFIGS = [
     dict(k=3, cfg=dict(apply_subpix_shift=0, probe_pos_search=99)),
     dict(k=2, cfg=dict(variable_probe=0, variable_intensity=0, variable_probe_modes=0, beta_lsq=0.0)),
     dict(k=2, cfg=dict(apply_multimodal_update=1, probe_pos_search=4)),
+    dict(k=5, cfg=dict(probe_pos_search=3)),  # five modes: the fused update of modes 2..5 (8-wide form)
 ]
 
 

@@ -2,8 +2,13 @@
 
 The GPU rounds weights, inputs, targets and stored activations to BF16 and accumulates in FP32;
 the oracle rounds the same operands and accumulates in FP64, so the two differ only by
-accumulation order and the occasional BF16 rounding flip that order causes.  Tolerances:
-output and loss relative L2 <= 5e-3; per-layer weight-gradient relative L2 <= 2e-2.
+accumulation order and the BF16 rounding flips that order causes.  Those flips propagate through
+the 15 layers: on the oracle alone, perturbing the weights by 1e-6 (relative) moves the emulated
+output by 4e-3 to 9e-3 and single layers' gradients by up to 6e-2 (relative L2) at these sizes.
+So each gradient test measures that noise floor for its own inputs (one oracle run with weights
+perturbed by 1e-6) and allows 3x it: output <= max(1.5e-2, 3 x floor), loss <= 1e-2 relative,
+each layer's weight and bias gradient <= max(2e-2, 3 x that layer's floor).  A wrong index or a
+missing term gives O(1).
 """
 import numpy as np
 import pytest
@@ -47,7 +52,7 @@ def sample(rng, batch, s):
 
 
 @pytest.mark.parametrize("s,c,batch,weighted", [(16, 4, 2, False), (32, 8, 3, False), (64, 16, 2, False),
-                                                (32, 8, 2, True), (256, 8, 2, True)])
+                                                (32, 8, 2, True), (256, 8, 2, True), (32, 32, 2, True)])
 def test_gradient_matches_oracle(s, c, batch, weighted):
     print(f"net gradient s={s} c={c} batch={batch} weighted={weighted}", flush=True)
     rng = np.random.default_rng(7 + s)
@@ -68,11 +73,17 @@ def test_gradient_matches_oracle(s, c, batch, weighted):
         loss, g = net.gradient(x, t)
         out = net.predict(x)
     ref_out, ref_loss, ref_g = oracle.nn_forward_backward(p, x, t, c, emulate_bf16=True, weight=w)
-    assert rel(out, ref_out) <= 5e-3
-    assert abs(loss - ref_loss) <= 5e-3 * ref_loss
-    for li, (w, b) in enumerate(sl):
-        assert rel(g[w], ref_g[w]) <= 2e-2, (li, rel(g[w], ref_g[w]))
-        assert rel(g[b], ref_g[b]) <= 2e-2, (li, rel(g[b], ref_g[b]))
+    # the emulation's own sensitivity to 1e-6 relative changes (accumulation-order scale)
+    pp = (p * (1 + 1e-6 * np.random.default_rng(99).standard_normal(p.size))).astype(np.float32)
+    n_out, _, n_g = oracle.nn_forward_backward(pp, x, t, c, emulate_bf16=True, weight=w)
+    floor_out = rel(n_out, ref_out)
+    print(f"output: GPU {rel(out, ref_out):.2e}, noise floor {floor_out:.2e}")
+    assert rel(out, ref_out) <= max(1.5e-2, 3 * floor_out)
+    assert abs(loss - ref_loss) <= 1e-2 * ref_loss
+    for li, (wl, bl) in enumerate(sl):  # each layer against its own noise floor
+        for part in (wl, bl):
+            err, floor = rel(g[part], ref_g[part]), rel(n_g[part], ref_g[part])
+            assert err <= max(2e-2, 3 * floor), (li, err, floor)
     # the unrounded FP64 network is close too (BF16 is the only approximation)
     f_out, f_loss, _ = oracle.nn_forward_backward(p, x, t, c, emulate_bf16=False)
     assert rel(out, f_out) <= 5e-2
@@ -89,7 +100,7 @@ def test_predict_handles_a_ragged_last_batch():
     np.testing.assert_array_equal(y6[4:], y2)  # samples do not interact inside a batch
     ref, _, _ = oracle.nn_forward_backward(p, x, np.zeros((6, 2, s, s), np.float32), c, emulate_bf16=True,
                                            grad=False)
-    assert rel(y6, ref) <= 5e-3
+    assert rel(y6, ref) <= 1.5e-2
 
 
 def test_first_adam_step_is_a_signed_learning_rate_step():
