@@ -84,11 +84,6 @@ __global__ void k_scale_c(float2* __restrict__ x, long count, const double* __re
   x[i] = cscale(x[i], f);
 }
 
-__global__ void k_copy_c(float2* __restrict__ dst, const float2* __restrict__ src, long count) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) dst[i] = src[i];
-}
-
 __global__ void k_copy2(float2* __restrict__ dst, const float2* __restrict__ a, const float2* __restrict__ b,
                         long count) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

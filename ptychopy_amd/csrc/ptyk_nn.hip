@@ -705,7 +705,6 @@ static hipError_t conv_bwd(NnNet& n, int li, View x, View dz, View* dx) {
   return hipGetLastError();
 }
 
-static long view_count(const View& v, int batch) { return (long)batch * v.c * v.h * v.w; }
 
 hipError_t nn_forward(NnNet& n) {
   const int c = n.c;
