@@ -5,6 +5,7 @@
 // (Diffractions.cpp:105-107) and the FFT entry point used by the tests.
 // Pass 1: each workgroup transforms a batch of whole rows (contiguous) in LDS.
 // Pass 2: each workgroup transforms strips of 16 columns (128 B per row segment).
+// Frames must start 16-byte aligned (every engine buffer does).
 #include "ptyk_device.h"
 #include "ptyk_internal.h"
 
@@ -16,22 +17,35 @@ struct FftCfg {
   static constexpr int CNB = (plan_min_batch<M, PTYK_NT>() + 15) / 16 * 16;
 };
 
+// Both passes move two adjacent complex values (16 B) per lane in fully unrolled loops, so every
+// load of a thread is in flight before its first LDS store.
 template <int M, bool INV>
 __global__ __launch_bounds__(PTYK_NT) void k_fft_rows(float2* __restrict__ data, long total_rows,
                                                       const float2* __restrict__ tw) {
   constexpr int NB = FftCfg<M>::RNB;
+  constexpr int PAIRS = NB * M / 2 / PTYK_NT;
+  static_assert((NB * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
   __shared__ float2 lds[NB * M];
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * NB;
-  for (int e = tid; e < NB * M; e += PTYK_NT) {
-    const long g = row0 + e / M;
-    lds[e] = g < total_rows ? data[g * M + (e % M)] : make_float2(0.f, 0.f);
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      const long g = row0 + e / M;
+      v[i] = g < total_rows ? *reinterpret_cast<const float4*>(data + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
   }
   __syncthreads();
   fft_lds<M, INV, NB, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
-  for (int e = tid; e < NB * M; e += PTYK_NT) {
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
     const long g = row0 + e / M;
-    if (g < total_rows) data[g * M + (e % M)] = lds[e];
+    if (g < total_rows) *reinterpret_cast<float4*>(data + g * M + (e % M)) = *reinterpret_cast<const float4*>(&lds[e]);
   }
 }
 
@@ -41,28 +55,38 @@ __global__ __launch_bounds__(PTYK_NT) void k_fft_cols(float2* __restrict__ data,
   constexpr int NB = FftCfg<M>::CNB;
   constexpr int G = NB / 16;
   constexpr int SPF = M / 16;
+  constexpr int PAIRS = NB * M / 2 / PTYK_NT;
+  static_assert((NB * M / 2) % PTYK_NT == 0, "column tile must divide among the threads");
   __shared__ float2 lds[NB * M];
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * M; e += PTYK_NT) {
-    const int g = e / (M * 16), i = (e >> 4) % M, c = e & 15;
-    const long s = (long)blockIdx.x * G + g;
-    float2 v = make_float2(0.f, 0.f);
-    if (s < total_strips) {
-      const long f = s / SPF;
-      const int col = (int)(s % SPF) * 16 + c;
-      v = data[(f * M + i) * M + col];
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const int e = 2 * (tid + k * PTYK_NT);
+      const int g = e / (M * 16), i = (e >> 4) % M, c = e & 15;
+      const long s = (long)blockIdx.x * G + g;
+      v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (s < total_strips) {
+        const long f = s / SPF;
+        const int col = (int)(s % SPF) * 16 + c;
+        v[k] = *reinterpret_cast<const float4*>(data + (f * M + i) * M + col);
+      }
     }
-    lds[e] = v;
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) *reinterpret_cast<float4*>(&lds[2 * (tid + k * PTYK_NT)]) = v[k];
   }
   __syncthreads();
   fft_lds<M, INV, NB, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
-  for (int e = tid; e < NB * M; e += PTYK_NT) {
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * PTYK_NT);
     const int g = e / (M * 16), i = (e >> 4) % M, c = e & 15;
     const long s = (long)blockIdx.x * G + g;
     if (s < total_strips) {
       const long f = s / SPF;
       const int col = (int)(s % SPF) * 16 + c;
-      data[(f * M + i) * M + col] = lds[e];
+      *reinterpret_cast<float4*>(data + (f * M + i) * M + col) = *reinterpret_cast<const float4*>(&lds[e]);
     }
   }
 }
