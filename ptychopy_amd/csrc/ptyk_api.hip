@@ -107,8 +107,8 @@ static hipError_t dalloc(T** p, size_t count) {
 static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
-                  d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal,
-                  d.gframes, d.ftot};
+                  d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
+                  d.ftot};  // d.tmp and d.tmp2 point into d.psi
   for (void* p : ptrs)
     if (p) hipFree(p);
   d = MlsDev();
@@ -771,19 +771,22 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const int ch = mls_chunks(d.m);
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
   if (g > d.gmax) {
-    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.tmp, d.tmp2, d.gscal, d.ftot};
+    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.gscal, d.ftot};  // tmp and tmp2 live inside psi
     for (void* p : ptrs)
       if (p) hipFree(p);
     d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = nullptr;
     d.gscal = d.ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
+    const long tcap = std::max<long>(g, d.k);
     HIPCHK(c, dalloc(&d.op, gm));
     HIPCHK(c, dalloc(&d.vp, gm));
-    HIPCHK(c, dalloc(&d.psi, gm * d.k));
+    // exit waves (K x g frames), then tmp (max(g, K) frames), then tmp2 (g frames): the position
+    // step's transforms of tmp and tmp2 share the exit waves' FFT launches
+    HIPCHK(c, dalloc(&d.psi, gm * d.k + (tcap + g) * mm));
+    d.tmp = d.psi + gm * d.k;
+    d.tmp2 = d.tmp + tcap * mm;
     HIPCHK(c, dalloc(&d.pu, gm));
-    HIPCHK(c, dalloc(&d.tmp, std::max<long>(g, d.k) * mm));
-    HIPCHK(c, dalloc(&d.tmp2, gm));
     HIPCHK(c, dalloc(&d.gscal, (size_t)MLS_S_COEF + g));
     HIPCHK(c, dalloc(&d.ftot, (size_t)8 * g));
     d.gmax = g;

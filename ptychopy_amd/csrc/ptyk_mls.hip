@@ -1005,7 +1005,9 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K, m,
            d.cols, subpix);
     CHK();
-    TRY(fft2_batch(d.psi, (long)K * G, m, false, d.tw, s));
+    // with the position step, tmp (the object patch, right after the exit-wave buffer) rides in
+    // the same forward transform (frames between K G and K gmax are unused scratch)
+    TRY(fft2_batch(d.psi, pos_step ? (long)K * d.gmax + G : (long)K * G, m, false, d.tw, s));
     LAUNCH(k_modulus, gch, dim3(NT), d.psi, d.amp, d.amp_scale, gf, G, K, m, d.part);
     CHK();
     TRY(fft2_batch(d.psi, (long)K * G, m, true, d.tw, s));
@@ -1066,11 +1068,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         LAUNCH(k_axpy_beta, g1(mm), dim3(NT), d.probe + (long)(ll - 1) * mm, d.pum, (long)mm, d.gscal + MLS_S_BP);
       CHK();
       if (pos_step && ll == 1) {
-        TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
-        LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
+        LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);  // tmp already holds its spectrum
         CHK();
-        TRY(fft2_batch(d.tmp, G, m, true, d.tw, s));
-        TRY(fft2_batch(d.tmp2, G, m, true, d.tw, s));
+        // tmp and tmp2 in one inverse transform (tmp2 follows tmp's max(gmax, K) frames)
+        TRY(fft2_batch(d.tmp, std::max<long>(d.gmax, K) + G, m, true, d.tw, s));
         LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
         LAUNCH(k_pos_update, g1(G), dim3(NT), d.part, ch, gf, G, d.pos);
         CHK();
