@@ -5,10 +5,11 @@
 // d_updateProbe(M) (ProbeKernels.cu:425-458), the thrust maxima behind
 // Sample/Probe::updateMaxIntensity and d_calculateER (utilitiesKernels.cu:684-712)).
 //
-// ePIE is a chain of dependent position updates, so one persistent workgroup (16 waves on
-// one CU) walks every position of an iteration in the given order; the exit waves live in
-// an L2-resident scratch and each 2D transform is staged through LDS in row and column
-// chunks. There is one launch per iteration and no host round trip per position.
+// ePIE is a chain of dependent position updates, so one persistent workgroup walks every
+// position of an iteration in the given order: one launch per iteration and no host round
+// trip per position. Single-mode frames of 64^2 and 128^2 keep the whole exit wave in LDS
+// (k_epie_lds); other sizes and the modal form keep the exit waves in an L2-resident scratch
+// and stage each 2D transform through LDS in row and column chunks (k_epie).
 #include "ptyk_device.h"
 #include "ptyk_internal.h"
 
@@ -60,26 +61,44 @@ __device__ void wg_fft2(float2* __restrict__ g, float2* lds, const float2* __res
 // Bilinear sample with clamp addressing, matching a linear-filtered, unnormalised texture
 // fetch at (u, v) = (col + offY + 0.5, row + offX + 0.5) (SampleKernels.cu:167-178, 493-508).
 // quantize: round the weights to 1/256 (8 fractional bits, as texture units store them).
-__device__ __forceinline__ float2 bilinear(const float2* __restrict__ o, int rows, int cols, float u, float v,
-                                           int quantize) {
+// The sample in steps (tap position and weights; clamped tap indices; the four gathers; the
+// weighted sum), so that callers can issue the gathers of several samples before combining any,
+// or take the taps from a staged copy of the object.
+struct TapPos {
+  int i0, j0;  // column and row of the first tap, before clamping
+  float al, be;
+};
+template <bool QUANT>
+__device__ __forceinline__ TapPos tap_pos(float u, float v) {
   const float xb = u - 0.5f, yb = v - 0.5f;
   const float fx = floorf(xb), fy = floorf(yb);
-  float al = xb - fx, be = yb - fy;
-  if (quantize) {
-    al = rintf(al * 256.f) * (1.f / 256.f);
-    be = rintf(be * 256.f) * (1.f / 256.f);
+  TapPos p;
+  p.al = xb - fx;
+  p.be = yb - fy;
+  if (QUANT) {
+    p.al = rintf(p.al * 256.f) * (1.f / 256.f);
+    p.be = rintf(p.be * 256.f) * (1.f / 256.f);
   }
-  int i0 = (int)fx, j0 = (int)fy;
-  int i1 = i0 + 1, j1 = j0 + 1;
-  i0 = min(max(i0, 0), cols - 1);
-  i1 = min(max(i1, 0), cols - 1);
-  j0 = min(max(j0, 0), rows - 1);
-  j1 = min(max(j1, 0), rows - 1);
-  const float2 t00 = o[(long)j0 * cols + i0], t01 = o[(long)j0 * cols + i1];
-  const float2 t10 = o[(long)j1 * cols + i0], t11 = o[(long)j1 * cols + i1];
+  p.i0 = (int)fx;
+  p.j0 = (int)fy;
+  return p;
+}
+__device__ __forceinline__ float2 bilinear_sum(float al, float be, float2 t00, float2 t01, float2 t10, float2 t11) {
   const float w00 = (1.f - al) * (1.f - be), w01 = al * (1.f - be), w10 = (1.f - al) * be, w11 = al * be;
   return make_float2(w00 * t00.x + w01 * t01.x + w10 * t10.x + w11 * t11.x,
                      w00 * t00.y + w01 * t01.y + w10 * t10.y + w11 * t11.y);
+}
+template <bool QUANT>
+__device__ __forceinline__ float2 bilinear_t(const float2* __restrict__ o, int rows, int cols, float u, float v) {
+  const TapPos p = tap_pos<QUANT>(u, v);
+  const int i0 = min(max(p.i0, 0), cols - 1), i1 = min(max(p.i0 + 1, 0), cols - 1);
+  const int j0 = min(max(p.j0, 0), rows - 1), j1 = min(max(p.j0 + 1, 0), rows - 1);
+  return bilinear_sum(p.al, p.be, o[(long)j0 * cols + i0], o[(long)j0 * cols + i1], o[(long)j1 * cols + i0],
+                      o[(long)j1 * cols + i1]);
+}
+__device__ __forceinline__ float2 bilinear(const float2* __restrict__ o, int rows, int cols, float u, float v,
+                                           int quantize) {
+  return quantize ? bilinear_t<true>(o, rows, cols, u, v) : bilinear_t<false>(o, rows, cols, u, v);
 }
 
 template <int M>
@@ -206,6 +225,323 @@ __global__ __launch_bounds__(PTYK_EPIE_NT) void k_epie(EpieArgs a) {
   }
 }
 
+// threadIdx.x behind an empty asm: per-lane addresses derived from it are recomputed in each
+// phase instead of being kept live (and spilled) across the transforms; the asm also bounds a
+// scheduling region, so each phase calls it once.
+__device__ __forceinline__ int lane() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// A f / |f| * nrm: the projection of modulus_phase without its atan2 / sincos (which would make
+// the modulus step the longest of a position on one CU), branch-free: f is first scaled by an
+// exact power of two so that |f|^2 cannot overflow or underflow; f = 0 gives A nrm, as there.
+__device__ __forceinline__ float2 modulus_unit(float2 f, float a, float nrm) {
+  const float s = fmaxf(fabsf(f.x), fabsf(f.y));
+  int e;
+  (void)frexpf(s, &e);
+  const float xs = ldexpf(f.x, -e), ys = ldexpf(f.y, -e);
+  const float k = a * nrm * rsqrtf(xs * xs + ys * ys);
+  return s > 0.f ? make_float2(xs * k, ys * k) : make_float2(a * nrm, 0.f);
+}
+
+// max of the |O|^2 cache over the canvas (non-negative values) and a flat index where it is
+// attained, 16 loads of 16 bytes in flight per thread
+struct MaxAt {
+  float v;
+  long at;
+};
+template <int NT>
+__device__ MaxAt canvas_max(const float* __restrict__ oi, long canvas, float* redf, long* redi) {
+  constexpr int U = 16;
+  const float4* oi4 = reinterpret_cast<const float4*>(oi);
+  const long c4 = canvas >> 2;
+  const int t = lane();
+  float mx = 0.f;
+  long ix = 0;
+  for (long q0 = 0; q0 < c4; q0 += (long)U * NT) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long q = q0 + t + (long)u * NT;
+      v[u] = q < c4 ? oi4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long q = 4 * (q0 + t + (long)u * NT);
+      if (v[u].x > mx) { mx = v[u].x; ix = q; }
+      if (v[u].y > mx) { mx = v[u].y; ix = q + 1; }
+      if (v[u].z > mx) { mx = v[u].z; ix = q + 2; }
+      if (v[u].w > mx) { mx = v[u].w; ix = q + 3; }
+    }
+  }
+  for (long q = (c4 << 2) + t; q < canvas; q += NT)
+    if (oi[q] > mx) { mx = oi[q]; ix = q; }
+  const float m = block_max<NT>(mx, redf);
+  // a location of the maximum: the lowest index among the threads that hold it
+  long cand = mx == m ? ix : canvas;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cand = min(cand, (long)__shfl_xor(cand, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = cand;
+  __syncthreads();
+  long best = redi[0];
+#pragma unroll
+  for (int i = 1; i < NT / 64; ++i) best = min(best, redi[i]);
+  __syncthreads();
+  return MaxAt{m, best};
+}
+
+// The object window [ib, ib + M + 2) x [jb, jb + M + 2) (indices clamped to the canvas) behind
+// a position's bilinear taps, staged in LDS with row pitch M + 2: every tap of the frame is an
+// element of it (the first tap of pixel (ii, jj) is at row ii or ii + 1 and column jj or jj + 1
+// of the window when ib = floor(pos.x), jb = floor(pos.y)).
+template <int M, int NT>
+__device__ __forceinline__ void load_window(const EpieArgs& a, int ib, int jb, float2* win) {
+  constexpr int WP = M + 2, WN = WP * WP, U = (WN + NT - 1) / NT;
+  const int t = lane();
+  float2 v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = t + u * NT;
+    const int r = min(max(ib + e / WP, 0), a.obj_rows - 1), c = min(max(jb + e % WP, 0), a.obj_cols - 1);
+    v[u] = e < WN ? a.obj[(long)r * a.obj_cols + c] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = t + u * NT;
+    if (e < WN) win[e] = v[u];
+  }
+}
+
+// psi_old = ROI * P (bilinear ROI at the fractional position) from the staged window, into
+// registers (the caller writes it into the frame after a barrier: frame and window share LDS).
+template <int M, int NT, int PPT, bool QUANT>
+__device__ __forceinline__ void epie_roi(const EpieArgs& a, float2 pos, int ib, int jb, const float2* win,
+                                         float2 (&po)[PPT]) {
+  constexpr int WP = M + 2;
+  const int t = lane();
+  float2 pv[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) pv[j] = a.probe[t + j * NT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int px = t + j * NT;
+    const TapPos q = tap_pos<QUANT>((float)(px % M) + pos.y + 0.5f, (float)(px / M) + pos.x + 0.5f);
+    const int r = min(max(q.j0 - ib, 0), M), c = min(max(q.i0 - jb, 0), M);
+    const float2* w = win + r * WP + c;
+    po[j] = cmul(bilinear_sum(q.al, q.be, w[0], w[1], w[WP], w[WP + 1]), pv[j]);
+  }
+}
+
+// Resident-frame layout: row pitch M + 1 (conflict-free rows and columns in LDS).
+template <int M>
+__device__ __forceinline__ int at(int px) {
+  return (px / M) * (M + 1) + (px % M);
+}
+
+// Object update at the rounded position (d_updateObjectArray): all loads of G pixels before
+// their stores (the stores could otherwise alias the next loads). Leaves conj(O_new) (psi -
+// psi_old), the probe update's increment before its scale, in place of psi in LDS, and returns
+// this thread's max of the new |O|^2 with its flat canvas index.
+template <int M, int NT, int PPT, bool PC>
+__device__ __forceinline__ MaxAt epie_obj_update(const EpieArgs& a, int rx, int ry, float co, float2* lds) {
+  constexpr int G = 4;
+  MaxAt m{0.f, 0};
+#pragma unroll
+  for (int j0 = 0; j0 < PPT; j0 += G) {
+    const int t = lane();
+    float2 ov[G], pv[G], sv[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int px = t + (j0 + u) * NT;
+      ov[u] = a.obj[(long)(rx + px / M) * a.obj_cols + (ry + px % M)];
+      pv[u] = a.probe[px];
+      sv[u] = a.psi_old[px];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int px = t + (j0 + u) * NT;
+      const long oi = (long)(rx + px / M) * a.obj_cols + (ry + px % M);
+      const float2 d = csub(lds[at<M>(px)], sv[u]);
+      float2 o = cadd(ov[u], cscale(cmulcj(pv[u], d), co));
+      const float mag = sqrtf(o.x * o.x + o.y * o.y);
+      if (PC) o = clamp_phase(o);
+      a.obj[oi] = o;
+      a.obj_int[oi] = mag * mag;
+      if (mag * mag > m.v) m = MaxAt{mag * mag, oi};
+      lds[at<M>(px)] = cmulcj(o, d);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return m;
+}
+
+// Probe update with the updated object (d_updateProbe), from the increment the object update
+// left in LDS; returns this thread's max |P|^2.
+template <int M, int NT, int PPT>
+__device__ __forceinline__ float epie_probe_update(const EpieArgs& a, float cp, const float2* lds) {
+  const int t = lane();
+  float2 pv[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) pv[j] = a.probe[t + j * NT];
+  float mx = 0.f;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int px = t + j * NT;
+    const float2 p = cadd(cscale(lds[at<M>(px)], cp), pv[j]);
+    a.probe[px] = p;
+    const float ia = sqrtf(p.x * p.x + p.y * p.y);
+    a.probe_int[px] = ia * ia;
+    mx = fmaxf(mx, ia * ia);
+  }
+  return mx;
+}
+
+// Single-mode ePIE with the whole M x M exit wave resident in LDS (M (M + 1) complex values;
+// the object window of the bilinear ROI, (M + 2)^2, shares the space, 132 KiB at M = 128): both
+// 2D transforms and the modulus constraint run without a global round trip. Each thread owns
+// the pixels px = tid + j NT of the frame in every phase (so it reads back only its own part of
+// the psi_old scratch). Same update equations and order as k_epie. One CU moves every byte of a
+// position, so the phases are arranged to move few: the ROI gathers come from the staged
+// window, the probe update takes its increment from LDS, and the canvas max after an object
+// update is rescanned only when the previous maximum lay inside the updated ROI (otherwise it is
+// the larger of the previous maximum and the ROI's new maximum, exactly).
+template <int M, int NT>
+__global__ __launch_bounds__(NT) void k_epie_lds(EpieArgs a) {
+  constexpr int P = M + 1;
+  constexpr int FRAME = (M + 2) * (M + 2);  // >= M * P
+  extern __shared__ float2 lds[];           // frame (pitch P) or window (pitch M + 2), then twiddles
+  float2* tws = lds + FRAME;
+  __shared__ float redf[NT / 64];
+  __shared__ double redd[NT / 64];
+  __shared__ long redi[NT / 64];
+  constexpr int MM = M * M;
+  constexpr int PPT = MM / NT;
+  static_assert(MM % NT == 0, "pixels must divide among the threads");
+  const int tid = threadIdx.x;
+  float obj_max = (float)a.scal[0];
+  float probe_max = (float)a.scal[1];
+  double err = 0.0;
+  const long canvas = (long)a.obj_rows * a.obj_cols;
+  const float nrm = 1.f / (float)MM;
+  for (int i = tid; i < PTYK_TWN; i += NT) tws[i] = a.tw[i];
+  // where the current obj_max is attained (valid while the probe is updated)
+  long max_at = 0;
+  if (a.update_probe) {
+    const MaxAt m = canvas_max<NT>(a.obj_int, canvas, redf, redi);
+    obj_max = m.v;
+    max_at = m.at;
+  }
+
+  for (int s = 0; s < a.n_pos; ++s) {
+    const int fi = a.order[s];
+    const float2 pos = a.pos[fi];
+    const float ascale = a.amp_scale[fi];
+    const __half* amp = a.amp + (long)fi * MM;
+
+    const int ib = (int)floorf(pos.x), jb = (int)floorf(pos.y);
+    load_window<M, NT>(a, ib, jb, lds);
+    __syncthreads();
+    {
+      float2 po[PPT];
+      if (a.tex_quantize)
+        epie_roi<M, NT, PPT, true>(a, pos, ib, jb, lds, po);
+      else
+        epie_roi<M, NT, PPT, false>(a, pos, ib, jb, lds, po);
+      __syncthreads();
+      const int t = lane();
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        lds[at<M>(t + j * NT)] = po[j];
+        a.psi_old[t + j * NT] = po[j];
+      }
+    }
+    __syncthreads();
+    fft_lds<M, false, M, NT, RowLayout<M, P>, true>(lds, tws, lane());
+    fft_lds<M, false, M, NT, ColLayout<P>, true>(lds, tws, lane());
+    {
+      const int t = lane();
+      __half ah[PPT];
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) ah[j] = amp[t + j * NT];
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        const int q = at<M>(t + j * NT);
+        lds[q] = modulus_unit(lds[q], __half2float(ah[j]) * ascale, nrm);
+      }
+    }
+    __syncthreads();
+    fft_lds<M, true, M, NT, RowLayout<M, P>, true>(lds, tws, lane());
+    fft_lds<M, true, M, NT, ColLayout<P>, true>(lds, tws, lane());
+
+    const int rx = (int)roundf(pos.x), ry = (int)roundf(pos.y);
+    const float co = (float)(1.0 / (double)probe_max);
+    const MaxAt mo = a.phase_constraint ? epie_obj_update<M, NT, PPT, true>(a, rx, ry, co, lds)
+                                        : epie_obj_update<M, NT, PPT, false>(a, rx, ry, co, lds);
+    __syncthreads();
+
+    // the probe update needs max |O|^2 after this position, which otherwise is only reported
+    // after the last one
+    if (a.update_probe) {
+      const long mr = max_at / a.obj_cols, mc = max_at % a.obj_cols;
+      if (mr >= rx && mr < rx + M && mc >= ry && mc < ry + M) {
+        const MaxAt m = canvas_max<NT>(a.obj_int, canvas, redf, redi);
+        obj_max = m.v;
+        max_at = m.at;
+      } else {
+        const float m = block_max<NT>(mo.v, redf);
+        if (m > obj_max) {
+          // a location of the new maximum: the lowest index among the threads that hold it
+          long cand = mo.v == m ? mo.at : canvas;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) cand = min(cand, (long)__shfl_xor(cand, o, 64));
+          if ((tid & 63) == 0) redi[tid >> 6] = cand;
+          __syncthreads();
+          long best = redi[0];
+#pragma unroll
+          for (int i = 1; i < NT / 64; ++i) best = min(best, redi[i]);
+          __syncthreads();
+          obj_max = m;
+          max_at = best;
+        }
+      }
+      const float cp = (float)(1.0 / (double)obj_max);
+      float mx = epie_probe_update<M, NT, PPT>(a, cp, lds);
+      // the other (unused) modes keep their intensities
+      for (long q = MM + lane(); q < (long)a.k_total * MM; q += NT) mx = fmaxf(mx, a.probe_int[q]);
+      probe_max = block_max<NT>(mx, redf);
+    }
+
+    // Optional Fourier error (PhaserUtil::calculateER; ePIE.cpp:124-125) of psi_old
+    if (a.rms) {
+      __syncthreads();
+      const int t = lane();
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) lds[at<M>(t + j * NT)] = a.psi_old[t + j * NT];
+      __syncthreads();
+      fft_lds<M, false, M, NT, RowLayout<M, P>, true>(lds, tws, lane());
+      fft_lds<M, false, M, NT, ColLayout<P>, true>(lds, tws, lane());
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        const float2 sm = lds[at<M>(t + j * NT)];
+        acc += fabs((double)load_amp(amp, t + j * NT, ascale) - sqrt((double)sm.x * sm.x + (double)sm.y * sm.y));
+      }
+      acc = block_sum<NT>(acc, redd);
+      err += acc / (double)a.amp_sum[fi];
+    }
+    __syncthreads();
+  }
+  if (!a.update_probe && a.n_pos > 0) obj_max = canvas_max<NT>(a.obj_int, canvas, redf, redi).v;
+  if (tid == 0) {
+    a.scal[0] = obj_max;
+    a.scal[1] = probe_max;
+    a.scal[2] = err;
+  }
+}
+
 template <int M>
 static hipError_t epie_impl(const EpieArgs& a, hipStream_t s) {
   const size_t lds = (size_t)WgFft<M, PTYK_EPIE_NT>::LDS * sizeof(float2);
@@ -215,7 +551,20 @@ static hipError_t epie_impl(const EpieArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Frames whose exit wave fits in LDS take the resident form (single mode only).
+template <int M, int NT>
+static hipError_t epie_lds_impl(const EpieArgs& a, hipStream_t s) {
+  const size_t lds = (size_t)((M + 2) * (M + 2) + PTYK_TWN) * sizeof(float2);
+  hipError_t e = hipFuncSetAttribute((const void*)k_epie_lds<M, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_epie_lds<M, NT>), dim3(1), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
 hipError_t epie_iteration(const EpieArgs& a, hipStream_t s) {
+  if (a.k_use == 1 && a.m == 128) return epie_lds_impl<128, 1024>(a, s);
+  if (a.k_use == 1 && a.m == 64) return epie_lds_impl<64, 512>(a, s);
   switch (a.m) {
     case 16: return epie_impl<16>(a, s);
     case 32: return epie_impl<32>(a, s);
