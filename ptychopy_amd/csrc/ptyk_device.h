@@ -180,6 +180,23 @@ template <int N, int BS>
 struct RowLayout {
   __device__ __forceinline__ static int addr(int b, int e) { return b * BS + e; }
 };
+// Rows whose elements are permuted to spread LDS banks: element e of a row of N >= 256 sits at
+// row_swz<N>(e) = e ^ (((e >> 1) ^ (e >> 3)) & 30), a bijection within each aligned block of 32
+// that leaves bit 0 alone (so the pairs (2i, 2i + 1) stay adjacent, in order and 16-byte
+// aligned). Unpermuted, the first Stockham stage's writes (stride R between lanes) meet 8-way
+// (N = 512) and 16-way (N = 256) bank conflicts; permuted, no access of the row transforms
+// exceeds 2-way, for 32 or 64 banks of 4 bytes.
+template <int N>
+__device__ __forceinline__ int row_swz(int e) {
+  if constexpr (N >= 256)
+    return e ^ (((e >> 1) ^ (e >> 3)) & 30);
+  else
+    return e;
+}
+template <int N, int BS>
+struct SwzRowLayout {
+  __device__ __forceinline__ static int addr(int b, int e) { return b * BS + row_swz<N>(e); }
+};
 // Column strips: 16 columns per strip, strip-major, column index fastest.
 template <int N>
 struct StripLayout {

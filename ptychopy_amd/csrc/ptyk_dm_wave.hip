@@ -53,6 +53,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
                                                         const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
   using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
   // two halves of the tile, each with all of its loads in flight before its LDS stores
@@ -83,23 +84,23 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
       const int e = 2 * (tid + (h0 + u) * PTYK_NT);
       const float2 a = cmul(make_float2(pv[u].x, pv[u].y), o0[u]);
       const float2 b = cmul(make_float2(pv[u].z, pv[u].w), o1[u]);
-      *reinterpret_cast<float4*>(&lds[e]) =
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
           make_float4(2.f * a.x - sv[u].x, 2.f * a.y - sv[u].y, 2.f * b.x - sv[u].z, 2.f * b.y - sv[u].w);
     }
   }
   __syncthreads();
-  fft_lds<M, false, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
+  fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
   for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
     const int g = t / M, c = t % M;
     const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
     float2 v[C::R0];
 #pragma unroll
-    for (int q = 0; q < C::R0; ++q) v[q] = lds[(g * C::R0 + q) * M + c];
+    for (int q = 0; q < C::R0; ++q) v[q] = lds[L::addr(g * C::R0 + q, c)];
     Dft<C::R0, false>::run(v);
 #pragma unroll
     for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]);
 #pragma unroll
-    for (int q = 0; q < C::R0; ++q) lds[(g * C::R0 + q) * M + c] = v[q];
+    for (int q = 0; q < C::R0; ++q) lds[L::addr(g * C::R0 + q, c)] = v[q];
   }
   __syncthreads();
 #pragma unroll
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
     long f;
     int r, c;
     if (IO::locate(e, f0, n_groups, &f, &r, &c))
-      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) = *reinterpret_cast<const float4*>(&lds[e]);
+      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) =
+          *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
   }
 }
 
@@ -180,6 +182,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi
                                                         const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
   using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
   {
@@ -193,7 +196,10 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi
                                                      : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
-    for (int i = 0; i < IO::PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = v[i];
+    }
   }
   __syncthreads();
   for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
@@ -201,22 +207,23 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi
     const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
     float2 v[C::R0];
 #pragma unroll
-    for (int q = 0; q < C::R0; ++q) v[q] = lds[(g * C::R0 + q) * M + c];
+    for (int q = 0; q < C::R0; ++q) v[q] = lds[L::addr(g * C::R0 + q, c)];
 #pragma unroll
     for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], cconj(tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]));
     Dft<C::R0, true>::run(v);
 #pragma unroll
-    for (int q = 0; q < C::R0; ++q) lds[(g * C::R0 + q) * M + c] = v[q];
+    for (int q = 0; q < C::R0; ++q) lds[L::addr(g * C::R0 + q, c)] = v[q];
   }
   __syncthreads();
-  fft_lds<M, true, C::ROWS, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
+  fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
 #pragma unroll
   for (int i = 0; i < IO::PAIRS; ++i) {
     const int e = 2 * (tid + i * PTYK_NT);
     long f;
     int r, c;
     if (IO::locate(e, f0, n_groups, &f, &r, &c))
-      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) = *reinterpret_cast<const float4*>(&lds[e]);
+      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) =
+          *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
   }
 }
 

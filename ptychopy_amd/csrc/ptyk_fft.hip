@@ -37,15 +37,20 @@ __global__ __launch_bounds__(PTYK_NT) void k_fft_rows(float2* __restrict__ data,
       v[i] = g < total_rows ? *reinterpret_cast<const float4*>(data + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
-    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(&lds[SwzRowLayout<M, M>::addr(e / M, e % M)]) = v[i];
+    }
   }
   __syncthreads();
-  fft_lds<M, INV, NB, PTYK_NT, RowLayout<M, M>, false>(lds, tw, tid);
+  fft_lds<M, INV, NB, PTYK_NT, SwzRowLayout<M, M>, false>(lds, tw, tid);
 #pragma unroll
   for (int i = 0; i < PAIRS; ++i) {
     const int e = 2 * (tid + i * PTYK_NT);
     const long g = row0 + e / M;
-    if (g < total_rows) *reinterpret_cast<float4*>(data + g * M + (e % M)) = *reinterpret_cast<const float4*>(&lds[e]);
+    if (g < total_rows)
+      *reinterpret_cast<float4*>(data + g * M + (e % M)) =
+          *reinterpret_cast<const float4*>(&lds[SwzRowLayout<M, M>::addr(e / M, e % M)]);
   }
 }
 
