@@ -56,6 +56,10 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
   using L = SwzRowLayout<M, M>;
   __shared__ float2 lds[C::ROWS * M];
   const int tid = threadIdx.x;
+  // every row group of a workgroup belongs to one frame (L is a multiple of GPW), so its ROI is
+  // one uniform load and the object addresses do not wait on a per-lane ROI load
+  static_assert(C::L % C::GPW == 0, "a workgroup's row groups must share a frame");
+  const int2 o = roi[f0 + ((long)blockIdx.x * C::GPW) / C::L];
   // two halves of the tile, each with all of its loads in flight before its LDS stores
   constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
 #pragma unroll
@@ -68,7 +72,6 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
       long f;
       int r, c;
       if (IO::locate(e, f0, n_groups, &f, &r, &c)) {
-        const int2 o = roi[f];
         const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
         sv[u] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
         pv[u] = *reinterpret_cast<const float4*>(probe + r * M + c);
