@@ -598,11 +598,17 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
   }
 }
 
-// x += y * beta, beta from gscal (object: O += precond * mean step; probe: P_ll += pum * mean step).
+// x += y * beta, beta from gscal (object: O += precond * mean step; probe: P_ll += pum * mean step);
+// the second update (x2, y2, count2, beta2), when count2 > 0, runs in the same launch after the
+// first's range.
 __global__ void k_axpy_beta(float2* __restrict__ x, const float2* __restrict__ y, long count,
-                            const double* __restrict__ beta) {
+                            const double* __restrict__ beta, float2* __restrict__ x2, const float2* __restrict__ y2,
+                            long count2, const double* __restrict__ beta2) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) x[i] = cadd(x[i], cmul(y[i], c_of(beta[0], beta[1])));
+  if (i < count)
+    x[i] = cadd(x[i], cmul(y[i], c_of(beta[0], beta[1])));
+  else if (i - count < count2)
+    x2[i - count] = cadd(x2[i - count], cmul(y2[i - count], c_of(beta2[0], beta2[1])));
 }
 
 // Gradient position solver spectra (Probe.cpp:956-1118): t = FFT(obj_proj) * 2 pi i;
@@ -748,13 +754,17 @@ __global__ __launch_bounds__(NT) void k_vp_norm(float2* __restrict__ vmode, int 
   if (q < mm) vmode[q] = cscale(vmode[q], inv);
 }
 
-// [sum |op_j page|^2, sum Re(chi_j conj(op_j page))] per frame chunk (d_multiplyAbsConjuReal).
+// [sum |op_j page|^2, sum Re(chi_j conj(op_j page))] per frame chunk (d_multiplyAbsConjuReal);
+// with page2, the same pair for page2 follows (four values per chunk), so the variable-probe and
+// variable-intensity dots share one pass over op and chi.
 __global__ __launch_bounds__(NT) void k_frame_dots(const float2* __restrict__ op, const float2* __restrict__ page,
-                                                   const float2* __restrict__ chi, int m, double* __restrict__ part) {
+                                                   const float2* __restrict__ page2, const float2* __restrict__ chi,
+                                                   int m, double* __restrict__ part) {
   __shared__ double red[NT / 64];
   const int mm = m * m, ppb = ppb_of(m);
   const int j = blockIdx.y;
-  double d = 0.0, u = 0.0;
+  const int nv = page2 ? 4 : 2;
+  double d = 0.0, u = 0.0, d2 = 0.0, u2 = 0.0;
   for (int t = threadIdx.x; t < ppb; t += NT) {
     const int q = blockIdx.x * ppb + t;
     const long i = (long)j * mm + q;
@@ -762,31 +772,50 @@ __global__ __launch_bounds__(NT) void k_frame_dots(const float2* __restrict__ op
     const double r = (double)a.x * b.x - (double)a.y * b.y, im = (double)a.x * b.y + (double)a.y * b.x;
     d += r * r + im * im;
     u += x.x * r + x.y * im;
+    if (page2) {
+      const float2 b2 = page2[q];
+      const double r2 = (double)a.x * b2.x - (double)a.y * b2.y, im2 = (double)a.x * b2.y + (double)a.y * b2.x;
+      d2 += r2 * r2 + im2 * im2;
+      u2 += x.x * r2 + x.y * im2;
+    }
   }
   d = block_sum<NT>(d, red);
   u = block_sum<NT>(u, red);
+  double* out = part + ((long)j * gridDim.x + blockIdx.x) * nv;
   if (threadIdx.x == 0) {
-    part[((long)j * gridDim.x + blockIdx.x) * 2] = d;
-    part[((long)j * gridDim.x + blockIdx.x) * 2 + 1] = u;
+    out[0] = d;
+    out[1] = u;
+  }
+  if (page2) {
+    d2 = block_sum<NT>(d2, red);
+    u2 = block_sum<NT>(u2, red);
+    if (threadIdx.x == 0) {
+      out[2] = d2;
+      out[3] = u2;
+    }
   }
 }
 
 // Evolution updates: e1 += mean(u) / (mean(d) + 0.1 mean_g mean(d)) (variable probe);
-// e0 += 0.1 sum(u) / sum(d) (variable intensity, Probe.cpp:1496-1516).
+// e0 += 0.1 sum(u) / sum(d) (variable intensity, Probe.cpp:1496-1516). which: 1 the first,
+// 0 the second (two values per chunk), 2 both (four values per chunk, variable probe first).
 __global__ __launch_bounds__(1024) void k_evo_update(const double* __restrict__ part, int ch,
                                                      const int* __restrict__ gf, int G, int mm,
                                                      double* __restrict__ evo, int n, int which,
                                                      double* __restrict__ ftot) {
   __shared__ double red[16];
-  frame_totals<1024>(part, ch, 2, G, ftot);
-  if (which == 1) {
+  const int nv = which == 2 ? 4 : 2;
+  frame_totals<1024>(part, ch, nv, G, ftot);
+  if (which >= 1) {
     double dm = 0.0;
-    for (int j = threadIdx.x; j < G; j += 1024) dm += ftot[2 * j] / mm;
+    for (int j = threadIdx.x; j < G; j += 1024) dm += ftot[nv * j] / mm;
     const double m3 = block_sum<1024>(dm, red) / G;
     for (int j = threadIdx.x; j < G; j += 1024)
-      evo[n + gf[j]] += (ftot[2 * j + 1] / mm) / (ftot[2 * j] / mm + 0.1 * m3);
-  } else {
-    for (int j = threadIdx.x; j < G; j += 1024) evo[gf[j]] += 0.1 * ftot[2 * j + 1] / ftot[2 * j];
+      evo[n + gf[j]] += (ftot[nv * j + 1] / mm) / (ftot[nv * j] / mm + 0.1 * m3);
+  }
+  if (which != 1) {
+    const int o = which == 2 ? 2 : 0;
+    for (int j = threadIdx.x; j < G; j += 1024) evo[gf[j]] += 0.1 * ftot[nv * j + o + 1] / ftot[nv * j + o];
   }
 }
 
@@ -1063,9 +1092,12 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
                d.beta_probe * d.beta_lsq, lsq ? 1 : 0, d.step_obj, d.step_probe, d.ftot, d.gscal);
         CHK();
       }
-      if (obj_rec) LAUNCH(k_axpy_beta, g1(canvas), dim3(NT), d.obj, d.precond, canvas, d.gscal + MLS_S_BO);
-      if (probe_rec && ll <= K)
-        LAUNCH(k_axpy_beta, g1(mm), dim3(NT), d.probe + (long)(ll - 1) * mm, d.pum, (long)mm, d.gscal + MLS_S_BP);
+      {  // object and probe steps in one launch (disjoint arrays, both after the step solve)
+        const long no = obj_rec ? canvas : 0, np = probe_rec && ll <= K ? (long)mm : 0;
+        if (no + np > 0)
+          LAUNCH(k_axpy_beta, g1(no + np), dim3(NT), d.obj, d.precond, no, d.gscal + MLS_S_BO,
+                 d.probe + (long)(ll - 1) * mm, d.pum, np, d.gscal + MLS_S_BP);
+      }
       CHK();
       if (pos_step && ll == 1) {
         LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);  // tmp already holds its spectrum
@@ -1087,12 +1119,15 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
           LAUNCH(k_vp_acc, dim3(nb), dim3(NT), d.acc, d.tmp, G, mm, d.gscal, p2);
           LAUNCH(k_vp_apply, dim3(nb), dim3(NT), d.vmode, d.acc, mm, p2, nb, (double)G / d.n, p3);
           LAUNCH(k_vp_norm, dim3(nb), dim3(NT), d.vmode, mm, p3, nb);
-          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.vmode, chi0, m, d.part);
-          LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, 1, d.ftot);
+          // with variable intensity too, both evolution updates take one pass (neither changes
+          // op, chi, the variable mode or the probe, and they write disjoint halves of evo)
+          const float2* page2 = d.variable_intensity ? d.probe : nullptr;
+          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.vmode, page2, chi0, m, d.part);
+          LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, page2 ? 2 : 1, d.ftot);
           CHK();
         }
-        if (d.variable_intensity) {
-          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.probe, chi0, m, d.part);
+        if (d.variable_intensity && !d.variable_probe) {
+          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.probe, nullptr, chi0, m, d.part);
           LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, 0, d.ftot);
           CHK();
         }
