@@ -212,7 +212,10 @@ struct ColLayout {
 // One radix-R Stockham stage over NB sequences, NT threads, in-place through registers.
 // BATCH_FAST: consecutive threads take consecutive sequences (strip layout) rather than
 // consecutive butterflies of one sequence (row layout).
-template <int N, int R, int NS, bool INV, int NB, int NT, class L, bool BATCH_FAST>
+// PREFETCH: issue every twiddle load of the thread before the first use (interleaved with the
+// LDS reads and the products, the scheduler waited on each load in turn); off where the extra
+// registers would spill (1024-thread workgroups).
+template <int N, int R, int NS, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true>
 __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __restrict__ tw, int tid) {
   constexpr int NBF = NB * (N / R);
   static_assert(NBF % NT == 0, "butterflies must divide evenly among threads");
@@ -220,22 +223,32 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
   constexpr int TWS = PTYK_TWN / (NS * R);  // twiddle table stride for this stage
   float2 v[BPT][R];
   int bb[BPT], jj[BPT];
+  float2 w[BPT][R > 1 ? R : 2];
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
     const int t = tid + q * NT;
-    const int b = BATCH_FAST ? (t % NB) : (t / (N / R));
-    const int j = BATCH_FAST ? (t / NB) : (t % (N / R));
-    bb[q] = b;
-    jj[q] = j;
+    bb[q] = BATCH_FAST ? (t % NB) : (t / (N / R));
+    jj[q] = BATCH_FAST ? (t / NB) : (t % (N / R));
+    if (PREFETCH && NS > 1) {
+      const int k = jj[q] & (NS - 1);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[q][r] = lds[L::addr(b, j + r * (N / R))];
+      for (int r = 1; r < R; ++r) w[q][r] = tw[(k * r * TWS) & (PTYK_TWN - 1)];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = lds[L::addr(bb[q], jj[q] + r * (N / R))];
+  }
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
     if (NS > 1) {
-      const int k = j & (NS - 1);
+      const int k = jj[q] & (NS - 1);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[(k * r * TWS) & (PTYK_TWN - 1)];
-        if (INV) w.y = -w.y;
-        v[q][r] = cmul(v[q][r], w);
+        float2 wr = PREFETCH ? w[q][r] : tw[(k * r * TWS) & (PTYK_TWN - 1)];
+        if (INV) wr.y = -wr.y;
+        v[q][r] = cmul(v[q][r], wr);
       }
     }
     Dft<R, INV>::run(v[q]);
@@ -252,13 +265,13 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
 }
 
 // Full transform of NB sequences already in LDS (caller syncs before the call).
-template <int N, bool INV, int NB, int NT, class L, bool BATCH_FAST>
+template <int N, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ tw, int tid) {
-  stockham_stage<N, plan_radix<N>(0), plan_ns<N>(0), INV, NB, NT, L, BATCH_FAST>(lds, tw, tid);
+  stockham_stage<N, plan_radix<N>(0), plan_ns<N>(0), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
   if constexpr (Plan<N>::S > 1)
-    stockham_stage<N, plan_radix<N>(1), plan_ns<N>(1), INV, NB, NT, L, BATCH_FAST>(lds, tw, tid);
+    stockham_stage<N, plan_radix<N>(1), plan_ns<N>(1), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
   if constexpr (Plan<N>::S > 2)
-    stockham_stage<N, plan_radix<N>(2), plan_ns<N>(2), INV, NB, NT, L, BATCH_FAST>(lds, tw, tid);
+    stockham_stage<N, plan_radix<N>(2), plan_ns<N>(2), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
 }
 
 // ---------------------------------------------------------------- reductions

@@ -46,7 +46,9 @@ struct RowIO {
   }
 };
 
-template <int M>
+// EXIT: the row input is DM's x = 2 P O_r - psi; otherwise psi itself (a plain forward row pass,
+// used by the MLs modulus passes; obj, probe and roi are then not read).
+template <int M, bool EXIT = true>
 __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi, const float2* __restrict__ obj,
                                                         int obj_cols, const float2* __restrict__ probe,
                                                         const int2* __restrict__ roi, long f0, long n_groups,
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
   // every row group of a workgroup belongs to one frame (L is a multiple of GPW), so its ROI is
   // one uniform load and the object addresses do not wait on a per-lane ROI load
   static_assert(C::L % C::GPW == 0, "a workgroup's row groups must share a frame");
-  const int2 o = roi[f0 + ((long)blockIdx.x * C::GPW) / C::L];
+  const int2 o = EXIT ? roi[f0 + ((long)blockIdx.x * C::GPW) / C::L] : make_int2(0, 0);
   // two halves of the tile, each with all of its loads in flight before its LDS stores
   constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
 #pragma unroll
@@ -72,11 +74,13 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
       long f;
       int r, c;
       if (IO::locate(e, f0, n_groups, &f, &r, &c)) {
-        const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
         sv[u] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
-        pv[u] = *reinterpret_cast<const float4*>(probe + r * M + c);
-        o0[u] = orow[0];
-        o1[u] = orow[1];
+        if (EXIT) {
+          const float2* orow = obj + (long)(o.x + r) * obj_cols + (o.y + c);
+          pv[u] = *reinterpret_cast<const float4*>(probe + r * M + c);
+          o0[u] = orow[0];
+          o1[u] = orow[1];
+        }
       } else {
         sv[u] = pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         o0[u] = o1[u] = make_float2(0.f, 0.f);
@@ -85,10 +89,14 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
 #pragma unroll
     for (int u = 0; u < HALF; ++u) {
       const int e = 2 * (tid + (h0 + u) * PTYK_NT);
-      const float2 a = cmul(make_float2(pv[u].x, pv[u].y), o0[u]);
-      const float2 b = cmul(make_float2(pv[u].z, pv[u].w), o1[u]);
-      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
-          make_float4(2.f * a.x - sv[u].x, 2.f * a.y - sv[u].y, 2.f * b.x - sv[u].z, 2.f * b.y - sv[u].w);
+      if (EXIT) {
+        const float2 a = cmul(make_float2(pv[u].x, pv[u].y), o0[u]);
+        const float2 b = cmul(make_float2(pv[u].z, pv[u].w), o1[u]);
+        *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
+            make_float4(2.f * a.x - sv[u].x, 2.f * a.y - sv[u].y, 2.f * b.x - sv[u].z, 2.f * b.y - sv[u].w);
+      } else {
+        *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = sv[u];
+      }
     }
   }
   __syncthreads();
@@ -405,6 +413,145 @@ static hipError_t wave_impl(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
                      a.probe_partial_sum, (int)gpa, a.scal, a.factors);
   if (tm) tm->mark(s, -1);
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ MLs modulus in three passes
+// The MLs exit waves (K modes x G frames, mode-major planes) go through forward FFT, the
+// multi-mode modulus step and the inverse FFT (MLS.cpp:119-140 with Diffractions.cpp:261-310) in
+// three passes instead of five: the row passes above (plain input), and between them one pass
+// over 64 x 32 blocks that holds every mode of a block, so the modulus, which needs
+// sum_k |Psi_k|^2 at each pixel, runs between the forward and inverse 64-point column transforms.
+#define PTYK_MLS_CB 32
+
+// K modes of a 64 x CB block as one batch: sequence b = mode * CB + column, element e = row.
+template <int CB>
+struct ModeColLayout {
+  __device__ __forceinline__ static int addr(int b, int e) { return (b / CB) * (64 * CB) + e * CB + (b % CB); }
+};
+
+// Per block: Psi_k (64-point columns of rows 64 k2 .. 64 k2 + 63 after the row pass, i.e. the
+// frequency rows R0 k1 + k2), s = |sum_k |Psi_k|^2|^(1/2), error partial 4 (A - s)^2, Psi_k *=
+// (A / (s + 1e-9) - 1) / M^2 (k_modulus's arithmetic), inverse columns. One error partial per
+// block: part[j * BPF + block].
+template <int M, int K>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_pass_b(float2* __restrict__ psi, int G, const __half* __restrict__ amp,
+                                                        const float* __restrict__ amp_scale, const int* __restrict__ gf,
+                                                        double* __restrict__ part, const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  constexpr int CB = PTYK_MLS_CB, NCB = M / CB, BPF = C::R0 * NCB;
+  constexpr int BLK = 64 * CB;                  // elements per mode
+  constexpr int PAIRS = BLK / 2 / PTYK_NT;      // float4 per thread and mode
+  constexpr int PIX = BLK / PTYK_NT;            // pixels per thread
+  static_assert(BLK % (2 * PTYK_NT) == 0, "block must divide among the threads");
+  extern __shared__ float2 lds[];               // K x 64 x CB (mode, row, column)
+  __shared__ double red[PTYK_NT / 64];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x / BPF, blk = blockIdx.x % BPF;
+  const int k2 = blk / NCB, c0 = (blk % NCB) * CB;
+  const long mm = (long)M * M;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float2* base = psi + ((long)k * G + j) * mm + (long)(64 * k2) * M + c0;
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      v[i] = *reinterpret_cast<const float4*>(base + (long)(e / CB) * M + (e % CB));
+    }
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[k * BLK + 2 * (tid + i * PTYK_NT)]) = v[i];
+  }
+  // amplitudes of this thread's pixels (frequency row R0 k1 + k2), in flight during the transform
+  const int f = gf[j];
+  const float sc = amp_scale[f];
+  float av[PIX];
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    const int idx = tid + p * PTYK_NT;
+    av[p] = __half2float(amp[(long)f * mm + (long)(C::R0 * (idx / CB) + k2) * M + c0 + idx % CB]) * sc;
+  }
+  __syncthreads();
+  // all modes in one batch: each stage's barrier serves every mode
+  fft_lds<64, false, K * CB, PTYK_NT, ModeColLayout<CB>, true>(lds, tw, tid);
+  const float inv_mm = 1.f / (float)mm;
+  double err = 0.0;
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    const int idx = tid + p * PTYK_NT;
+    float s2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) s2 += cabs2(lds[k * BLK + idx]);
+    const float ap = sqrtf(s2);
+    const double d = (double)(av[p] - ap);
+    err += d * d * 4.0;
+    const float R = (float)((double)av[p] / ((double)ap + 1e-9) - 1.0) * inv_mm;
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * BLK + idx] = cscale(lds[k * BLK + idx], R);
+  }
+  __syncthreads();
+  fft_lds<64, true, K * CB, PTYK_NT, ModeColLayout<CB>, true>(lds, tw, tid);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float2* base = psi + ((long)k * G + j) * mm + (long)(64 * k2) * M + c0;
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(base + (long)(e / CB) * M + (e % CB)) =
+          *reinterpret_cast<const float4*>(&lds[k * BLK + e]);
+    }
+  }
+  err = block_sum<PTYK_NT>(err, red);
+  if (tid == 0) part[(long)j * BPF + blk] = err;
+}
+
+template <int M, int K>
+static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
+                                double* part, const float2* tw, hipStream_t s) {
+  using C = WaveCfg<M>;
+  constexpr int BPF = C::R0 * (M / PTYK_MLS_CB);
+  const long groups = (long)K * G * C::L;
+  const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
+  const size_t lds = (size_t)K * 64 * PTYK_MLS_CB * sizeof(float2);
+  hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
+                     groups, tw);
+  hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(PTYK_NT), lds, s, psi, G, amp, amp_scale,
+                     gf, part, tw);
+  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, 0L, groups, tw);
+  return hipGetLastError();
+}
+
+template <int M>
+static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, const float* amp_scale, const int* gf,
+                                double* part, const float2* tw, hipStream_t s) {
+  switch (K) {
+    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int mls_modulus_blocks(int m, int k) {
+  if ((m != 128 && m != 256 && m != 512) || k < 1 || k > 8) return 0;
+  return (m / 64) * (m / PTYK_MLS_CB);
+}
+
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
+                              const int* gf, double* part, const float2* tw, hipStream_t s) {
+  switch (mls_modulus_blocks(m, K) ? m : 0) {
+    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, s);
+    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, s);
+    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 bool dm_wave_supported(int m) { return m == 128 || m == 256 || m == 512; }

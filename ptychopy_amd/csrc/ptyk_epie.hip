@@ -36,7 +36,7 @@ __device__ void wg_fft2(float2* __restrict__ g, float2* lds, const float2* __res
       lds[e] = row < M ? g[row * M + (e % M)] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<M, INV, RNB, NT, RowLayout<M, M>, false>(lds, tw, tid);
+    fft_lds<M, INV, RNB, NT, RowLayout<M, M>, false, false>(lds, tw, tid);
     for (int e = tid; e < RNB * M; e += NT) {
       const int row = r0 + e / M;
       if (row < M) g[row * M + (e % M)] = lds[e];
@@ -49,7 +49,7 @@ __device__ void wg_fft2(float2* __restrict__ g, float2* lds, const float2* __res
       lds[e] = col < M ? g[(e / CNB) * M + col] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<M, INV, CNB, NT, ColLayout<CNB>, true>(lds, tw, tid);
+    fft_lds<M, INV, CNB, NT, ColLayout<CNB>, true, false>(lds, tw, tid);
     for (int e = tid; e < CNB * M; e += NT) {
       const int col = c0 + e % CNB;
       if (col < M) g[(e / CNB) * M + col] = lds[e];

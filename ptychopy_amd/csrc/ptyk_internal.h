@@ -80,6 +80,12 @@ struct DmWaveArgs {
 bool dm_wave_supported(int m);
 int dm_wave_probe_blocks(int m);
 hipError_t dm_wave_iteration(const DmWaveArgs& a, hipStream_t s, DmTimer* tm /* nullable */);
+// MLs forward FFT + multi-mode modulus step + inverse FFT of K mode-major planes of G frames in
+// three passes (ptyk_dm_wave.hip). mls_modulus_blocks: error partials per frame, 0 when (m, K)
+// is not supported (m in {128, 256, 512}, K <= 8).
+int mls_modulus_blocks(int m, int k);
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
+                              const int* gf, double* part, const float2* tw, hipStream_t s);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {

@@ -1034,13 +1034,20 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K, m,
            d.cols, subpix);
     CHK();
-    // with the position step, tmp (the object patch, right after the exit-wave buffer) rides in
-    // the same forward transform (frames between K G and K gmax are unused scratch)
-    TRY(fft2_batch(d.psi, pos_step ? (long)K * d.gmax + G : (long)K * G, m, false, d.tw, s));
-    LAUNCH(k_modulus, gch, dim3(NT), d.psi, d.amp, d.amp_scale, gf, G, K, m, d.part);
-    CHK();
-    TRY(fft2_batch(d.psi, (long)K * G, m, true, d.tw, s));
-    if (iter < 10) LAUNCH(k_ferr, g1(G), dim3(NT), d.part, ch, gf, G, mm, d.ferr);
+    const int mod_blocks = mls_modulus_blocks(m, K);
+    if (mod_blocks) {
+      // three passes for the exit waves; the object patch takes its own forward transform
+      if (pos_step) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, s));
+    } else {
+      // with the position step, tmp (the object patch, right after the exit-wave buffer) rides
+      // in the same forward transform (frames between K G and K gmax are unused scratch)
+      TRY(fft2_batch(d.psi, pos_step ? (long)K * d.gmax + G : (long)K * G, m, false, d.tw, s));
+      LAUNCH(k_modulus, gch, dim3(NT), d.psi, d.amp, d.amp_scale, gf, G, K, m, d.part);
+      CHK();
+      TRY(fft2_batch(d.psi, (long)K * G, m, true, d.tw, s));
+    }
+    if (iter < 10) LAUNCH(k_ferr, g1(G), dim3(NT), d.part, mod_blocks ? mod_blocks : ch, gf, G, mm, d.ferr);
     if (!probe_rec) {  // probe_update / probe_update_m stay zero (MLS.cpp:146-153)
       LAUNCH(k_fill_c, g1(gm), dim3(NT), d.pu, gm, make_float2(0.f, 0.f));
       LAUNCH(k_fill_c, g1(mm), dim3(NT), d.pum, (long)mm, make_float2(0.f, 0.f));

@@ -72,13 +72,17 @@ CONFIGS = [
     dict(k=2, cfg=dict(variable_probe=0, variable_intensity=0, variable_probe_modes=0, beta_lsq=0.0)),
     dict(k=2, cfg=dict(apply_multimodal_update=1, probe_pos_search=4)),
     dict(k=5, cfg=dict(probe_pos_search=3)),  # five modes: the fused update of modes 2..5 (8-wide form)
+    # 128^2 / 256^2 frames take the three-pass modulus (row pass, K-mode 64 x 32 blocks, row pass)
+    dict(k=1, m=128, n_side=3, cfg=dict(probe_pos_search=2, variable_probe=0)),
+    dict(k=3, m=128, n_side=3, cfg=dict(probe_pos_search=3)),
+    dict(k=5, m=256, n_side=3, cfg=dict(probe_pos_search=99)),
 ]
 
 
 @pytest.mark.parametrize("case", range(len(CONFIGS)))
 def test_mls_iterations_match_oracle(gpu, orc, case):
     c = CONFIGS[case]
-    P = _problem(k=c["k"], seed=case)
+    P = _problem(k=c["k"], m=c.get("m", 32), n_side=c.get("n_side", 5), seed=case)
     e = _engine_for(P, P["state"].amp, **c["cfg"])
     amp = e.get_frames().astype(np.float64)
     okw = {("pps" if key == "probe_pos_search" else key): v for key, v in c["cfg"].items()}
