@@ -46,6 +46,43 @@ struct RowIO {
   }
 };
 
+// The radix-R0 step across the rows {n1 + 64 n2} of each row group in LDS, with the twiddle
+// W_M^(n1 k2) after the forward DFT or (conjugated) before the inverse one. For M < 512 the
+// GPW M / NT steps of a thread are unrolled so that all their twiddle loads are in flight at
+// once; at M = 512 (two steps) the runtime loop measured faster.
+template <int M, bool INV>
+__device__ __forceinline__ void cross_rows(float2* lds, const float2* __restrict__ tw, int tid) {
+  using C = WaveCfg<M>;
+  using L = SwzRowLayout<M, M>;
+  constexpr int STEPS = C::GPW * M / PTYK_NT;
+  static_assert(C::GPW * M % PTYK_NT == 0, "cross-row step must divide among the threads");
+  auto step = [&](int t) {
+    const int g = t / M, c = t % M;
+    const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
+    float2 w[C::R0];
+#pragma unroll
+    for (int k2 = 1; k2 < C::R0; ++k2) {
+      w[k2] = tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)];
+      if (INV) w[k2] = cconj(w[k2]);
+    }
+    float2 v[C::R0];
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) v[q] = lds[L::addr(g * C::R0 + q, c)];
+    if (!INV) Dft<C::R0, false>::run(v);
+#pragma unroll
+    for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], w[k2]);
+    if (INV) Dft<C::R0, true>::run(v);
+#pragma unroll
+    for (int q = 0; q < C::R0; ++q) lds[L::addr(g * C::R0 + q, c)] = v[q];
+  };
+  if constexpr (M < 512) {
+#pragma unroll
+    for (int it = 0; it < STEPS; ++it) step(tid + it * PTYK_NT);
+  } else {
+    for (int t = tid; t < C::GPW * M; t += PTYK_NT) step(t);
+  }
+}
+
 // EXIT: the row input is DM's x = 2 P O_r - psi; otherwise psi itself (a plain forward row pass,
 // used by the MLs modulus passes; obj, probe and roi are then not read).
 template <int M, bool EXIT = true>
@@ -101,18 +138,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
   }
   __syncthreads();
   fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
-  for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
-    const int g = t / M, c = t % M;
-    const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
-    float2 v[C::R0];
-#pragma unroll
-    for (int q = 0; q < C::R0; ++q) v[q] = lds[L::addr(g * C::R0 + q, c)];
-    Dft<C::R0, false>::run(v);
-#pragma unroll
-    for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]);
-#pragma unroll
-    for (int q = 0; q < C::R0; ++q) lds[L::addr(g * C::R0 + q, c)] = v[q];
-  }
+  cross_rows<M, false>(lds, tw, tid);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < IO::PAIRS; ++i) {
@@ -213,18 +239,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_c(float2* __restrict__ psi
     }
   }
   __syncthreads();
-  for (int t = tid; t < C::GPW * M; t += PTYK_NT) {
-    const int g = t / M, c = t % M;
-    const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
-    float2 v[C::R0];
-#pragma unroll
-    for (int q = 0; q < C::R0; ++q) v[q] = lds[L::addr(g * C::R0 + q, c)];
-#pragma unroll
-    for (int k2 = 1; k2 < C::R0; ++k2) v[k2] = cmul(v[k2], cconj(tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)]));
-    Dft<C::R0, true>::run(v);
-#pragma unroll
-    for (int q = 0; q < C::R0; ++q) lds[L::addr(g * C::R0 + q, c)] = v[q];
-  }
+  cross_rows<M, true>(lds, tw, tid);
   __syncthreads();
   fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
 #pragma unroll
