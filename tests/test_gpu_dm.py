@@ -1,6 +1,6 @@
 """DM iteration on the GPU against the FP64 oracle, starting each check from the GPU's state.
 
-Tolerance (north star): per-iteration object and probe within 1e-5 relative (L2 norm) of
+Tolerance: per-iteration object and probe within 1e-5 relative (L2 norm) of
 the oracle's FP64 result from identical inputs (dequantised FP16 amplitudes, the same
 positions and the same cached factors).  The exit-wave state psi_new = psi + Pf - 2 P O_r
 is a difference of terms of the size of the transform input x = 2 P O_r - psi, so in any
