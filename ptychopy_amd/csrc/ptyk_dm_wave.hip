@@ -186,24 +186,27 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_b(float2* __restrict__ psi
   }
   __syncthreads();
   fft_lds<64, false, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
+  // each lane's pair of values moves as one 16-byte LDS access (two 8-byte accesses 16 bytes
+  // apart across the lanes would meet bank conflicts)
 #pragma unroll
   for (int i = 0; i < PAIRS; ++i) {
     const int e = 2 * (tid + i * PTYK_NT);
     const float2 amp2 = __half22float2(av[i]);
+    const float4 q = *reinterpret_cast<const float4*>(&lds[e]);
+    float2 d[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const float A = (h ? amp2.y : amp2.x) * sc;
-      const float2 ph = lds[e + h];
+      const float2 ph = h ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
       const float m2 = cabs2(ph);
-      float2 d;
       if (m2 > 0.f) {
         const float k = (A * rsqrtf(m2) - 1.f) * inv_m2;
-        d = make_float2(ph.x * k, ph.y * k);
+        d[h] = make_float2(ph.x * k, ph.y * k);
       } else {
-        d = make_float2(A * inv_m2, 0.f);
+        d[h] = make_float2(A * inv_m2, 0.f);
       }
-      lds[e + h] = d;
     }
+    *reinterpret_cast<float4*>(&lds[e]) = make_float4(d[0].x, d[0].y, d[1].x, d[1].y);
   }
   __syncthreads();
   fft_lds<64, true, 64, PTYK_NT, ColLayout<64>, true>(lds, tw, tid);
