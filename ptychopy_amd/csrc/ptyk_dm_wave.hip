@@ -83,6 +83,29 @@ __device__ __forceinline__ void cross_rows(float2* lds, const float2* __restrict
   }
 }
 
+// Forward half of a row pass once its input rows are in LDS (the caller has not synchronised):
+// row FFTs, the radix-R0 step across rows, and the store of the tile.
+template <int M>
+__device__ __forceinline__ void row_pass_forward(float2* lds, float2* __restrict__ psi, long f0, long n_groups,
+                                                 const float2* __restrict__ tw, int tid) {
+  using C = WaveCfg<M>;
+  using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
+  __syncthreads();
+  fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+  cross_rows<M, false>(lds, tw, tid);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c;
+    if (IO::locate(e, f0, n_groups, &f, &r, &c))
+      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) =
+          *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+  }
+}
+
 // EXIT: the row input is DM's x = 2 P O_r - psi; otherwise psi itself (a plain forward row pass,
 // used by the MLs modulus passes; obj, probe and roi are then not read).
 template <int M, bool EXIT = true>
@@ -136,19 +159,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_dmw_pass_a(float2* __restrict__ psi
       }
     }
   }
-  __syncthreads();
-  fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
-  cross_rows<M, false>(lds, tw, tid);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < IO::PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    long f;
-    int r, c;
-    if (IO::locate(e, f0, n_groups, &f, &r, &c))
-      *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) =
-          *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
-  }
+  row_pass_forward<M>(lds, psi, f0, n_groups, tw, tid);
 }
 
 template <int M>
@@ -441,6 +452,68 @@ static hipError_t wave_impl(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
 // sum_k |Psi_k|^2 at each pixel, runs between the forward and inverse 64-point column transforms.
 #define PTYK_MLS_CB 32
 
+// Row pass of the MLs exit waves formed in place of their load (MLS.cpp:119-126 with
+// d_readFromArray_c, utilitiesKernels.cu:1064-1084): plane k G + j, row r holds
+// psi_k = P_k O_r, or O_r vp_j for mode 0 with the sub-pixel shift; mode 0's workgroups also
+// write the object patch O_r (op, and op2 for the position step). Same arithmetic as the exit-wave
+// kernel it replaces, so the transform sees the same values.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_pass_a(float2* __restrict__ psi, MlsExitArgs x, int G,
+                                                        long n_groups, const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
+  static_assert(C::L % C::GPW == 0, "a workgroup's row groups must share a plane");
+  __shared__ float2 lds[C::ROWS * M];
+  const int tid = threadIdx.x;
+  const long plane = ((long)blockIdx.x * C::GPW) / C::L;  // uniform: k G + j
+  const int k = (int)(plane / G), j = (int)(plane % G);
+  const int2 o = x.roi[x.gf[j]];
+  const long mm = (long)M * M;
+  const float2* pk = x.probe + (long)k * mm;
+  const bool shifted = k == 0 && x.subpix;
+  constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
+#pragma unroll
+  for (int h0 = 0; h0 < IO::PAIRS; h0 += HALF) {
+    float4 pv[HALF];
+    float2 o0[HALF], o1[HALF];
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      long f;
+      int r, c;
+      if (IO::locate(e, 0, n_groups, &f, &r, &c)) {
+        const float2* orow = x.obj + (long)(o.x + r) * x.cols + (o.y + c);
+        o0[u] = orow[0];
+        o1[u] = orow[1];
+        pv[u] = shifted ? *reinterpret_cast<const float4*>(x.vp + (long)j * mm + r * M + c)
+                        : *reinterpret_cast<const float4*>(pk + r * M + c);
+      } else {
+        pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        o0[u] = o1[u] = make_float2(0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      long f;
+      int r, c;
+      const bool in = IO::locate(e, 0, n_groups, &f, &r, &c);
+      const float2 p0 = make_float2(pv[u].x, pv[u].y), p1 = make_float2(pv[u].z, pv[u].w);
+      // k_psi's operand order: O * vp for the shifted mode 0, P * O otherwise
+      const float2 a = shifted ? cmul(o0[u], p0) : cmul(p0, o0[u]);
+      const float2 b = shifted ? cmul(o1[u], p1) : cmul(p1, o1[u]);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = make_float4(a.x, a.y, b.x, b.y);
+      if (k == 0 && in) {
+        const float4 ov = make_float4(o0[u].x, o0[u].y, o1[u].x, o1[u].y);
+        *reinterpret_cast<float4*>(x.op + (long)j * mm + r * M + c) = ov;
+        if (x.op2) *reinterpret_cast<float4*>(x.op2 + (long)j * mm + r * M + c) = ov;
+      }
+    }
+  }
+  row_pass_forward<M>(lds, psi, 0, n_groups, tw, tid);
+}
+
 // K modes of a 64 x CB block as one batch: sequence b = mode * CB + column, element e = row.
 template <int CB>
 struct ModeColLayout {
@@ -524,7 +597,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_b(float2* __restrict__ psi
 
 template <int M, int K>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, hipStream_t s) {
+                                double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
   using C = WaveCfg<M>;
   constexpr int BPF = C::R0 * (M / PTYK_MLS_CB);
   const long groups = (long)K * G * C::L;
@@ -533,8 +606,11 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
   hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
-                     groups, tw);
+  if (exit)
+    hipLaunchKernelGGL((k_mls_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
+  else
+    hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
+                       groups, tw);
   hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(PTYK_NT), lds, s, psi, G, amp, amp_scale,
                      gf, part, tw);
   hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, 0L, groups, tw);
@@ -543,16 +619,16 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
 
 template <int M>
 static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, hipStream_t s) {
+                                double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
   switch (K) {
-    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, s);
-    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, s);
+    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -563,11 +639,11 @@ int mls_modulus_blocks(int m, int k) {
 }
 
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
-                              const int* gf, double* part, const float2* tw, hipStream_t s) {
+                              const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
   switch (mls_modulus_blocks(m, K) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, s);
-    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, s);
-    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, s);
+    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -83,9 +83,24 @@ hipError_t dm_wave_iteration(const DmWaveArgs& a, hipStream_t s, DmTimer* tm /* 
 // MLs forward FFT + multi-mode modulus step + inverse FFT of K mode-major planes of G frames in
 // three passes (ptyk_dm_wave.hip). mls_modulus_blocks: error partials per frame, 0 when (m, K)
 // is not supported (m in {128, 256, 512}, K <= 8).
+// exit: when given, the first pass forms the exit waves itself (psi_k = P_k O_r, or O_r vp_j for
+// mode 0 with the sub-pixel shift) and writes the object patches op (and op2); otherwise psi
+// already holds them.
+struct MlsExitArgs {
+  const float2* obj;
+  int cols;
+  const float2* probe;  // K x M x M
+  const float2* vp;     // G x M x M, mode 0's shifted probe (subpix)
+  const int2* roi;
+  const int* gf;
+  int subpix;
+  float2* op;
+  float2* op2;  // nullable
+};
 int mls_modulus_blocks(int m, int k);
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
-                              const int* gf, double* part, const float2* tw, hipStream_t s);
+                              const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
+                              hipStream_t s);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {

@@ -1031,15 +1031,17 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
     CHK();
     if (subpix) TRY(fft2_batch(d.vp, G, m, true, d.tw, s));
-    LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K, m,
-           d.cols, subpix);
-    CHK();
     const int mod_blocks = mls_modulus_blocks(m, K);
     if (mod_blocks) {
-      // three passes for the exit waves; the object patch takes its own forward transform
+      // three passes for the exit waves, the first of which forms them (and the object patches)
+      // from O, P and vp; the position step's patch copy takes its own forward transform
+      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, pos_step ? d.tmp : nullptr};
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, s));
       if (pos_step) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
-      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, s));
     } else {
+      LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
+             m, d.cols, subpix);
+      CHK();
       // with the position step, tmp (the object patch, right after the exit-wave buffer) rides
       // in the same forward transform (frames between K G and K gmax are unused scratch)
       TRY(fft2_batch(d.psi, pos_step ? (long)K * d.gmax + G : (long)K * G, m, false, d.tw, s));
