@@ -514,6 +514,97 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_a(float2* __restrict__ psi
   row_pass_forward<M>(lds, psi, 0, n_groups, tw, tid);
 }
 
+// Row pass of the MLs probe-update planes formed in place of their load: plane j, row r holds
+// chi0_j conj(op_j) (the arithmetic of the probe-update kernel it replaces).
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_pu_pass_a(float2* __restrict__ pu, const float2* __restrict__ chi,
+                                                           const float2* __restrict__ op, long n_groups,
+                                                           const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[C::ROWS * M];
+  const int tid = threadIdx.x;
+  constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
+#pragma unroll
+  for (int h0 = 0; h0 < IO::PAIRS; h0 += HALF) {
+    float4 av[HALF], bv[HALF];
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      long f;
+      int r, c;
+      if (IO::locate(e, 0, n_groups, &f, &r, &c)) {
+        av[u] = *reinterpret_cast<const float4*>(chi + (f * M + r) * M + c);
+        bv[u] = *reinterpret_cast<const float4*>(op + (f * M + r) * M + c);
+      } else {
+        av[u] = bv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
+      const float4 a = av[u], b = bv[u];
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
+          make_float4(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y, a.z * b.z + a.w * b.w, a.w * b.z - a.z * b.w);
+    }
+  }
+  row_pass_forward<M>(lds, pu, 0, n_groups, tw, tid);
+}
+
+// Frequency of FFT index i in cycles per sample, in [-0.5, 0.5) (the MLs spectral shift's).
+__device__ __forceinline__ float shift_freq(int i, int m) {
+  return (float)((i + (m >> 1)) & (m - 1)) / (float)m - 0.5f;
+}
+
+// One plane's 64 x 32 block between the row passes: 64-point columns, the spectral shift
+// x *= exp(-2 pi i sign (sub.x f[col] + sub.y f[row])) / M^2 (the arithmetic of the ramp kernel it
+// replaces), inverse columns.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_ramp_b(float2* __restrict__ x, const float2* __restrict__ sub,
+                                                        const int* __restrict__ gf, float sign,
+                                                        const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  constexpr int CB = PTYK_MLS_CB, NCB = M / CB, BPF = C::R0 * NCB;
+  constexpr int BLK = 64 * CB, PAIRS = BLK / 2 / PTYK_NT, PIX = BLK / PTYK_NT;
+  __shared__ float2 lds[BLK];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x / BPF, blk = blockIdx.x % BPF;
+  const int k2 = blk / NCB, c0 = (blk % NCB) * CB;
+  const long mm = (long)M * M;
+  float2* base = x + (long)j * mm + (long)(64 * k2) * M + c0;
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      v[i] = *reinterpret_cast<const float4*>(base + (long)(e / CB) * M + (e % CB));
+    }
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
+  }
+  const float2 sv = sub[gf[j]];
+  __syncthreads();
+  fft_lds<64, false, CB, PTYK_NT, ColLayout<CB>, true>(lds, tw, tid);
+  const float inv_mm = 1.f / (float)mm;
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    const int idx = tid + p * PTYK_NT;
+    const int r = C::R0 * (idx / CB) + k2, c = c0 + idx % CB;
+    const float nom = (sign * sv.x) * shift_freq(c, M) + (sign * sv.y) * shift_freq(r, M);
+    float sn, cs;
+    sincosf(nom * -6.283185307179586f, &sn, &cs);
+    lds[idx] = cscale(cmul(lds[idx], make_float2(cs, sn)), inv_mm);
+  }
+  __syncthreads();
+  fft_lds<64, true, CB, PTYK_NT, ColLayout<CB>, true>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    *reinterpret_cast<float4*>(base + (long)(e / CB) * M + (e % CB)) = *reinterpret_cast<const float4*>(&lds[e]);
+  }
+}
+
 // K modes of a 64 x CB block as one batch: sequence b = mode * CB + column, element e = row.
 template <int CB>
 struct ModeColLayout {
@@ -644,6 +735,29 @@ hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* am
     case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
     case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
     case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int M>
+static hipError_t mls_shift_m(float2* pu, const float2* chi, const float2* op, int G, const float2* sub,
+                              const int* gf, float sign, const float2* tw, hipStream_t s) {
+  using C = WaveCfg<M>;
+  constexpr int BPF = C::R0 * (M / PTYK_MLS_CB);
+  const long groups = (long)G * C::L;
+  const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
+  hipLaunchKernelGGL((k_mls_pu_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, pu, chi, op, groups, tw);
+  hipLaunchKernelGGL((k_mls_ramp_b<M>), dim3((unsigned)(G * BPF)), dim3(PTYK_NT), 0, s, pu, sub, gf, sign, tw);
+  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, pu, 0L, groups, tw);
+  return hipGetLastError();
+}
+
+hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,
+                            const int* gf, float sign, const float2* tw, hipStream_t s) {
+  switch (m) {
+    case 128: return mls_shift_m<128>(pu, chi, op, G, sub, gf, sign, tw, s);
+    case 256: return mls_shift_m<256>(pu, chi, op, G, sub, gf, sign, tw, s);
+    case 512: return mls_shift_m<512>(pu, chi, op, G, sub, gf, sign, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

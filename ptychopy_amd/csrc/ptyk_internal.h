@@ -98,6 +98,10 @@ struct MlsExitArgs {
   float2* op2;  // nullable
 };
 int mls_modulus_blocks(int m, int k);
+// pu_j = IFFT(exp(-2 pi i sign (sub f)) / M^2 FFT(chi_j conj(op_j))) for G planes in three passes
+// (m in {128, 256, 512}; ptyk_dm_wave.hip).
+hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,
+                            const int* gf, float sign, const float2* tw, hipStream_t s);
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
                               hipStream_t s);

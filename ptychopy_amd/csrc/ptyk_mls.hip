@@ -1066,13 +1066,18 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       const float2* chi = d.psi + (long)(ll - 1) * gm;
       if (probe_rec) {
         if (ll == 1) {
-          LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
-          CHK();
-          if (subpix) {
-            TRY(fft2_batch(d.pu, G, m, false, d.tw, s));
-            LAUNCH(k_ramp, gpix, dim3(NT), d.pu, d.sub, gf, m, -1.f);
+          if (subpix && mod_blocks) {
+            // chi0 conj(op), its transform, the shift and the inverse in three passes
+            TRY(mls_shift_passes(d.pu, chi0, d.op, G, m, d.sub, gf, -1.f, d.tw, s));
+          } else {
+            LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
             CHK();
-            TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
+            if (subpix) {
+              TRY(fft2_batch(d.pu, G, m, false, d.tw, s));
+              LAUNCH(k_ramp, gpix, dim3(NT), d.pu, d.sub, gf, m, -1.f);
+              CHK();
+              TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
+            }
           }
           LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
         } else {
