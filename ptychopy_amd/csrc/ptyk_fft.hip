@@ -108,6 +108,25 @@ static hipError_t fft2_impl(float2* data, long n, const float2* tw, hipStream_t 
   return hipGetLastError();
 }
 
+template <int M, bool INV>
+static hipError_t fft_cols_impl(float2* data, long n, const float2* tw, hipStream_t s) {
+  const long strips = n * (M / 16);
+  const long gc = (strips + FftCfg<M>::CNB / 16 - 1) / (FftCfg<M>::CNB / 16);
+  if (gc > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_fft_cols<M, INV>), dim3((unsigned)gc), dim3(PTYK_NT), 0, s, data, strips, tw);
+  return hipGetLastError();
+}
+
+hipError_t fft_cols_batch(float2* data, long n, int m, bool inv, const float2* tw, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  switch (m) {
+    case 128: return inv ? fft_cols_impl<128, true>(data, n, tw, s) : fft_cols_impl<128, false>(data, n, tw, s);
+    case 256: return inv ? fft_cols_impl<256, true>(data, n, tw, s) : fft_cols_impl<256, false>(data, n, tw, s);
+    case 512: return inv ? fft_cols_impl<512, true>(data, n, tw, s) : fft_cols_impl<512, false>(data, n, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t fft2_batch(float2* data, long n, int m, bool inv, const float2* tw, hipStream_t s) {
   if (n <= 0) return hipSuccess;
 #define PTYK_FFT_CASE(MM) \

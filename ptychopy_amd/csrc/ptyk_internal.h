@@ -13,6 +13,9 @@ inline bool frame_size_supported(int m) {
 
 // ---- batched 2D FFT of n frames of m x m complex64, in place (rows then columns).
 hipError_t fft2_batch(float2* data, long n_frames, int m, bool inverse, const float2* tw, hipStream_t s);
+// The column half of fft2_batch alone (m in {128, 256, 512}), for callers whose row half is fused
+// into a kernel of their own.
+hipError_t fft_cols_batch(float2* data, long n_frames, int m, bool inverse, const float2* tw, hipStream_t s);
 
 // ---- DM (difference map) per-iteration launches; see ptyk_dm.hip for the equations.
 struct DmArgs {

@@ -303,6 +303,76 @@ __global__ void k_vp(float2* __restrict__ vp, const float2* __restrict__ spec, c
   vp[(long)j * mm + q] = v;
 }
 
+// The sub-pixel form of k_vp fused into the inverse row transform: the spectrum
+// ramp_j (e0 F0 + e1 FV) / M^2 is formed (k_vp's arithmetic) in the load of each row, the rows are
+// inverse-transformed in LDS and stored; the caller then runs the inverse column pass.
+template <int M>
+__global__ __launch_bounds__(NT) void k_vp_rows(float2* __restrict__ vp, const float2* __restrict__ spec,
+                                                const double* __restrict__ evo, const float2* __restrict__ sub,
+                                                const int* __restrict__ gf, int n, int var_any,
+                                                const float2* __restrict__ tw) {
+  constexpr int NB = plan_min_batch<M, NT>();
+  constexpr int PAIRS = NB * M / 2 / NT;
+  static_assert(M % NB == 0 && (NB * M / 2) % NT == 0, "a workgroup's rows must share a plane");
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[NB * M];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * NB;
+  const int j = (int)(row0 / M), f = gf[j];
+  const float e0 = var_any ? (float)evo[f] : 1.f, e1 = var_any ? (float)evo[n + f] : 0.f;
+  const float2 sv = sub[f];
+  constexpr int MM = M * M;
+  float4 av[PAIRS], bv[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * NT);
+    const int q = (int)(row0 % M + e / M) * M + e % M;
+    av[i] = *reinterpret_cast<const float4*>(spec + q);
+    bv[i] = *reinterpret_cast<const float4*>(spec + MM + q);
+  }
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * NT);
+    const int r = (int)(row0 % M) + e / M, c = e % M;
+    float2 v[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float2 a = h ? make_float2(av[i].z, av[i].w) : make_float2(av[i].x, av[i].y);
+      const float2 b = h ? make_float2(bv[i].z, bv[i].w) : make_float2(bv[i].x, bv[i].y);
+      const float2 w = make_float2(e0 * a.x + e1 * b.x, e0 * a.y + e1 * b.y);
+      const float nom = sv.x * freq(c + h, M) + sv.y * freq(r, M);
+      float sn, cs;
+      sincosf(nom * -kTwoPi, &sn, &cs);
+      v[h] = cscale(cmul(w, make_float2(cs, sn)), 1.f / (float)MM);
+    }
+    *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NB, NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * NT);
+    *reinterpret_cast<float4*>(vp + (row0 + e / M) * M + e % M) =
+        *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+  }
+}
+
+static hipError_t vp_rows(float2* vp, const float2* spec, const double* evo, const float2* sub, const int* gf, int n,
+                          int var_any, int G, int m, const float2* tw, hipStream_t s) {
+  switch (m) {
+#define PTYK_VP_CASE(MM)                                                                                        \
+  case MM:                                                                                                      \
+    hipLaunchKernelGGL((k_vp_rows<MM>), dim3((unsigned)((long)G * MM / plan_min_batch<MM, NT>())), dim3(NT), 0, \
+                       s, vp, spec, evo, sub, gf, n, var_any, tw);                                              \
+    return hipGetLastError();
+    PTYK_VP_CASE(128)
+    PTYK_VP_CASE(256)
+    PTYK_VP_CASE(512)
+#undef PTYK_VP_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // obj_proj = O at the ROI (d_readFromArray_c, utilitiesKernels.cu:1064-1084);
 // psi_0 = obj_proj * vp (with the sub-pixel shift) or P0 * obj_proj, psi_k = P_k * obj_proj.
 __global__ void k_psi(float2* __restrict__ psi, float2* __restrict__ op, float2* __restrict__ op2,
@@ -1028,10 +1098,16 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       CHK();
       TRY(fft2_batch(d.spec, 2, m, false, d.tw, s));
     }
-    LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
-    CHK();
-    if (subpix) TRY(fft2_batch(d.vp, G, m, true, d.tw, s));
     const int mod_blocks = mls_modulus_blocks(m, K);
+    if (subpix && mod_blocks) {
+      // the shifted illumination probes' spectra formed in the inverse row pass itself
+      TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
+      TRY(fft_cols_batch(d.vp, G, m, true, d.tw, s));
+    } else {
+      LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
+      CHK();
+      if (subpix) TRY(fft2_batch(d.vp, G, m, true, d.tw, s));
+    }
     if (mod_blocks) {
       // three passes for the exit waves, the first of which forms them (and the object patches)
       // from O, P and vp; the position step's patch copy takes its own forward transform
