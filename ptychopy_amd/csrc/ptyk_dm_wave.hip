@@ -613,53 +613,57 @@ struct ModeColLayout {
 
 // Per block: Psi_k (64-point columns of rows 64 k2 .. 64 k2 + 63 after the row pass, i.e. the
 // frequency rows R0 k1 + k2), s = |sum_k |Psi_k|^2|^(1/2), error partial 4 (A - s)^2, Psi_k *=
-// (A / (s + 1e-9) - 1) / M^2 (k_modulus's arithmetic), inverse columns. One error partial per
-// block: part[j * BPF + block].
+// (A / (s + 1e-9) - 1) / M^2 (k_modulus's arithmetic), inverse columns. One wave per mode (64 K
+// threads) over 64 x 16 blocks: 16 KiB of LDS per mode, so four workgroups share a CU at K = 5.
+// One error partial per block: part[j * BPF + block].
+#define PTYK_MLS_MOD_CB 16
 template <int M, int K>
-__global__ __launch_bounds__(PTYK_NT) void k_mls_pass_b(float2* __restrict__ psi, int G, const __half* __restrict__ amp,
-                                                        const float* __restrict__ amp_scale, const int* __restrict__ gf,
-                                                        double* __restrict__ part, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi, int G, const __half* __restrict__ amp,
+                                                       const float* __restrict__ amp_scale, const int* __restrict__ gf,
+                                                       double* __restrict__ part, const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
-  constexpr int CB = PTYK_MLS_CB, NCB = M / CB, BPF = C::R0 * NCB;
-  constexpr int BLK = 64 * CB;                  // elements per mode
-  constexpr int PAIRS = BLK / 2 / PTYK_NT;      // float4 per thread and mode
-  constexpr int PIX = BLK / PTYK_NT;            // pixels per thread
-  static_assert(BLK % (2 * PTYK_NT) == 0, "block must divide among the threads");
-  extern __shared__ float2 lds[];               // K x 64 x CB (mode, row, column)
-  __shared__ double red[PTYK_NT / 64];
+  constexpr int BT = 64 * K;
+  constexpr int CB = PTYK_MLS_MOD_CB, NCB = M / CB, BPF = C::R0 * NCB;
+  constexpr int BLK = 64 * CB;         // elements per mode
+  constexpr int PAIRS = BLK / 2 / 64;  // float4 per lane of the mode's wave
+  extern __shared__ float2 lds[];      // K x 64 x CB (mode, row, column)
+  __shared__ double red[K];
   const int tid = threadIdx.x;
+  const int wk = tid >> 6, ln = tid & 63;  // this wave's mode, lane
   const int j = blockIdx.x / BPF, blk = blockIdx.x % BPF;
   const int k2 = blk / NCB, c0 = (blk % NCB) * CB;
   const long mm = (long)M * M;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const float2* base = psi + ((long)k * G + j) * mm + (long)(64 * k2) * M + c0;
+  {
+    const float2* base = psi + ((long)wk * G + j) * mm + (long)(64 * k2) * M + c0;
     float4 v[PAIRS];
 #pragma unroll
     for (int i = 0; i < PAIRS; ++i) {
-      const int e = 2 * (tid + i * PTYK_NT);
+      const int e = 2 * (ln + i * 64);
       v[i] = *reinterpret_cast<const float4*>(base + (long)(e / CB) * M + (e % CB));
     }
 #pragma unroll
-    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[k * BLK + 2 * (tid + i * PTYK_NT)]) = v[i];
+    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[wk * BLK + 2 * (ln + i * 64)]) = v[i];
   }
   // amplitudes of this thread's pixels (frequency row R0 k1 + k2), in flight during the transform
+  constexpr int PIX = (BLK + BT - 1) / BT;
   const int f = gf[j];
   const float sc = amp_scale[f];
   float av[PIX];
 #pragma unroll
   for (int p = 0; p < PIX; ++p) {
-    const int idx = tid + p * PTYK_NT;
-    av[p] = __half2float(amp[(long)f * mm + (long)(C::R0 * (idx / CB) + k2) * M + c0 + idx % CB]) * sc;
+    const int idx = tid + p * BT;
+    av[p] = idx < BLK ? __half2float(amp[(long)f * mm + (long)(C::R0 * (idx / CB) + k2) * M + c0 + idx % CB]) * sc
+                      : 0.f;
   }
   __syncthreads();
   // all modes in one batch: each stage's barrier serves every mode
-  fft_lds<64, false, K * CB, PTYK_NT, ModeColLayout<CB>, true>(lds, tw, tid);
+  fft_lds<64, false, K * CB, BT, ModeColLayout<CB>, true>(lds, tw, tid);
   const float inv_mm = 1.f / (float)mm;
   double err = 0.0;
 #pragma unroll
   for (int p = 0; p < PIX; ++p) {
-    const int idx = tid + p * PTYK_NT;
+    const int idx = tid + p * BT;
+    if (idx >= BLK) break;
     float s2 = 0.f;
 #pragma unroll
     for (int k = 0; k < K; ++k) s2 += cabs2(lds[k * BLK + idx]);
@@ -671,18 +675,17 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_b(float2* __restrict__ psi
     for (int k = 0; k < K; ++k) lds[k * BLK + idx] = cscale(lds[k * BLK + idx], R);
   }
   __syncthreads();
-  fft_lds<64, true, K * CB, PTYK_NT, ModeColLayout<CB>, true>(lds, tw, tid);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    float2* base = psi + ((long)k * G + j) * mm + (long)(64 * k2) * M + c0;
+  fft_lds<64, true, K * CB, BT, ModeColLayout<CB>, true>(lds, tw, tid);
+  {
+    float2* base = psi + ((long)wk * G + j) * mm + (long)(64 * k2) * M + c0;
 #pragma unroll
     for (int i = 0; i < PAIRS; ++i) {
-      const int e = 2 * (tid + i * PTYK_NT);
+      const int e = 2 * (ln + i * 64);
       *reinterpret_cast<float4*>(base + (long)(e / CB) * M + (e % CB)) =
-          *reinterpret_cast<const float4*>(&lds[k * BLK + e]);
+          *reinterpret_cast<const float4*>(&lds[wk * BLK + e]);
     }
   }
-  err = block_sum<PTYK_NT>(err, red);
+  err = block_sum<BT>(err, red);
   if (tid == 0) part[(long)j * BPF + blk] = err;
 }
 
@@ -690,10 +693,10 @@ template <int M, int K>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
                                 double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
   using C = WaveCfg<M>;
-  constexpr int BPF = C::R0 * (M / PTYK_MLS_CB);
+  constexpr int BPF = C::R0 * (M / PTYK_MLS_MOD_CB);
   const long groups = (long)K * G * C::L;
   const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
-  const size_t lds = (size_t)K * 64 * PTYK_MLS_CB * sizeof(float2);
+  const size_t lds = (size_t)K * 64 * PTYK_MLS_MOD_CB * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
@@ -702,7 +705,7 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
   else
     hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
                        groups, tw);
-  hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(PTYK_NT), lds, s, psi, G, amp, amp_scale,
+  hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp, amp_scale,
                      gf, part, tw);
   hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, 0L, groups, tw);
   return hipGetLastError();
@@ -726,7 +729,7 @@ static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, co
 
 int mls_modulus_blocks(int m, int k) {
   if ((m != 128 && m != 256 && m != 512) || k < 1 || k > 8) return 0;
-  return (m / 64) * (m / PTYK_MLS_CB);
+  return (m / 64) * (m / PTYK_MLS_MOD_CB);
 }
 
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
