@@ -553,15 +553,39 @@ __global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const 
       __syncthreads();
       if ((int)threadIdx.x < cnt) lr[threadIdx.x] = roi[gf[base + threadIdx.x]];
       __syncthreads();
+      // frames whose ROI meets the tile (a block-uniform test), four at a time: their loads are
+      // issued before the in-order accumulation
+      constexpr int U = 4;
+      int pend[U];
+      int np = 0;
+      auto flush = [&]() {
+        float2 d[U];
+        bool in[U];
+#pragma unroll
+        for (int t = 0; t < U; ++t) {
+          in[t] = false;
+          if (t < np) {
+            const int2 o = lr[pend[t]];
+            const int lrw = r - o.x, lc = c - o.y;
+            in[t] = lrw >= 0 && lrw < m && lc >= 0 && lc < m;
+            if (in[t]) {
+              const int q = lrw * m + lc;
+              d[t] = obj_dir(chi, vp, pl, (long)(base + pend[t]) * mm + q, q);
+            }
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < U; ++t)
+          if (in[t]) u = cadd(u, d[t]);
+        np = 0;
+      };
       for (int jj = 0; jj < cnt; ++jj) {
         const int2 o = lr[jj];
         if (o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
-        const int lrw = r - o.x, lc = c - o.y;
-        if (lrw >= 0 && lrw < m && lc >= 0 && lc < m) {
-          const int q = lrw * m + lc;
-          u = cadd(u, obj_dir(chi, vp, pl, (long)(base + jj) * mm + q, q));
-        }
+        pend[np++] = jj;
+        if (np == U) flush();
       }
+      flush();
     }
   }
   if (r < rows && c < cols) {
