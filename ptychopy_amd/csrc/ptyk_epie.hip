@@ -341,10 +341,10 @@ __device__ __forceinline__ int at(int px) {
   return (px / M) * (M + 1) + (px % M);
 }
 
-// Object update at the rounded position (d_updateObjectArray): all loads of G pixels before
-// their stores (the stores could otherwise alias the next loads). Leaves conj(O_new) (psi -
-// psi_old), the probe update's increment before its scale, in place of psi in LDS, and returns
-// this thread's max of the new |O|^2 with its flat canvas index.
+// Object update at the rounded position (d_updateObjectArray) from psi - psi_old in LDS: all
+// loads of G pixels before their stores (the stores could otherwise alias the next loads).
+// Leaves conj(O_new) (psi - psi_old), the probe update's increment before its scale, in its
+// place, and returns this thread's max of the new |O|^2 with its flat canvas index.
 template <int M, int NT, int PPT, bool PC>
 __device__ __forceinline__ MaxAt epie_obj_update(const EpieArgs& a, int rx, int ry, float co, float2* lds) {
   constexpr int G = 4;
@@ -352,19 +352,18 @@ __device__ __forceinline__ MaxAt epie_obj_update(const EpieArgs& a, int rx, int 
 #pragma unroll
   for (int j0 = 0; j0 < PPT; j0 += G) {
     const int t = lane();
-    float2 ov[G], pv[G], sv[G];
+    float2 ov[G], pv[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int px = t + (j0 + u) * NT;
       ov[u] = a.obj[(long)(rx + px / M) * a.obj_cols + (ry + px % M)];
       pv[u] = a.probe[px];
-      sv[u] = a.psi_old[px];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int px = t + (j0 + u) * NT;
       const long oi = (long)(rx + px / M) * a.obj_cols + (ry + px % M);
-      const float2 d = csub(lds[at<M>(px)], sv[u]);
+      const float2 d = lds[at<M>(px)];  // psi - psi_old
       float2 o = cadd(ov[u], cscale(cmulcj(pv[u], d), co));
       const float mag = sqrtf(o.x * o.x + o.y * o.y);
       if (PC) o = clamp_phase(o);
@@ -401,10 +400,11 @@ __device__ __forceinline__ float epie_probe_update(const EpieArgs& a, float cp, 
 
 // Single-mode ePIE with the whole M x M exit wave resident in LDS (M (M + 1) complex values;
 // the object window of the bilinear ROI, (M + 2)^2, shares the space, 132 KiB at M = 128): both
-// 2D transforms and the modulus constraint run without a global round trip. Each thread owns
-// the pixels px = tid + j NT of the frame in every phase (so it reads back only its own part of
-// the psi_old scratch). Same update equations and order as k_epie. One CU moves every byte of a
-// position, so the phases are arranged to move few: the ROI gathers come from the staged
+// 2D transforms and the modulus constraint run without a global round trip, and psi_old is
+// never stored (the modulus step transforms the difference). Each thread owns the pixels
+// px = tid + j NT of the frame in every phase. Same update equations and order as k_epie, with
+// psi - psi_old = IFFT(Psi' - Psi / M^2) in place of IFFT(Psi') - psi_old. One CU moves every
+// byte of a position, so the phases are arranged to move few: the ROI gathers come from the staged
 // window, the probe update takes its increment from LDS, and the canvas max after an object
 // update is rescanned only when the previous maximum lay inside the updated ROI (otherwise it is
 // the larger of the previous maximum and the ROI's new maximum, exactly).
@@ -453,23 +453,32 @@ __global__ __launch_bounds__(NT) void k_epie_lds(EpieArgs a) {
       __syncthreads();
       const int t = lane();
 #pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        lds[at<M>(t + j * NT)] = po[j];
-        a.psi_old[t + j * NT] = po[j];
-      }
+      for (int j = 0; j < PPT; ++j) lds[at<M>(t + j * NT)] = po[j];
     }
     __syncthreads();
     fft_lds<M, false, M, NT, RowLayout<M, P>, true>(lds, tws, lane());
     fft_lds<M, false, M, NT, ColLayout<P>, true>(lds, tws, lane());
+    // The modulus step leaves Psi' - Psi / M^2 in place of Psi, so the inverse transform yields
+    // psi - psi_old directly (the transform is linear) and psi_old need not be kept. The optional
+    // Fourier error (PhaserUtil::calculateER, ePIE.cpp:124-125) is of psi_old's spectrum, which
+    // is Psi itself: it is summed here instead of by a third transform.
     {
       const int t = lane();
       __half ah[PPT];
 #pragma unroll
       for (int j = 0; j < PPT; ++j) ah[j] = amp[t + j * NT];
+      double acc = 0.0;
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
         const int q = at<M>(t + j * NT);
-        lds[q] = modulus_unit(lds[q], __half2float(ah[j]) * ascale, nrm);
+        const float2 f = lds[q];
+        const float A = __half2float(ah[j]) * ascale;
+        if (a.rms) acc += fabs((double)A - sqrt((double)f.x * f.x + (double)f.y * f.y));
+        lds[q] = csub(modulus_unit(f, A, nrm), cscale(f, nrm));
+      }
+      if (a.rms) {
+        acc = block_sum<NT>(acc, redd);
+        err += acc / (double)a.amp_sum[fi];
       }
     }
     __syncthreads();
@@ -514,24 +523,6 @@ __global__ __launch_bounds__(NT) void k_epie_lds(EpieArgs a) {
       probe_max = block_max<NT>(mx, redf);
     }
 
-    // Optional Fourier error (PhaserUtil::calculateER; ePIE.cpp:124-125) of psi_old
-    if (a.rms) {
-      __syncthreads();
-      const int t = lane();
-#pragma unroll
-      for (int j = 0; j < PPT; ++j) lds[at<M>(t + j * NT)] = a.psi_old[t + j * NT];
-      __syncthreads();
-      fft_lds<M, false, M, NT, RowLayout<M, P>, true>(lds, tws, lane());
-      fft_lds<M, false, M, NT, ColLayout<P>, true>(lds, tws, lane());
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        const float2 sm = lds[at<M>(t + j * NT)];
-        acc += fabs((double)load_amp(amp, t + j * NT, ascale) - sqrt((double)sm.x * sm.x + (double)sm.y * sm.y));
-      }
-      acc = block_sum<NT>(acc, redd);
-      err += acc / (double)a.amp_sum[fi];
-    }
     __syncthreads();
   }
   if (!a.update_probe && a.n_pos > 0) obj_max = canvas_max<NT>(a.obj_int, canvas, redf, redi).v;
