@@ -378,7 +378,8 @@ __device__ __forceinline__ MaxAt epie_obj_update(const EpieArgs& a, int rx, int 
 }
 
 // Probe update with the updated object (d_updateProbe), from the increment the object update
-// left in LDS; returns this thread's max |P|^2.
+// left in LDS; returns this thread's max |P|^2. The |P|^2 cache of the updated mode is written
+// once, after the last position (nothing reads it in between).
 template <int M, int NT, int PPT>
 __device__ __forceinline__ float epie_probe_update(const EpieArgs& a, float cp, const float2* lds) {
   const int t = lane();
@@ -392,7 +393,6 @@ __device__ __forceinline__ float epie_probe_update(const EpieArgs& a, float cp, 
     const float2 p = cadd(cscale(lds[at<M>(px)], cp), pv[j]);
     a.probe[px] = p;
     const float ia = sqrtf(p.x * p.x + p.y * p.y);
-    a.probe_int[px] = ia * ia;
     mx = fmaxf(mx, ia * ia);
   }
   return mx;
@@ -526,6 +526,16 @@ __global__ __launch_bounds__(NT) void k_epie_lds(EpieArgs a) {
     __syncthreads();
   }
   if (!a.update_probe && a.n_pos > 0) obj_max = canvas_max<NT>(a.obj_int, canvas, redf, redi).v;
+  if (a.update_probe && a.n_pos > 0) {
+    // each thread rereads only the probe pixels it wrote itself
+    const int t = lane();
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const float2 p = a.probe[t + j * NT];
+      const float ia = sqrtf(p.x * p.x + p.y * p.y);
+      a.probe_int[t + j * NT] = ia * ia;
+    }
+  }
   if (tid == 0) {
     a.scal[0] = obj_max;
     a.scal[1] = probe_max;

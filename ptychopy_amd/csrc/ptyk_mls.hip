@@ -655,17 +655,20 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
                                                     double c_probe, int solve, float2* __restrict__ beta_obj,
                                                     float2* __restrict__ beta_probe, double* __restrict__ ftot,
                                                     double* __restrict__ gscal) {
-  __shared__ double red[16];
+  // after the frame totals one wave does the rest: a group has few frames, and wave
+  // reductions need no block barrier
+  if (solve) frame_totals<1024>(part, ch, 6, G, ftot);
+  if (threadIdx.x >= 64) return;
+  const int ln = threadIdx.x;
   if (solve) {
-    frame_totals<1024>(part, ch, 6, G, ftot);
     double a1 = 0.0, a4 = 0.0;
-    for (int j = threadIdx.x; j < G; j += 1024) {
+    for (int j = ln; j < G; j += 64) {
       a1 += ftot[j * 6 + 0] + lam_mm;
       a4 += ftot[j * 6 + 3] + lam_mm;
     }
-    const double m1 = 0.1 * block_sum<1024>(a1, red) / G;
-    const double m4 = 0.1 * block_sum<1024>(a4, red) / G;
-    for (int j = threadIdx.x; j < G; j += 1024) {
+    const double m1 = 0.1 * wave_sum(a1) / G;
+    const double m4 = 0.1 * wave_sum(a4) / G;
+    for (int j = ln; j < G; j += 64) {
       const double* t = ftot + j * 6;
       const double A11 = t[0] + lam_mm + m1, A22 = t[3] + lam_mm + m4;
       const double a2r = t[1], a2i = t[2], b1 = t[4], b2 = t[5];
@@ -676,19 +679,19 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
       beta_obj[gf[j]] = c_of(c_obj * x0r, c_obj * x0i);
       beta_probe[gf[j]] = c_of(c_probe * x1r, c_probe * x1i);
     }
-    __syncthreads();
   }
   double sm[4] = {0, 0, 0, 0};
-  for (int j = threadIdx.x; j < G; j += 1024) {
+  for (int j = ln; j < G; j += 64) {  // each lane reads back only the steps it wrote
     const float2 bo = beta_obj[gf[j]], bp = beta_probe[gf[j]];
     sm[0] += bo.x;
     sm[1] += bo.y;
     sm[2] += bp.x;
     sm[3] += bp.y;
   }
+#pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const double t = block_sum<1024>(sm[v], red);
-    if (threadIdx.x == 0) gscal[MLS_S_BO + v] = t / G;
+    const double t = wave_sum(sm[v]);
+    if (ln == 0) gscal[MLS_S_BO + v] = t / G;
   }
 }
 
@@ -790,12 +793,13 @@ __global__ __launch_bounds__(NT) void k_vp_resid(float2* __restrict__ img, const
 __global__ __launch_bounds__(1024) void k_vp_coef(const double* __restrict__ part, int ch, const int* __restrict__ gf,
                                                   int G, int mm, const double* __restrict__ evo, int n,
                                                   double* __restrict__ ftot, double* __restrict__ gscal) {
-  __shared__ double red[16];
   frame_totals<1024>(part, ch, 1, G, ftot);
+  if (threadIdx.x >= 64) return;  // one wave does the rest (wave reductions, no block barrier)
+  const int ln = threadIdx.x;
   double e = 0.0;
-  for (int j = threadIdx.x; j < G; j += 1024) e += evo[n + gf[j]] * evo[n + gf[j]];
-  e = block_sum<1024>(e, red);
-  for (int j = threadIdx.x; j < G; j += 1024) gscal[MLS_S_COEF + j] = (ftot[j] / mm + evo[n + gf[j]]) / e;
+  for (int j = ln; j < G; j += 64) e += evo[n + gf[j]] * evo[n + gf[j]];
+  e = wave_sum(e);
+  for (int j = ln; j < G; j += 64) gscal[MLS_S_COEF + j] = (ftot[j] / mm + evo[n + gf[j]]) / e;
 }
 
 // acc = (1/M^2) sum_j c_j img_j; partial sums of |acc|^2 per block.
@@ -897,19 +901,19 @@ __global__ __launch_bounds__(1024) void k_evo_update(const double* __restrict__ 
                                                      const int* __restrict__ gf, int G, int mm,
                                                      double* __restrict__ evo, int n, int which,
                                                      double* __restrict__ ftot) {
-  __shared__ double red[16];
   const int nv = which == 2 ? 4 : 2;
   frame_totals<1024>(part, ch, nv, G, ftot);
+  if (threadIdx.x >= 64) return;  // one wave does the rest (wave reductions, no block barrier)
+  const int ln = threadIdx.x;
   if (which >= 1) {
     double dm = 0.0;
-    for (int j = threadIdx.x; j < G; j += 1024) dm += ftot[nv * j] / mm;
-    const double m3 = block_sum<1024>(dm, red) / G;
-    for (int j = threadIdx.x; j < G; j += 1024)
-      evo[n + gf[j]] += (ftot[nv * j + 1] / mm) / (ftot[nv * j] / mm + 0.1 * m3);
+    for (int j = ln; j < G; j += 64) dm += ftot[nv * j] / mm;
+    const double m3 = wave_sum(dm) / G;
+    for (int j = ln; j < G; j += 64) evo[n + gf[j]] += (ftot[nv * j + 1] / mm) / (ftot[nv * j] / mm + 0.1 * m3);
   }
   if (which != 1) {
     const int o = which == 2 ? 2 : 0;
-    for (int j = threadIdx.x; j < G; j += 1024) evo[gf[j]] += 0.1 * ftot[nv * j + o + 1] / ftot[nv * j + o];
+    for (int j = ln; j < G; j += 64) evo[gf[j]] += 0.1 * ftot[nv * j + o + 1] / ftot[nv * j + o];
   }
 }
 
