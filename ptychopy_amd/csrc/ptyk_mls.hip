@@ -483,43 +483,34 @@ __global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, 
   pum[q] = cscale(acc, (float)(1.0 / G));
 }
 
-// Probe updates of modes 2..K in one pass (MLS.cpp:251-279 for ll >= 2, no sub-pixel shift):
-// pum_ll = mean_j w_j chi_ll,j conj(op_j), P_ll += pum_ll * mean probe step.  Nothing else in the
-// group reads these modes between the reference's per-mode passes, so they fuse exactly.
-template <int KMAX>
+// Probe updates of modes 2..K in one launch (MLS.cpp:251-279 for ll >= 2, no sub-pixel shift):
+// pum_ll = mean_j w_j chi_ll,j conj(op_j), P_ll += pum_ll * mean probe step, one thread per
+// (pixel, mode) with the frames summed in group order.  Nothing else in the group reads these
+// modes between the reference's per-mode passes, so they fuse exactly.
 __global__ void k_pum_modes(float2* __restrict__ probe, const float2* __restrict__ psi, const float2* __restrict__ op,
                             const float* __restrict__ illum, const int2* __restrict__ roi, const int* __restrict__ gf,
-                            int G, int K, int m, int cols, float fw, const double* __restrict__ gscal) {
+                            int G, int m, int cols, float fw, const double* __restrict__ gscal) {
   const int mm = m * m;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= mm) return;
+  const int k = blockIdx.y + 1;
   const int r = q / m, c = q % m;
-  float2 acc[KMAX];
-#pragma unroll
-  for (int k = 0; k < KMAX; ++k) acc[k] = make_float2(0.f, 0.f);
-  const long gm = (long)G * mm;
+  float2 acc = make_float2(0.f, 0.f);
+  const float2* chi = psi + (long)k * G * mm;
   for (int j = 0; j < G; ++j) {
     const int2 o = roi[gf[j]];
     const float p = illum[(long)(o.x + r) * cols + o.y + c];
     const float w = p / (p + fw);
     const float2 b = op[(long)j * mm + q];
-#pragma unroll
-    for (int k = 1; k < KMAX; ++k) {
-      if (k >= K) break;
-      const float2 u = psi[k * gm + (long)j * mm + q];
-      const float2 d = make_float2(u.x * b.x + u.y * b.y, u.y * b.x - u.x * b.y);
-      acc[k].x += w * d.x;
-      acc[k].y += w * d.y;
-    }
+    const float2 u = chi[(long)j * mm + q];
+    const float2 d = make_float2(u.x * b.x + u.y * b.y, u.y * b.x - u.x * b.y);
+    acc.x += w * d.x;
+    acc.y += w * d.y;
   }
   const float inv = (float)(1.0 / G);
   const float2 bp = c_of(gscal[MLS_S_BP], gscal[MLS_S_BP + 1]);
-#pragma unroll
-  for (int k = 1; k < KMAX; ++k) {
-    if (k >= K) break;
-    const long i = (long)k * mm + q;
-    probe[i] = cadd(probe[i], cmul(cscale(acc[k], inv), bp));
-  }
+  const long i = (long)k * mm + q;
+  probe[i] = cadd(probe[i], cmul(cscale(acc, inv), bp));
 }
 
 // Object update direction of frame j at local pixel q: chi conj(vp_j) for the first probe mode,
@@ -1252,12 +1243,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       }
     }
     if (fuse_modes) {
-      if (K <= 4) LAUNCH(k_pum_modes<4>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m, d.cols,
-                         fw, d.gscal);
-      else if (K <= 8) LAUNCH(k_pum_modes<8>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m,
-                              d.cols, fw, d.gscal);
-      else LAUNCH(k_pum_modes<16>, g1(mm), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, K, m, d.cols, fw,
-                  d.gscal);
+      LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, m,
+             d.cols, fw, d.gscal);
       CHK();
     }
   }
