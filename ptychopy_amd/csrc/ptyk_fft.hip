@@ -18,10 +18,13 @@ struct FftCfg {
 };
 
 // Both passes move two adjacent complex values (16 B) per lane in fully unrolled loops, so every
-// load of a thread is in flight before its first LDS store.
-template <int M, bool INV>
+// load of a thread is in flight before its first LDS store.  PAIR: two frames read from a and b
+// (rows 0..M-1 and M..2M-1) and written to data.
+template <int M, bool INV, bool PAIR = false>
 __global__ __launch_bounds__(PTYK_NT) void k_fft_rows(float2* __restrict__ data, long total_rows,
-                                                      const float2* __restrict__ tw) {
+                                                      const float2* __restrict__ tw,
+                                                      const float2* __restrict__ a = nullptr,
+                                                      const float2* __restrict__ b = nullptr) {
   constexpr int NB = FftCfg<M>::RNB;
   constexpr int PAIRS = NB * M / 2 / PTYK_NT;
   static_assert((NB * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
@@ -34,7 +37,8 @@ __global__ __launch_bounds__(PTYK_NT) void k_fft_rows(float2* __restrict__ data,
     for (int i = 0; i < PAIRS; ++i) {
       const int e = 2 * (tid + i * PTYK_NT);
       const long g = row0 + e / M;
-      v[i] = g < total_rows ? *reinterpret_cast<const float4*>(data + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float2* src = PAIR ? (g < M ? a + g * M : b + (g - M) * M) : data + g * M;
+      v[i] = g < total_rows ? *reinterpret_cast<const float4*>(src + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int i = 0; i < PAIRS; ++i) {
@@ -115,6 +119,28 @@ static hipError_t fft_cols_impl(float2* data, long n, const float2* tw, hipStrea
   if (gc > 0x7fffffffL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_fft_cols<M, INV>), dim3((unsigned)gc), dim3(PTYK_NT), 0, s, data, strips, tw);
   return hipGetLastError();
+}
+
+template <int M>
+static hipError_t fft2_pair_impl(float2* dst, const float2* a, const float2* b, const float2* tw, hipStream_t s) {
+  const long rows = 2L * M, strips = 2L * (M / 16);
+  const long gr = (rows + FftCfg<M>::RNB - 1) / FftCfg<M>::RNB;
+  const long gc = (strips + FftCfg<M>::CNB / 16 - 1) / (FftCfg<M>::CNB / 16);
+  hipLaunchKernelGGL((k_fft_rows<M, false, true>), dim3((unsigned)gr), dim3(PTYK_NT), 0, s, dst, rows, tw, a, b);
+  hipLaunchKernelGGL((k_fft_cols<M, false>), dim3((unsigned)gc), dim3(PTYK_NT), 0, s, dst, strips, tw);
+  return hipGetLastError();
+}
+
+hipError_t fft2_pair(float2* dst, const float2* a, const float2* b, int m, const float2* tw, hipStream_t s) {
+  switch (m) {
+    case 16: return fft2_pair_impl<16>(dst, a, b, tw, s);
+    case 32: return fft2_pair_impl<32>(dst, a, b, tw, s);
+    case 64: return fft2_pair_impl<64>(dst, a, b, tw, s);
+    case 128: return fft2_pair_impl<128>(dst, a, b, tw, s);
+    case 256: return fft2_pair_impl<256>(dst, a, b, tw, s);
+    case 512: return fft2_pair_impl<512>(dst, a, b, tw, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t fft_cols_batch(float2* data, long n, int m, bool inv, const float2* tw, hipStream_t s) {

@@ -42,17 +42,47 @@ __device__ __forceinline__ float freq(int i, int m) { return (float)((i + (m >> 
 __device__ __forceinline__ float2 c_of(double re, double im) { return make_float2((float)re, (float)im); }
 
 // Per-frame totals of chunk partials: one wave per frame, lanes over chunks, fixed order.
-// tot[j * nv + v] = sum_c part[(j * ch + c) * nv + v].  Ends with a block barrier.
-template <int BT>
-__device__ void frame_totals(const double* __restrict__ part, int ch, int nv, int G, double* __restrict__ tot) {
+// tot[j * NV + v] = sum_c part[(j * ch + c) * NV + v].  A wave takes its two frames j and
+// j + BT/64 together, every load issued before the first sum.  Ends with a block barrier.
+template <int BT, int NV>
+__device__ void frame_totals(const double* __restrict__ part, int ch, int G, double* __restrict__ tot) {
+  constexpr int W = BT / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = w; j < G; j += BT / 64)
-    for (int v = 0; v < nv; ++v) {
-      double s = 0.0;
-      for (int c = lane; c < ch; c += 64) s += part[((long)j * ch + c) * nv + v];
-      s = wave_sum(s);
-      if (lane == 0) tot[(long)j * nv + v] = s;
+  for (int j0 = w; j0 < G; j0 += 2 * W) {
+    const int j1 = j0 + W;
+    const bool two = j1 < G;  // wave-uniform
+    double s0[NV], s1[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s0[v] = s1[v] = 0.0;
+    for (int c = lane; c < ch; c += 64) {
+      const double* p0 = part + ((long)j0 * ch + c) * NV;
+      const double* p1 = part + ((long)j1 * ch + c) * NV;
+      double x0[NV], x1[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x0[v] = p0[v];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x1[v] = two ? p1[v] : 0.0;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        s0[v] += x0[v];
+        s1[v] += x1[v];
+      }
     }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s0[v] = wave_sum(s0[v]);
+    if (two) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s1[v] = wave_sum(s1[v]);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) tot[(long)j0 * NV + v] = s0[v];
+      if (two) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tot[(long)j1 * NV + v] = s1[v];
+      }
+    }
+  }
   __syncthreads();
 }
 
@@ -82,15 +112,6 @@ __global__ void k_scale_c(float2* __restrict__ x, long count, const double* __re
   if (i >= count) return;
   const float f = invert ? (float)(1.0 / *s) : (float)(*s);
   x[i] = cscale(x[i], f);
-}
-
-__global__ void k_copy2(float2* __restrict__ dst, const float2* __restrict__ a, const float2* __restrict__ b,
-                        long count) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) {
-    dst[i] = a[i];
-    dst[count + i] = b[i];
-  }
 }
 
 __global__ void k_fill_c(float2* __restrict__ x, long count, float2 v) {
@@ -527,17 +548,28 @@ __device__ __forceinline__ float2 obj_dir(const float2* __restrict__ chi, const 
 
 // object_upd_sum = (0, eps) + set_projections(direction), preconditioned by
 // 1 / (p + delta_p MAX_ILLUM) (d_object_sum_update_Gfun, utilitiesKernels.cu:181-189).
-// Gather by 4 x 64 canvas tile over the group's frames in group order.
+// Gather by canvas tile of (4 RPT) x 64 pixels, RPT rows per thread, over the group's frames
+// in group order.
+constexpr int PRE_RPT = 4;
+template <int RPT>
 __global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const float* __restrict__ illum,
                                                 const float2* __restrict__ chi, const float2* __restrict__ vp,
                                                 const float2* __restrict__ pl, const int2* __restrict__ roi,
                                                 const int* __restrict__ gf, int G, int m, int rows, int cols, float fd,
                                                 int scatter) {
+  constexpr int TH = 4 * RPT;
   __shared__ int2 lr[NT];
   const int mm = m * m;
-  const int r0 = blockIdx.y * 4, c0 = blockIdx.x * 64;
-  const int r = r0 + (threadIdx.x >> 6), c = c0 + (threadIdx.x & 63);
-  float2 u = make_float2(0.f, kEpsF);
+  const int r0 = blockIdx.y * TH, c0 = blockIdx.x * 64;
+  const int ty = threadIdx.x >> 6, c = c0 + (threadIdx.x & 63);
+  float2 u[RPT];
+  float il[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    u[k] = make_float2(0.f, kEpsF);
+    const int r = r0 + ty + 4 * k;
+    il[k] = scatter && r < rows && c < cols ? illum[(long)r * cols + c] : 0.f;
+  }
   if (scatter) {
     for (int base = 0; base < G; base += NT) {
       const int cnt = min(NT, G - base);
@@ -550,54 +582,51 @@ __global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const 
       int pend[U];
       int np = 0;
       auto flush = [&]() {
-        float2 d[U];
-        bool in[U];
+        float2 d[U][RPT];
+        bool in[U][RPT];
 #pragma unroll
         for (int t = 0; t < U; ++t) {
-          in[t] = false;
-          if (t < np) {
-            const int2 o = lr[pend[t]];
-            const int lrw = r - o.x, lc = c - o.y;
-            in[t] = lrw >= 0 && lrw < m && lc >= 0 && lc < m;
-            if (in[t]) {
+          const int2 o = lr[pend[t < np ? t : 0]];
+          const int lc = c - o.y;
+          const bool col_in = t < np && lc >= 0 && lc < m;
+#pragma unroll
+          for (int k = 0; k < RPT; ++k) {
+            const int lrw = r0 + ty + 4 * k - o.x;
+            in[t][k] = col_in && lrw >= 0 && lrw < m;
+            if (in[t][k]) {
               const int q = lrw * m + lc;
-              d[t] = obj_dir(chi, vp, pl, (long)(base + pend[t]) * mm + q, q);
+              d[t][k] = obj_dir(chi, vp, pl, (long)(base + pend[t]) * mm + q, q);
             }
           }
         }
 #pragma unroll
         for (int t = 0; t < U; ++t)
-          if (in[t]) u = cadd(u, d[t]);
+#pragma unroll
+          for (int k = 0; k < RPT; ++k)
+            if (in[t][k]) u[k] = cadd(u[k], d[t][k]);
         np = 0;
       };
       for (int jj = 0; jj < cnt; ++jj) {
         const int2 o = lr[jj];
-        if (o.x > r0 + 3 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
+        if (o.x > r0 + TH - 1 || o.x + m <= r0 || o.y > c0 + 63 || o.y + m <= c0) continue;  // block-uniform
         pend[np++] = jj;
         if (np == U) flush();
       }
-      flush();
+      if (np > 0) flush();
     }
   }
-  if (r < rows && c < cols) {
-    const long i = (long)r * cols + c;
-    if (scatter) {
-      const float d = illum[i] + fd;
-      u = make_float2(u.x / d, u.y / d);
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    const int r = r0 + ty + 4 * k;
+    if (r < rows && c < cols) {
+      float2 v = u[k];
+      if (scatter) {
+        const float dn = il[k] + fd;
+        v = make_float2(v.x / dn, v.y / dn);
+      }
+      pre[(long)r * cols + c] = v;
     }
-    pre[i] = u;
   }
-}
-
-// out_j = canvas at ROI j (get_projections).
-__global__ void k_gather(float2* __restrict__ out, const float2* __restrict__ canvas, const int2* __restrict__ roi,
-                         const int* __restrict__ gf, int m, int cols) {
-  const int mm = m * m;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= mm) return;
-  const int j = blockIdx.y;
-  const int2 o = roi[gf[j]];
-  out[(long)j * mm + q] = canvas[(long)(o.x + q / m) * cols + o.y + q % m];
 }
 
 __global__ void k_dir(float2* __restrict__ out, const float2* __restrict__ chi, const float2* __restrict__ vp,
@@ -611,19 +640,25 @@ __global__ void k_dir(float2* __restrict__ out, const float2* __restrict__ chi, 
 
 // LSQ sums per frame chunk (d_get_optimal_step_lsq, utilitiesKernels.cu:445-466), FP64:
 // [|dOP|^2, Re dOP conj dPO, Im dOP conj dPO, |dPO|^2, Re conj(dOP) chi, Re conj(dPO) chi].
-__global__ __launch_bounds__(NT) void k_lsq_partial(const float2* __restrict__ oup, const float2* __restrict__ vp,
+// The object direction dO is read from the canvas at the frame's ROI when canvas is given
+// (get_projections of the preconditioned update), else from oup frame by frame.
+__global__ __launch_bounds__(NT) void k_lsq_partial(const float2* __restrict__ oup, const float2* __restrict__ canvas,
+                                                    const int2* __restrict__ roi, const int* __restrict__ gf,
+                                                    int cols, const float2* __restrict__ vp,
                                                     const float2* __restrict__ pum, const float2* __restrict__ op,
                                                     const float2* __restrict__ chi, int m, double* __restrict__ part) {
   __shared__ double red[NT / 64];
   const int mm = m * m, ppb = ppb_of(m);
   const int j = blockIdx.y;
+  const int2 o = canvas ? roi[gf[j]] : make_int2(0, 0);
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < ppb; t += NT) {
     const int q = blockIdx.x * ppb + t;
     const long i = (long)j * mm + q;
-    const float2 a = oup[i], b = vp[i], p = pum[q], o = op[i], x = chi[i];
+    const float2 a = canvas ? canvas[(long)(o.x + q / m) * cols + o.y + q % m] : oup[i];
+    const float2 b = vp[i], p = pum[q], o2 = op[i], x = chi[i];
     const double opr = (double)a.x * b.x - (double)a.y * b.y, opi = (double)a.x * b.y + (double)a.y * b.x;
-    const double por = (double)p.x * o.x - (double)p.y * o.y, poi = (double)p.x * o.y + (double)p.y * o.x;
+    const double por = (double)p.x * o2.x - (double)p.y * o2.y, poi = (double)p.x * o2.y + (double)p.y * o2.x;
     s[0] += opr * opr + opi * opi;
     s[1] += opr * por + opi * poi;
     s[2] += opi * por - opr * poi;
@@ -648,9 +683,10 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
                                                     double* __restrict__ gscal) {
   // after the frame totals one wave does the rest: a group has few frames, and wave
   // reductions need no block barrier
-  if (solve) frame_totals<1024>(part, ch, 6, G, ftot);
+  if (solve) frame_totals<1024, 6>(part, ch, G, ftot);
   if (threadIdx.x >= 64) return;
   const int ln = threadIdx.x;
+  double sm[4] = {0, 0, 0, 0};
   if (solve) {
     double a1 = 0.0, a4 = 0.0;
     for (int j = ln; j < G; j += 64) {
@@ -667,17 +703,22 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
       const double det = A11 * A22 - (a2r * a2r + a2i * a2i);
       const double x0r = (b1 * A22 - a2r * b2) / det, x0i = (-a2i * b2) / det;
       const double x1r = (A11 * b2 - a2r * b1) / det, x1i = (a2i * b1) / det;
-      beta_obj[gf[j]] = c_of(c_obj * x0r, c_obj * x0i);
-      beta_probe[gf[j]] = c_of(c_probe * x1r, c_probe * x1i);
+      const float2 bo = c_of(c_obj * x0r, c_obj * x0i), bp = c_of(c_probe * x1r, c_probe * x1i);
+      beta_obj[gf[j]] = bo;
+      beta_probe[gf[j]] = bp;
+      sm[0] += bo.x;  // the means take the stored (float) steps
+      sm[1] += bo.y;
+      sm[2] += bp.x;
+      sm[3] += bp.y;
     }
-  }
-  double sm[4] = {0, 0, 0, 0};
-  for (int j = ln; j < G; j += 64) {  // each lane reads back only the steps it wrote
-    const float2 bo = beta_obj[gf[j]], bp = beta_probe[gf[j]];
-    sm[0] += bo.x;
-    sm[1] += bo.y;
-    sm[2] += bp.x;
-    sm[3] += bp.y;
+  } else {
+    for (int j = ln; j < G; j += 64) {
+      const float2 bo = beta_obj[gf[j]], bp = beta_probe[gf[j]];
+      sm[0] += bo.x;
+      sm[1] += bo.y;
+      sm[2] += bp.x;
+      sm[3] += bp.y;
+    }
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -713,6 +754,72 @@ __global__ void k_deriv(float2* __restrict__ x, float2* __restrict__ y, int m) {
   y[i] = cscale(t, freq(q / m, m) * inv);
 }
 
+// k_deriv fused into the inverse row pass: each workgroup loads NB/2 spectrum rows, forms both
+// derivative spectra of each row in LDS (NB rows) and transforms them together; x rows go back
+// to x, y rows to y.  The column pass follows as a plain inverse column transform.  Same float
+// operations as k_deriv followed by the row transform.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_deriv_rows(float2* __restrict__ x, float2* __restrict__ y, long rows,
+                                                        const float2* __restrict__ tw) {
+  constexpr int NB = plan_min_batch<M, PTYK_NT>();
+  constexpr int NI = NB / 2;
+  constexpr int PAIRS = NI * M / 2 / PTYK_NT;
+  static_assert(NB % 2 == 0 && (NI * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[NB * M];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * NI;
+  const float inv = 1.f / (float)(M * M);
+  float4 v[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const long g = row0 + e / M;
+    v[i] = g < rows ? *reinterpret_cast<const float4*>(x + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const int rr = e / M, c = e % M;
+    const int r = (int)((row0 + rr) % M);
+    const float2 t0 = make_float2(-v[i].y * kTwoPi, v[i].x * kTwoPi);
+    const float2 t1 = make_float2(-v[i].w * kTwoPi, v[i].z * kTwoPi);
+    const float fc0 = freq(c, M) * inv, fc1 = freq(c + 1, M) * inv, fr = freq(r, M) * inv;
+    const float2 x0 = cscale(t0, fc0), x1 = cscale(t1, fc1);
+    const float2 y0 = cscale(t0, fr), y1 = cscale(t1, fr);
+    *reinterpret_cast<float4*>(&lds[L::addr(rr, c)]) = make_float4(x0.x, x0.y, x1.x, x1.y);
+    *reinterpret_cast<float4*>(&lds[L::addr(NI + rr, c)]) = make_float4(y0.x, y0.y, y1.x, y1.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NB, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const int rr = e / M, c = e % M;
+    const long g = row0 + rr;
+    if (g < rows) {
+      *reinterpret_cast<float4*>(x + g * M + c) = *reinterpret_cast<const float4*>(&lds[L::addr(rr, c)]);
+      *reinterpret_cast<float4*>(y + g * M + c) = *reinterpret_cast<const float4*>(&lds[L::addr(NI + rr, c)]);
+    }
+  }
+}
+
+// Derivative spectra of G frames of x (in place) and into y, inverse-transformed along rows;
+// 0 when m has no fused form (the caller then runs k_deriv and the full transform).
+static int deriv_rows(float2* x, float2* y, int G, int m, const float2* tw, hipStream_t s) {
+  auto go = [&](auto kern, int ni) {
+    const long rows = (long)G * m;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((rows + ni - 1) / ni)), dim3(PTYK_NT), 0, s, x, y, rows, tw);
+    return 1;
+  };
+  switch (m) {
+    case 128: return go(k_deriv_rows<128>, plan_min_batch<128, PTYK_NT>() / 2);
+    case 256: return go(k_deriv_rows<256>, plan_min_batch<256, PTYK_NT>() / 2);
+    case 512: return go(k_deriv_rows<512>, plan_min_batch<512, PTYK_NT>() / 2);
+    default: return 0;
+  }
+}
+
 // [sum |dX vp|^2, sum Re(chi conj(dX vp)), same for dY] per frame chunk.
 __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ dx, const float2* __restrict__ dy,
                                                     const float2* __restrict__ vp, const float2* __restrict__ chi, int m,
@@ -740,18 +847,19 @@ __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ d
 // Step clipped to 0.2 px per axis; the column-derivative step is added to the row
 // coordinate and the row-derivative step to the column coordinate, as the reference does
 // (probe_positions[i].x += shift from dX, Probe.cpp:1022-1033).
-__global__ void k_pos_update(const double* __restrict__ part, int ch, const int* __restrict__ gf, int G,
-                             float2* __restrict__ pos) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= G) return;
-  double s[4] = {0, 0, 0, 0};
-  for (int c = 0; c < ch; ++c)
-    for (int v = 0; v < 4; ++v) s[v] += part[((long)j * ch + c) * 4 + v];
-  const double dx = s[1] / s[0], dy = s[3] / s[2];
-  const double sx = fmin(fabs(dx), 0.2) * (double)((0.0 < dx) - (dx < 0.0));
-  const double sy = fmin(fabs(dy), 0.2) * (double)((0.0 < dy) - (dy < 0.0));
-  const int f = gf[j];
-  pos[f] = make_float2((float)((double)pos[f].x + sx), (float)((double)pos[f].y + sy));
+__global__ __launch_bounds__(1024) void k_pos_update(const double* __restrict__ part, int ch,
+                                                     const int* __restrict__ gf, int G, float2* __restrict__ pos,
+                                                     double* __restrict__ ftot) {
+  frame_totals<1024, 4>(part, ch, G, ftot);
+  if (threadIdx.x >= 64) return;  // one wave applies the steps
+  for (int j = threadIdx.x; j < G; j += 64) {
+    const double* s = ftot + (long)j * 4;
+    const double dx = s[1] / s[0], dy = s[3] / s[2];
+    const double sx = fmin(fabs(dx), 0.2) * (double)((0.0 < dx) - (dx < 0.0));
+    const double sy = fmin(fabs(dy), 0.2) * (double)((0.0 < dy) - (dy < 0.0));
+    const int f = gf[j];
+    pos[f] = make_float2((float)((double)pos[f].x + sx), (float)((double)pos[f].y + sy));
+  }
 }
 
 // Variable probe (Probe::update_variable_probe, Probe.cpp:1366-1464).
@@ -784,7 +892,7 @@ __global__ __launch_bounds__(NT) void k_vp_resid(float2* __restrict__ img, const
 __global__ __launch_bounds__(1024) void k_vp_coef(const double* __restrict__ part, int ch, const int* __restrict__ gf,
                                                   int G, int mm, const double* __restrict__ evo, int n,
                                                   double* __restrict__ ftot, double* __restrict__ gscal) {
-  frame_totals<1024>(part, ch, 1, G, ftot);
+  frame_totals<1024, 1>(part, ch, G, ftot);
   if (threadIdx.x >= 64) return;  // one wave does the rest (wave reductions, no block barrier)
   const int ln = threadIdx.x;
   double e = 0.0;
@@ -893,7 +1001,8 @@ __global__ __launch_bounds__(1024) void k_evo_update(const double* __restrict__ 
                                                      double* __restrict__ evo, int n, int which,
                                                      double* __restrict__ ftot) {
   const int nv = which == 2 ? 4 : 2;
-  frame_totals<1024>(part, ch, nv, G, ftot);
+  if (nv == 4) frame_totals<1024, 4>(part, ch, G, ftot);
+  else frame_totals<1024, 2>(part, ch, G, ftot);
   if (threadIdx.x >= 64) return;  // one wave does the rest (wave reductions, no block barrier)
   const int ln = threadIdx.x;
   if (which >= 1) {
@@ -1070,6 +1179,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   const int m = d.m, mm = m * m, K = d.k, ch = mls_chunks(m);
   const long canvas = (long)d.rows * d.cols;
   const dim3 tiles((d.cols + 63) / 64, (d.rows + 3) / 4);
+  const dim3 pre_tiles((d.cols + 63) / 64, (d.rows + 4 * PRE_RPT - 1) / (4 * PRE_RPT));
   const bool var_any = d.variable_probe || d.variable_intensity;
   const int subpix = d.apply_subpix_shift ? 1 : 0;
 
@@ -1112,11 +1222,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
     const bool pos_step = iter >= d.pps;
-    if (subpix) {
-      LAUNCH(k_copy2, g1(mm), dim3(NT), d.spec, d.probe, d.vmode, (long)mm);
-      CHK();
-      TRY(fft2_batch(d.spec, 2, m, false, d.tw, s));
-    }
+    if (subpix) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     const int mod_blocks = mls_modulus_blocks(m, K);
     if (subpix && mod_blocks) {
       // the shifted illumination probes' spectra formed in the inverse row pass itself
@@ -1183,16 +1289,17 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       const float2* pl = llp == 1 ? nullptr : d.probe + (long)(llp - 1) * mm;
       if (obj_rec) {
         const int scatter = d.delta_p != 0 ? 1 : 0;
-        LAUNCH(k_precond, tiles, dim3(NT), d.precond, d.illum, chi, d.vp, pl, d.roi, gf, G, m, d.rows, d.cols, fd,
+        LAUNCH(k_precond<PRE_RPT>, pre_tiles, dim3(NT), d.precond, d.illum, chi, d.vp, pl, d.roi, gf, G, m, d.rows, d.cols, fd,
                scatter);
-        CHK();
-        if (scatter && d.beta_lsq > 0) LAUNCH(k_gather, gpix, dim3(NT), d.tmp2, d.precond, d.roi, gf, m, d.cols);
-        else LAUNCH(k_dir, gpix, dim3(NT), d.tmp2, chi, d.vp, pl, m);
         CHK();
       }
       const bool lsq = d.beta_lsq > 0 && ll == 1 && obj_rec && probe_rec;
       if (lsq) {
-        LAUNCH(k_lsq_partial, gch, dim3(NT), d.tmp2, d.vp, d.pum, d.op, chi0, m, d.part);
+        // the object direction: the preconditioned update at each ROI, or chi conj(vp) unscaled
+        const bool scatter = d.delta_p != 0;
+        if (!scatter) LAUNCH(k_dir, gpix, dim3(NT), d.tmp2, chi, d.vp, pl, m);
+        LAUNCH(k_lsq_partial, gch, dim3(NT), d.tmp2, scatter ? d.precond : nullptr, d.roi, gf, d.cols, d.vp, d.pum,
+               d.op, chi0, m, d.part);
         CHK();
       }
       const double lam_mm = (double)(float)(1.1920928955078125e-07 / mm) * mm;
@@ -1209,12 +1316,19 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       }
       CHK();
       if (pos_step && ll == 1) {
-        LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);  // tmp already holds its spectrum
-        CHK();
-        // tmp and tmp2 in one inverse transform (tmp2 follows tmp's max(gmax, K) frames)
-        TRY(fft2_batch(d.tmp, std::max<long>(d.gmax, K) + G, m, true, d.tw, s));
+        // tmp already holds its spectrum; tmp and tmp2 take one inverse column transform (tmp2
+        // follows tmp's max(gmax, K) frames)
+        const long span = std::max<long>(d.gmax, K) + G;
+        if (deriv_rows(d.tmp, d.tmp2, G, m, d.tw, s)) {
+          CHK();
+          TRY(fft_cols_batch(d.tmp, span, m, true, d.tw, s));
+        } else {
+          LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
+          CHK();
+          TRY(fft2_batch(d.tmp, span, m, true, d.tw, s));
+        }
         LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
-        LAUNCH(k_pos_update, g1(G), dim3(NT), d.part, ch, gf, G, d.pos);
+        LAUNCH(k_pos_update, dim3(1), dim3(1024), d.part, ch, gf, G, d.pos, d.ftot);
         CHK();
       }
       if (var_update && ll == 1) {

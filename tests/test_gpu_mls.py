@@ -76,6 +76,9 @@ CONFIGS = [
     dict(k=1, m=128, n_side=3, cfg=dict(probe_pos_search=2, variable_probe=0)),
     dict(k=3, m=128, n_side=3, cfg=dict(probe_pos_search=3)),
     dict(k=5, m=256, n_side=3, cfg=dict(probe_pos_search=99)),
+    # position refinement at 256^2 and 512^2 (derivative spectra formed in the inverse row pass)
+    dict(k=2, m=256, n_side=3, cfg=dict(probe_pos_search=2)),
+    dict(k=1, m=512, n_side=3, cfg=dict(probe_pos_search=2, variable_probe=0)),
 ]
 
 
