@@ -844,6 +844,92 @@ __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ d
   }
 }
 
+// The inverse column transform of both derivative images fused with k_pos_partial: a workgroup
+// takes the same NC columns (whole 16-column strips) of frame j from dx and from dy, transforms
+// them in LDS, and sums the four products with vp and chi over its columns; the derivative
+// images are never written back.  Partials: part[(j * gridDim.x + blockIdx.x) * 4 + v].
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__ dx, const float2* __restrict__ dy,
+                                                      const float2* __restrict__ vp, const float2* __restrict__ chi,
+                                                      const float2* __restrict__ tw, double* __restrict__ part) {
+  constexpr int NC = (plan_min_batch<M, PTYK_NT>() + 15) / 16 * 16;
+  constexpr int PAIRS = NC * M / 2 / PTYK_NT;
+  static_assert((NC * M / 2) % PTYK_NT == 0, "column tile must divide among the threads");
+  __shared__ float2 lds[2 * NC * M];  // dx columns, then dy columns
+  float2* lds_y = lds + NC * M;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y, c0 = blockIdx.x * NC;
+  const long fbase = (long)j * M * M + c0;
+  auto gaddr = [&](int e) {  // e: strip-major element index of the tile (even)
+    const int g = e / (M * 16), i = (e >> 4) % M, c = e & 15;
+    return fbase + (long)i * M + g * 16 + c;
+  };
+  {
+    float4 a[PAIRS], b[PAIRS];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const long q = gaddr(2 * (tid + k * PTYK_NT));
+      a[k] = *reinterpret_cast<const float4*>(dx + q);
+      b[k] = *reinterpret_cast<const float4*>(dy + q);
+    }
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      *reinterpret_cast<float4*>(&lds[2 * (tid + k * PTYK_NT)]) = a[k];
+      *reinterpret_cast<float4*>(&lds_y[2 * (tid + k * PTYK_NT)]) = b[k];
+    }
+  }
+  // vp and chi of the same pixels, in flight during the transforms
+  float4 v[PAIRS], x[PAIRS];
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const long q = gaddr(2 * (tid + k * PTYK_NT));
+    v[k] = *reinterpret_cast<const float4*>(vp + q);
+    x[k] = *reinterpret_cast<const float4*>(chi + q);
+  }
+  __syncthreads();
+  fft_lds<M, true, NC, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
+  fft_lds<M, true, NC, PTYK_NT, StripLayout<M>, true>(lds_y, tw, tid);
+  double s[4] = {0, 0, 0, 0};
+  auto acc = [&](float2 a, float2 b, float2 vv, float2 xx) {
+    const double axr = (double)a.x * vv.x - (double)a.y * vv.y, axi = (double)a.x * vv.y + (double)a.y * vv.x;
+    const double ayr = (double)b.x * vv.x - (double)b.y * vv.y, ayi = (double)b.x * vv.y + (double)b.y * vv.x;
+    s[0] += axr * axr + axi * axi;
+    s[1] += xx.x * axr + xx.y * axi;
+    s[2] += ayr * ayr + ayi * ayi;
+    s[3] += xx.x * ayr + xx.y * ayi;
+  };
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * PTYK_NT);
+    const float4 a = *reinterpret_cast<const float4*>(&lds[e]);
+    const float4 b = *reinterpret_cast<const float4*>(&lds_y[e]);
+    acc(make_float2(a.x, a.y), make_float2(b.x, b.y), make_float2(v[k].x, v[k].y), make_float2(x[k].x, x[k].y));
+    acc(make_float2(a.z, a.w), make_float2(b.z, b.w), make_float2(v[k].z, v[k].w), make_float2(x[k].z, x[k].w));
+  }
+  double* red = reinterpret_cast<double*>(lds);  // block_sum syncs before its first write
+  for (int t = 0; t < 4; ++t) {
+    const double r = block_sum<PTYK_NT>(s[t], red);
+    if (tid == 0) part[((long)j * gridDim.x + blockIdx.x) * 4 + t] = r;
+  }
+}
+
+// Position-step partials straight from the derivative spectra's row transforms; returns the
+// chunks per frame, or 0 when m has no fused form (the caller then runs the column transform
+// and k_pos_partial).
+static int pos_cols(const float2* dx, const float2* dy, const float2* vp, const float2* chi, int G, int m,
+                    const float2* tw, double* part, hipStream_t s) {
+  auto go = [&](auto kern, int nc) {
+    const int nb = m / nc;
+    hipLaunchKernelGGL(kern, dim3(nb, G), dim3(PTYK_NT), 0, s, dx, dy, vp, chi, tw, part);
+    return nb;
+  };
+  switch (m) {
+    case 128: return go(k_pos_cols<128>, (plan_min_batch<128, PTYK_NT>() + 15) / 16 * 16);
+    case 256: return go(k_pos_cols<256>, (plan_min_batch<256, PTYK_NT>() + 15) / 16 * 16);
+    default: return 0;
+  }
+}
+
 // Step clipped to 0.2 px per axis; the column-derivative step is added to the row
 // coordinate and the row-derivative step to the column coordinate, as the reference does
 // (probe_positions[i].x += shift from dX, Probe.cpp:1022-1033).
@@ -1319,16 +1405,22 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         // tmp already holds its spectrum; tmp and tmp2 take one inverse column transform (tmp2
         // follows tmp's max(gmax, K) frames)
         const long span = std::max<long>(d.gmax, K) + G;
+        int pch = 0;  // position-step chunks per frame once the partials are in part
         if (deriv_rows(d.tmp, d.tmp2, G, m, d.tw, s)) {
           CHK();
-          TRY(fft_cols_batch(d.tmp, span, m, true, d.tw, s));
+          pch = pos_cols(d.tmp, d.tmp2, d.vp, chi0, G, m, d.tw, d.part, s);
+          if (!pch) TRY(fft_cols_batch(d.tmp, span, m, true, d.tw, s));
         } else {
           LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
           CHK();
           TRY(fft2_batch(d.tmp, span, m, true, d.tw, s));
         }
-        LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
-        LAUNCH(k_pos_update, dim3(1), dim3(1024), d.part, ch, gf, G, d.pos, d.ftot);
+        CHK();
+        if (!pch) {
+          LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
+          pch = ch;
+        }
+        LAUNCH(k_pos_update, dim3(1), dim3(1024), d.part, pch, gf, G, d.pos, d.ftot);
         CHK();
       }
       if (var_update && ll == 1) {
