@@ -754,72 +754,6 @@ __global__ void k_deriv(float2* __restrict__ x, float2* __restrict__ y, int m) {
   y[i] = cscale(t, freq(q / m, m) * inv);
 }
 
-// k_deriv fused into the inverse row pass: each workgroup loads NB/2 spectrum rows, forms both
-// derivative spectra of each row in LDS (NB rows) and transforms them together; x rows go back
-// to x, y rows to y.  The column pass follows as a plain inverse column transform.  Same float
-// operations as k_deriv followed by the row transform.
-template <int M>
-__global__ __launch_bounds__(PTYK_NT) void k_deriv_rows(float2* __restrict__ x, float2* __restrict__ y, long rows,
-                                                        const float2* __restrict__ tw) {
-  constexpr int NB = plan_min_batch<M, PTYK_NT>();
-  constexpr int NI = NB / 2;
-  constexpr int PAIRS = NI * M / 2 / PTYK_NT;
-  static_assert(NB % 2 == 0 && (NI * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
-  using L = SwzRowLayout<M, M>;
-  __shared__ float2 lds[NB * M];
-  const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * NI;
-  const float inv = 1.f / (float)(M * M);
-  float4 v[PAIRS];
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const long g = row0 + e / M;
-    v[i] = g < rows ? *reinterpret_cast<const float4*>(x + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const int rr = e / M, c = e % M;
-    const int r = (int)((row0 + rr) % M);
-    const float2 t0 = make_float2(-v[i].y * kTwoPi, v[i].x * kTwoPi);
-    const float2 t1 = make_float2(-v[i].w * kTwoPi, v[i].z * kTwoPi);
-    const float fc0 = freq(c, M) * inv, fc1 = freq(c + 1, M) * inv, fr = freq(r, M) * inv;
-    const float2 x0 = cscale(t0, fc0), x1 = cscale(t1, fc1);
-    const float2 y0 = cscale(t0, fr), y1 = cscale(t1, fr);
-    *reinterpret_cast<float4*>(&lds[L::addr(rr, c)]) = make_float4(x0.x, x0.y, x1.x, x1.y);
-    *reinterpret_cast<float4*>(&lds[L::addr(NI + rr, c)]) = make_float4(y0.x, y0.y, y1.x, y1.y);
-  }
-  __syncthreads();
-  fft_lds<M, true, NB, PTYK_NT, L, false>(lds, tw, tid);
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const int rr = e / M, c = e % M;
-    const long g = row0 + rr;
-    if (g < rows) {
-      *reinterpret_cast<float4*>(x + g * M + c) = *reinterpret_cast<const float4*>(&lds[L::addr(rr, c)]);
-      *reinterpret_cast<float4*>(y + g * M + c) = *reinterpret_cast<const float4*>(&lds[L::addr(NI + rr, c)]);
-    }
-  }
-}
-
-// Derivative spectra of G frames of x (in place) and into y, inverse-transformed along rows;
-// 0 when m has no fused form (the caller then runs k_deriv and the full transform).
-static int deriv_rows(float2* x, float2* y, int G, int m, const float2* tw, hipStream_t s) {
-  auto go = [&](auto kern, int ni) {
-    const long rows = (long)G * m;
-    hipLaunchKernelGGL(kern, dim3((unsigned)((rows + ni - 1) / ni)), dim3(PTYK_NT), 0, s, x, y, rows, tw);
-    return 1;
-  };
-  switch (m) {
-    case 128: return go(k_deriv_rows<128>, plan_min_batch<128, PTYK_NT>() / 2);
-    case 256: return go(k_deriv_rows<256>, plan_min_batch<256, PTYK_NT>() / 2);
-    case 512: return go(k_deriv_rows<512>, plan_min_batch<512, PTYK_NT>() / 2);
-    default: return 0;
-  }
-}
-
 // [sum |dX vp|^2, sum Re(chi conj(dX vp)), same for dY] per frame chunk.
 __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ dx, const float2* __restrict__ dy,
                                                     const float2* __restrict__ vp, const float2* __restrict__ chi, int m,
@@ -844,19 +778,78 @@ __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ d
   }
 }
 
-// The inverse column transform of both derivative images fused with k_pos_partial: a workgroup
-// takes the same NC columns (whole 16-column strips) of frame j from dx and from dy, transforms
-// them in LDS, and sums the four products with vp and chi over its columns; the derivative
-// images are never written back.  Partials: part[(j * gridDim.x + blockIdx.x) * 4 + v].
+// Position step from one-dimensional transforms.  The gradient solver's derivative images
+// dX = IFFT2(FFT2(O_r) 2 pi i f_col) / M^2 and dY = IFFT2(FFT2(O_r) 2 pi i f_row) / M^2
+// (Probe.cpp:956-1118) have multipliers that depend on one frequency index only, so the
+// transforms along the other axis cancel (IFFT o FFT = M along an axis):
+// dX = IFFT_rows(FFT_rows(O_r) 2 pi i f_col) / M and dY = IFFT_cols(FFT_cols(O_r) 2 pi i f_row) / M.
+// k_dx_rows writes dX; k_pos_cols forms dY of whole 16-column strips in LDS and sums the four
+// products with vp and chi over them, so dY is never stored.
+
+// t 2 pi i s for a spectrum value t and a real scale s
+__device__ __forceinline__ float2 deriv_of(float2 t, float sc) {
+  return cscale(make_float2(-t.y * kTwoPi, t.x * kTwoPi), sc);
+}
+
 template <int M>
-__global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__ dx, const float2* __restrict__ dy,
+__global__ __launch_bounds__(PTYK_NT) void k_dx_rows(const float2* __restrict__ op, float2* __restrict__ dx, long rows,
+                                                     const float2* __restrict__ tw) {
+  constexpr int NB = plan_min_batch<M, PTYK_NT>();
+  constexpr int PAIRS = NB * M / 2 / PTYK_NT;
+  static_assert((NB * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[NB * M];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * NB;
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      const long g = row0 + e / M;
+      v[i] = g < rows ? *reinterpret_cast<const float4*>(op + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = v[i];
+    }
+  }
+  __syncthreads();
+  fft_lds<M, false, NB, PTYK_NT, L, false>(lds, tw, tid);
+  const float inv = 1.f / (float)M;
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const int c = e % M;
+    float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, c)]);
+    const float4 t = *p;
+    const float2 a = deriv_of(make_float2(t.x, t.y), freq(c, M) * inv);
+    const float2 b = deriv_of(make_float2(t.z, t.w), freq(c + 1, M) * inv);
+    *p = make_float4(a.x, a.y, b.x, b.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NB, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    const long g = row0 + e / M;
+    if (g < rows)
+      *reinterpret_cast<float4*>(dx + g * M + (e % M)) = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+  }
+}
+
+// Partials part[(j * gridDim.x + blockIdx.x) * 4 + v] of [sum |dX vp|^2, sum Re(chi conj(dX vp)),
+// same for dY] over the workgroup's NC columns of frame j.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__ op, const float2* __restrict__ dx,
                                                       const float2* __restrict__ vp, const float2* __restrict__ chi,
                                                       const float2* __restrict__ tw, double* __restrict__ part) {
   constexpr int NC = (plan_min_batch<M, PTYK_NT>() + 15) / 16 * 16;
   constexpr int PAIRS = NC * M / 2 / PTYK_NT;
+  constexpr bool PRE = PAIRS <= 8;  // dX, vp and chi in flight during the transforms
   static_assert((NC * M / 2) % PTYK_NT == 0, "column tile must divide among the threads");
-  __shared__ float2 lds[2 * NC * M];  // dx columns, then dy columns
-  float2* lds_y = lds + NC * M;
+  __shared__ float2 lds[NC * M];
   const int tid = threadIdx.x;
   const int j = blockIdx.y, c0 = blockIdx.x * NC;
   const long fbase = (long)j * M * M + c0;
@@ -865,34 +858,42 @@ __global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__
     return fbase + (long)i * M + g * 16 + c;
   };
   {
-    float4 a[PAIRS], b[PAIRS];
+    float4 o[PAIRS];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) o[k] = *reinterpret_cast<const float4*>(op + gaddr(2 * (tid + k * PTYK_NT)));
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) *reinterpret_cast<float4*>(&lds[2 * (tid + k * PTYK_NT)]) = o[k];
+  }
+  float4 a[PAIRS], v[PAIRS], x[PAIRS];
+  auto load_rest = [&]() {
 #pragma unroll
     for (int k = 0; k < PAIRS; ++k) {
       const long q = gaddr(2 * (tid + k * PTYK_NT));
       a[k] = *reinterpret_cast<const float4*>(dx + q);
-      b[k] = *reinterpret_cast<const float4*>(dy + q);
+      v[k] = *reinterpret_cast<const float4*>(vp + q);
+      x[k] = *reinterpret_cast<const float4*>(chi + q);
     }
-#pragma unroll
-    for (int k = 0; k < PAIRS; ++k) {
-      *reinterpret_cast<float4*>(&lds[2 * (tid + k * PTYK_NT)]) = a[k];
-      *reinterpret_cast<float4*>(&lds_y[2 * (tid + k * PTYK_NT)]) = b[k];
-    }
-  }
-  // vp and chi of the same pixels, in flight during the transforms
-  float4 v[PAIRS], x[PAIRS];
+  };
+  if constexpr (PRE) load_rest();
+  __syncthreads();
+  fft_lds<M, false, NC, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
+  const float inv = 1.f / (float)M;
 #pragma unroll
   for (int k = 0; k < PAIRS; ++k) {
-    const long q = gaddr(2 * (tid + k * PTYK_NT));
-    v[k] = *reinterpret_cast<const float4*>(vp + q);
-    x[k] = *reinterpret_cast<const float4*>(chi + q);
+    const int e = 2 * (tid + k * PTYK_NT);
+    const float sc = freq((e >> 4) % M, M) * inv;  // both values of the pair share row (e >> 4) % M
+    float4* p = reinterpret_cast<float4*>(&lds[e]);
+    const float4 t = *p;
+    const float2 b0 = deriv_of(make_float2(t.x, t.y), sc), b1 = deriv_of(make_float2(t.z, t.w), sc);
+    *p = make_float4(b0.x, b0.y, b1.x, b1.y);
   }
   __syncthreads();
   fft_lds<M, true, NC, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
-  fft_lds<M, true, NC, PTYK_NT, StripLayout<M>, true>(lds_y, tw, tid);
+  if constexpr (!PRE) load_rest();
   double s[4] = {0, 0, 0, 0};
-  auto acc = [&](float2 a, float2 b, float2 vv, float2 xx) {
-    const double axr = (double)a.x * vv.x - (double)a.y * vv.y, axi = (double)a.x * vv.y + (double)a.y * vv.x;
-    const double ayr = (double)b.x * vv.x - (double)b.y * vv.y, ayi = (double)b.x * vv.y + (double)b.y * vv.x;
+  auto acc = [&](float2 ax, float2 ay, float2 vv, float2 xx) {
+    const double axr = (double)ax.x * vv.x - (double)ax.y * vv.y, axi = (double)ax.x * vv.y + (double)ax.y * vv.x;
+    const double ayr = (double)ay.x * vv.x - (double)ay.y * vv.y, ayi = (double)ay.x * vv.y + (double)ay.y * vv.x;
     s[0] += axr * axr + axi * axi;
     s[1] += xx.x * axr + xx.y * axi;
     s[2] += ayr * ayr + ayi * ayi;
@@ -900,11 +901,11 @@ __global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__
   };
 #pragma unroll
   for (int k = 0; k < PAIRS; ++k) {
-    const int e = 2 * (tid + k * PTYK_NT);
-    const float4 a = *reinterpret_cast<const float4*>(&lds[e]);
-    const float4 b = *reinterpret_cast<const float4*>(&lds_y[e]);
-    acc(make_float2(a.x, a.y), make_float2(b.x, b.y), make_float2(v[k].x, v[k].y), make_float2(x[k].x, x[k].y));
-    acc(make_float2(a.z, a.w), make_float2(b.z, b.w), make_float2(v[k].z, v[k].w), make_float2(x[k].z, x[k].w));
+    const float4 b = *reinterpret_cast<const float4*>(&lds[2 * (tid + k * PTYK_NT)]);
+    acc(make_float2(a[k].x, a[k].y), make_float2(b.x, b.y), make_float2(v[k].x, v[k].y),
+        make_float2(x[k].x, x[k].y));
+    acc(make_float2(a[k].z, a[k].w), make_float2(b.z, b.w), make_float2(v[k].z, v[k].w),
+        make_float2(x[k].z, x[k].w));
   }
   double* red = reinterpret_cast<double*>(lds);  // block_sum syncs before its first write
   for (int t = 0; t < 4; ++t) {
@@ -913,22 +914,29 @@ __global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__
   }
 }
 
-// Position-step partials straight from the derivative spectra's row transforms; returns the
-// chunks per frame, or 0 when m has no fused form (the caller then runs the column transform
-// and k_pos_partial).
-static int pos_cols(const float2* dx, const float2* dy, const float2* vp, const float2* chi, int G, int m,
-                    const float2* tw, double* part, hipStream_t s) {
-  auto go = [&](auto kern, int nc) {
-    const int nb = m / nc;
-    hipLaunchKernelGGL(kern, dim3(nb, G), dim3(PTYK_NT), 0, s, dx, dy, vp, chi, tw, part);
-    return nb;
+// Position-step partials of G frames from the object patches op; dX goes to dx.  Returns the
+// chunks per frame, or 0 when m has no one-dimensional form (the caller then takes the 2D one).
+static int pos_1d(const float2* op, float2* dx, const float2* vp, const float2* chi, int G, int m, const float2* tw,
+                  double* part, hipStream_t s) {
+  auto go = [&](auto rows_kern, auto cols_kern, int nb, int nc) {
+    const long rows = (long)G * m;
+    hipLaunchKernelGGL(rows_kern, dim3((unsigned)((rows + nb - 1) / nb)), dim3(PTYK_NT), 0, s, op, dx, rows, tw);
+    hipLaunchKernelGGL(cols_kern, dim3(m / nc, G), dim3(PTYK_NT), 0, s, op, dx, vp, chi, tw, part);
+    return m / nc;
   };
+#define PTYK_POS_CASE(MM)                                                                         \
+  case MM:                                                                                        \
+    return go(k_dx_rows<MM>, k_pos_cols<MM>, plan_min_batch<MM, PTYK_NT>(),                       \
+              (plan_min_batch<MM, PTYK_NT>() + 15) / 16 * 16);
   switch (m) {
-    case 128: return go(k_pos_cols<128>, (plan_min_batch<128, PTYK_NT>() + 15) / 16 * 16);
-    case 256: return go(k_pos_cols<256>, (plan_min_batch<256, PTYK_NT>() + 15) / 16 * 16);
+    PTYK_POS_CASE(128)
+    PTYK_POS_CASE(256)
+    PTYK_POS_CASE(512)
     default: return 0;
   }
+#undef PTYK_POS_CASE
 }
+static bool has_pos_1d(int m) { return m == 128 || m == 256 || m == 512; }
 
 // Step clipped to 0.2 px per axis; the column-derivative step is added to the row
 // coordinate and the row-derivative step to the column coordinate, as the reference does
@@ -1310,6 +1318,9 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const bool pos_step = iter >= d.pps;
     if (subpix) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     const int mod_blocks = mls_modulus_blocks(m, K);
+    // with the three-pass exit waves the object patches stay in op, and the position step's
+    // derivative images come from one-dimensional transforms of them
+    const bool pos_1d_step = pos_step && mod_blocks && has_pos_1d(m);
     if (subpix && mod_blocks) {
       // the shifted illumination probes' spectra formed in the inverse row pass itself
       TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
@@ -1322,9 +1333,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     if (mod_blocks) {
       // three passes for the exit waves, the first of which forms them (and the object patches)
       // from O, P and vp; the position step's patch copy takes its own forward transform
-      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, pos_step ? d.tmp : nullptr};
+      const bool spec2d = pos_step && !pos_1d_step;
+      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
       TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, s));
-      if (pos_step) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
+      if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
     } else {
       LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
              m, d.cols, subpix);
@@ -1404,22 +1416,17 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       if (pos_step && ll == 1) {
         // tmp already holds its spectrum; tmp and tmp2 take one inverse column transform (tmp2
         // follows tmp's max(gmax, K) frames)
-        const long span = std::max<long>(d.gmax, K) + G;
-        int pch = 0;  // position-step chunks per frame once the partials are in part
-        if (deriv_rows(d.tmp, d.tmp2, G, m, d.tw, s)) {
-          CHK();
-          pch = pos_cols(d.tmp, d.tmp2, d.vp, chi0, G, m, d.tw, d.part, s);
-          if (!pch) TRY(fft_cols_batch(d.tmp, span, m, true, d.tw, s));
+        int pch = ch;  // chunks per frame of the position-step partials
+        if (pos_1d_step) {
+          pch = pos_1d(d.op, d.tmp, d.vp, chi0, G, m, d.tw, d.part, s);
         } else {
-          LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);
+          LAUNCH(k_deriv, gpix, dim3(NT), d.tmp, d.tmp2, m);  // tmp already holds its spectrum
           CHK();
-          TRY(fft2_batch(d.tmp, span, m, true, d.tw, s));
+          // tmp and tmp2 in one inverse transform (tmp2 follows tmp's max(gmax, K) frames)
+          TRY(fft2_batch(d.tmp, std::max<long>(d.gmax, K) + G, m, true, d.tw, s));
+          LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
         }
         CHK();
-        if (!pch) {
-          LAUNCH(k_pos_partial, gch, dim3(NT), d.tmp, d.tmp2, d.vp, chi0, m, d.part);
-          pch = ch;
-        }
         LAUNCH(k_pos_update, dim3(1), dim3(1024), d.part, pch, gf, G, d.pos, d.ftot);
         CHK();
       }

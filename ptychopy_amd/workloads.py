@@ -133,6 +133,7 @@ def mls_bytes_per_frame(m: int, obj_px: int, n: int, groups: int) -> float:
 
 def mls_flops_per_frame(m: int, k: int) -> float:
     """FFT work of one MLs frame-iteration, 5 n log2 n per complex n-point transform: forward
-    and inverse per mode (2K), the probe's sub-pixel shift and its inverse (2) and the three
-    transforms of the position solver."""
-    return (2 * k + 2 + 3) * 10.0 * m * m * np.log2(m)
+    and inverse 2D transforms per mode (2K), the probe's sub-pixel shift and its inverse (2) and
+    the position solver's forward and inverse one-dimensional transforms along rows (dX) and
+    along columns (dY), two 2D transforms' worth."""
+    return (2 * k + 2 + 2) * 10.0 * m * m * np.log2(m)
