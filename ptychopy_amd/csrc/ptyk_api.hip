@@ -108,9 +108,12 @@ static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
-                  d.ftot};  // d.tmp and d.tmp2 point into d.psi
+                  d.ftot, d.dxb, d.pos_part, d.pos_ftot};  // d.tmp and d.tmp2 point into d.psi
   for (void* p : ptrs)
     if (p) hipFree(p);
+  if (d.ev_fork) hipEventDestroy(d.ev_fork);
+  if (d.ev_join) hipEventDestroy(d.ev_join);
+  if (d.side) hipStreamDestroy(d.side);
   d = MlsDev();
   c->mls_ready = false;
 }
@@ -770,12 +773,17 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const long mm = c->mm;
   const int ch = mls_chunks(d.m);
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
+  if (!d.side) {
+    HIPCHK(c, hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&d.ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&d.ev_join, hipEventDisableTiming));
+  }
   if (g > d.gmax) {
-    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.gscal, d.ftot};  // tmp and tmp2 live inside psi
+    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.gscal, d.ftot, d.dxb, d.pos_part, d.pos_ftot};  // tmp, tmp2 in psi
     for (void* p : ptrs)
       if (p) hipFree(p);
-    d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = nullptr;
-    d.gscal = d.ftot = nullptr;
+    d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = d.dxb = nullptr;
+    d.gscal = d.ftot = d.pos_part = d.pos_ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
     const long tcap = std::max<long>(g, d.k);
@@ -789,6 +797,9 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.pu, gm));
     HIPCHK(c, dalloc(&d.gscal, (size_t)MLS_S_COEF + g));
     HIPCHK(c, dalloc(&d.ftot, (size_t)8 * g));
+    HIPCHK(c, dalloc(&d.dxb, gm));
+    HIPCHK(c, dalloc(&d.pos_part, (size_t)4 * g * (d.m / 16)));
+    HIPCHK(c, dalloc(&d.pos_ftot, (size_t)4 * g));
     d.gmax = g;
   }
   const long need = std::max<long>({6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L});

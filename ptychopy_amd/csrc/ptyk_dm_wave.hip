@@ -448,9 +448,8 @@ static hipError_t wave_impl(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
 // The MLs exit waves (K modes x G frames, mode-major planes) go through forward FFT, the
 // multi-mode modulus step and the inverse FFT (MLS.cpp:119-140 with Diffractions.cpp:261-310) in
 // three passes instead of five: the row passes above (plain input), and between them one pass
-// over 64 x 32 blocks that holds every mode of a block, so the modulus, which needs
+// over 64 x 16 blocks that holds every mode of a block, so the modulus, which needs
 // sum_k |Psi_k|^2 at each pixel, runs between the forward and inverse 64-point column transforms.
-#define PTYK_MLS_CB 32
 
 // Row pass of the MLs exit waves formed in place of their load (MLS.cpp:119-126 with
 // d_readFromArray_c, utilitiesKernels.cu:1064-1084): plane k G + j, row r holds
@@ -514,94 +513,127 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_a(float2* __restrict__ psi
   row_pass_forward<M>(lds, psi, 0, n_groups, tw, tid);
 }
 
-// Row pass of the MLs probe-update planes formed in place of their load: plane j, row r holds
-// chi0_j conj(op_j) (the arithmetic of the probe-update kernel it replaces).
-template <int M>
-__global__ __launch_bounds__(PTYK_NT) void k_mls_pu_pass_a(float2* __restrict__ pu, const float2* __restrict__ chi,
-                                                           const float2* __restrict__ op, long n_groups,
-                                                           const float2* __restrict__ tw) {
-  using C = WaveCfg<M>;
-  using IO = RowIO<M>;
-  using L = SwzRowLayout<M, M>;
-  __shared__ float2 lds[C::ROWS * M];
-  const int tid = threadIdx.x;
-  constexpr int HALF = IO::PAIRS > 1 ? IO::PAIRS / 2 : 1;
-#pragma unroll
-  for (int h0 = 0; h0 < IO::PAIRS; h0 += HALF) {
-    float4 av[HALF], bv[HALF];
-#pragma unroll
-    for (int u = 0; u < HALF; ++u) {
-      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
-      long f;
-      int r, c;
-      if (IO::locate(e, 0, n_groups, &f, &r, &c)) {
-        av[u] = *reinterpret_cast<const float4*>(chi + (f * M + r) * M + c);
-        bv[u] = *reinterpret_cast<const float4*>(op + (f * M + r) * M + c);
-      } else {
-        av[u] = bv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < HALF; ++u) {
-      const int e = 2 * (tid + (h0 + u) * PTYK_NT);
-      const float4 a = av[u], b = bv[u];
-      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
-          make_float4(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y, a.z * b.z + a.w * b.w, a.w * b.z - a.z * b.w);
-    }
-  }
-  row_pass_forward<M>(lds, pu, 0, n_groups, tw, tid);
-}
-
 // Frequency of FFT index i in cycles per sample, in [-0.5, 0.5) (the MLs spectral shift's).
 __device__ __forceinline__ float shift_freq(int i, int m) {
   return (float)((i + (m >> 1)) & (m - 1)) / (float)m - 0.5f;
 }
 
-// One plane's 64 x 32 block between the row passes: 64-point columns, the spectral shift
-// x *= exp(-2 pi i sign (sub.x f[col] + sub.y f[row])) / M^2 (the arithmetic of the ramp kernel it
-// replaces), inverse columns.
+// The MLs sub-pixel shift x <- IFFT2(FFT2(x) exp(-2 pi i sign (sub.x f[col] + sub.y f[row]))) / M^2
+// (Probe::shift_probe, Probe.cpp:1301-1331) is separable: the ramp is a product of a factor
+// that depends on the column frequency only and one that depends on the row frequency only, so
+// it is a shift of every row (transform, ramp, inverse transform along the row, / M) followed by
+// a shift of every column.  Two passes over the planes instead of a 2D transform pair.
+
+// exp(-2 pi i nom) t / M
+__device__ __forceinline__ float2 shift_ramp(float2 t, float nom, float inv_m) {
+  float sn, cs;
+  sincosf(nom * -6.283185307179586f, &sn, &cs);
+  return cscale(cmul(t, make_float2(cs, sn)), inv_m);
+}
+
+// Row shifts of the probe-update planes formed in place of their load: plane j, row r holds
+// chi0_j conj(op_j) (the arithmetic of the probe-update kernel it replaces).  A workgroup's rows
+// belong to one plane (NB divides M).
 template <int M>
-__global__ __launch_bounds__(PTYK_NT) void k_mls_ramp_b(float2* __restrict__ x, const float2* __restrict__ sub,
-                                                        const int* __restrict__ gf, float sign,
-                                                        const float2* __restrict__ tw) {
-  using C = WaveCfg<M>;
-  constexpr int CB = PTYK_MLS_CB, NCB = M / CB, BPF = C::R0 * NCB;
-  constexpr int BLK = 64 * CB, PAIRS = BLK / 2 / PTYK_NT, PIX = BLK / PTYK_NT;
-  __shared__ float2 lds[BLK];
+__global__ __launch_bounds__(PTYK_NT) void k_mls_pu_rows(float2* __restrict__ pu, const float2* __restrict__ chi,
+                                                         const float2* __restrict__ op, const float2* __restrict__ sub,
+                                                         const int* __restrict__ gf, float sign,
+                                                         const float2* __restrict__ tw) {
+  constexpr int NB = plan_min_batch<M, PTYK_NT>();
+  constexpr int PAIRS = NB * M / 2 / PTYK_NT;
+  static_assert(M % NB == 0 && (NB * M / 2) % PTYK_NT == 0, "row tile must divide the plane and the threads");
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[NB * M];
   const int tid = threadIdx.x;
-  const int j = blockIdx.x / BPF, blk = blockIdx.x % BPF;
-  const int k2 = blk / NCB, c0 = (blk % NCB) * CB;
-  const long mm = (long)M * M;
-  float2* base = x + (long)j * mm + (long)(64 * k2) * M + c0;
+  const long row0 = (long)blockIdx.x * NB;
+  const int j = (int)(row0 / M);
   {
-    float4 v[PAIRS];
+    float4 av[PAIRS], bv[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const long g = row0 * M + 2 * (tid + i * PTYK_NT);
+      av[i] = *reinterpret_cast<const float4*>(chi + g);
+      bv[i] = *reinterpret_cast<const float4*>(op + g);
+    }
 #pragma unroll
     for (int i = 0; i < PAIRS; ++i) {
       const int e = 2 * (tid + i * PTYK_NT);
-      v[i] = *reinterpret_cast<const float4*>(base + (long)(e / CB) * M + (e % CB));
+      const float4 a = av[i], b = bv[i];
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
+          make_float4(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y, a.z * b.z + a.w * b.w, a.w * b.z - a.z * b.w);
     }
-#pragma unroll
-    for (int i = 0; i < PAIRS; ++i) *reinterpret_cast<float4*>(&lds[2 * (tid + i * PTYK_NT)]) = v[i];
   }
-  const float2 sv = sub[gf[j]];
+  // a thread's columns are the same in every row it holds (2 NT is a multiple of M), so its
+  // two ramp factors are computed once
+  static_assert((2 * PTYK_NT) % M == 0, "a thread's columns must repeat across its rows");
+  const float sx = sign * sub[gf[j]].x;
+  const int c = (2 * tid) % M;
+  float2 w0, w1;
+  sincosf(sx * shift_freq(c, M) * -6.283185307179586f, &w0.y, &w0.x);
+  sincosf(sx * shift_freq(c + 1, M) * -6.283185307179586f, &w1.y, &w1.x);
   __syncthreads();
-  fft_lds<64, false, CB, PTYK_NT, ColLayout<CB>, true>(lds, tw, tid);
-  const float inv_mm = 1.f / (float)mm;
-#pragma unroll
-  for (int p = 0; p < PIX; ++p) {
-    const int idx = tid + p * PTYK_NT;
-    const int r = C::R0 * (idx / CB) + k2, c = c0 + idx % CB;
-    const float nom = (sign * sv.x) * shift_freq(c, M) + (sign * sv.y) * shift_freq(r, M);
-    float sn, cs;
-    sincosf(nom * -6.283185307179586f, &sn, &cs);
-    lds[idx] = cscale(cmul(lds[idx], make_float2(cs, sn)), inv_mm);
-  }
-  __syncthreads();
-  fft_lds<64, true, CB, PTYK_NT, ColLayout<CB>, true>(lds, tw, tid);
+  fft_lds<M, false, NB, PTYK_NT, L, false>(lds, tw, tid);
+  const float inv_m = 1.f / (float)M;
 #pragma unroll
   for (int i = 0; i < PAIRS; ++i) {
     const int e = 2 * (tid + i * PTYK_NT);
-    *reinterpret_cast<float4*>(base + (long)(e / CB) * M + (e % CB)) = *reinterpret_cast<const float4*>(&lds[e]);
+    float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, c)]);
+    const float4 t = *p;
+    const float2 u0 = cscale(cmul(make_float2(t.x, t.y), w0), inv_m);
+    const float2 u1 = cscale(cmul(make_float2(t.z, t.w), w1), inv_m);
+    *p = make_float4(u0.x, u0.y, u1.x, u1.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NB, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    *reinterpret_cast<float4*>(pu + row0 * M + e) = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+  }
+}
+
+// Column shifts in place: a workgroup takes NC columns (whole 16-column strips) of plane j.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_shift_cols(float2* __restrict__ x, const float2* __restrict__ sub,
+                                                            const int* __restrict__ gf, float sign,
+                                                            const float2* __restrict__ tw) {
+  constexpr int NC = (plan_min_batch<M, PTYK_NT>() + 15) / 16 * 16;
+  constexpr int PAIRS = NC * M / 2 / PTYK_NT;
+  static_assert((NC * M / 2) % PTYK_NT == 0, "column tile must divide among the threads");
+  __shared__ float2 lds[NC * M];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y;
+  float2* base = x + (long)j * M * M + blockIdx.x * NC;
+  auto off = [&](int e) {  // e: strip-major element index of the tile (even)
+    return (long)((e >> 4) % M) * M + (e / (M * 16)) * 16 + (e & 15);
+  };
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) v[k] = *reinterpret_cast<const float4*>(base + off(2 * (tid + k * PTYK_NT)));
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) *reinterpret_cast<float4*>(&lds[2 * (tid + k * PTYK_NT)]) = v[k];
+  }
+  const float sy = sign * sub[gf[j]].y;
+  __syncthreads();
+  fft_lds<M, false, NC, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
+  const float inv_m = 1.f / (float)M;
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * PTYK_NT);
+    const float nom = sy * shift_freq((e >> 4) % M, M);  // both values of the pair share the row
+    float4* p = reinterpret_cast<float4*>(&lds[e]);
+    const float4 t = *p;
+    const float2 u0 = shift_ramp(make_float2(t.x, t.y), nom, inv_m);
+    const float2 u1 = shift_ramp(make_float2(t.z, t.w), nom, inv_m);
+    *p = make_float4(u0.x, u0.y, u1.x, u1.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NC, PTYK_NT, StripLayout<M>, true>(lds, tw, tid);
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * PTYK_NT);
+    *reinterpret_cast<float4*>(base + off(e)) = *reinterpret_cast<const float4*>(&lds[e]);
   }
 }
 
@@ -745,13 +777,10 @@ hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* am
 template <int M>
 static hipError_t mls_shift_m(float2* pu, const float2* chi, const float2* op, int G, const float2* sub,
                               const int* gf, float sign, const float2* tw, hipStream_t s) {
-  using C = WaveCfg<M>;
-  constexpr int BPF = C::R0 * (M / PTYK_MLS_CB);
-  const long groups = (long)G * C::L;
-  const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
-  hipLaunchKernelGGL((k_mls_pu_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, pu, chi, op, groups, tw);
-  hipLaunchKernelGGL((k_mls_ramp_b<M>), dim3((unsigned)(G * BPF)), dim3(PTYK_NT), 0, s, pu, sub, gf, sign, tw);
-  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, pu, 0L, groups, tw);
+  constexpr int NB = plan_min_batch<M, PTYK_NT>(), NC = (NB + 15) / 16 * 16;
+  hipLaunchKernelGGL((k_mls_pu_rows<M>), dim3((unsigned)((long)G * M / NB)), dim3(PTYK_NT), 0, s, pu, chi, op, sub, gf,
+                     sign, tw);
+  hipLaunchKernelGGL((k_mls_shift_cols<M>), dim3(M / NC, G), dim3(PTYK_NT), 0, s, pu, sub, gf, sign, tw);
   return hipGetLastError();
 }
 

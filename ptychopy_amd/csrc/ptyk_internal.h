@@ -104,8 +104,8 @@ struct MlsExitArgs {
   float2* op2;  // nullable
 };
 int mls_modulus_blocks(int m, int k);
-// pu_j = IFFT(exp(-2 pi i sign (sub f)) / M^2 FFT(chi_j conj(op_j))) for G planes in three passes
-// (m in {128, 256, 512}; ptyk_dm_wave.hip).
+// pu_j = IFFT(exp(-2 pi i sign (sub f)) / M^2 FFT(chi_j conj(op_j))) for G planes in two passes,
+// the separable ramp applied as row shifts then column shifts (m in {128, 256, 512}; ptyk_dm_wave.hip).
 hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,
                             const int* gf, float sign, const float2* tw, hipStream_t s);
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
@@ -168,6 +168,13 @@ struct MlsDev {
   double* ftot = nullptr;       // per-frame totals of the group (gmax x 8)
   double* gscal = nullptr;
   int* gframes = nullptr;       // frames of all groups of one iteration
+  // position step on a second stream (it reads only the group's exit waves, illumination
+  // probes and object patches, and writes only these and the positions)
+  float2* dxb = nullptr;        // column-derivative images, gmax frames
+  double* pos_part = nullptr;   // partials, gmax x (m / 16) x 4
+  double* pos_ftot = nullptr;   // per-frame totals, gmax x 4
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 int mls_chunks(int m);  // frame chunks of the per-frame reductions
 // Probe::initEvo: normalisations and the initial illumination (host-synchronous).
