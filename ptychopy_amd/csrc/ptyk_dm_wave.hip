@@ -56,7 +56,7 @@ __device__ __forceinline__ void cross_rows(float2* lds, const float2* __restrict
   using L = SwzRowLayout<M, M>;
   constexpr int STEPS = C::GPW * M / PTYK_NT;
   static_assert(C::GPW * M % PTYK_NT == 0, "cross-row step must divide among the threads");
-  auto step = [&](int t) {
+  auto step = [&](int t) __attribute__((always_inline)) {
     const int g = t / M, c = t % M;
     const int n1 = (int)(((long)blockIdx.x * C::GPW + g) % C::L);
     float2 w[C::R0];
@@ -723,11 +723,13 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
 
 template <int M, int K>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
+                                double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
+                                hipStream_t s) {
   using C = WaveCfg<M>;
   constexpr int BPF = C::R0 * (M / PTYK_MLS_MOD_CB);
-  const long groups = (long)K * G * C::L;
+  const long groups = (long)K * G * C::L, groups_c = (long)c_modes * G * C::L;
   const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
+  const unsigned gc = (unsigned)((groups_c + C::GPW - 1) / C::GPW);
   const size_t lds = (size_t)K * 64 * PTYK_MLS_MOD_CB * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
@@ -739,22 +741,22 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
                        groups, tw);
   hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp, amp_scale,
                      gf, part, tw);
-  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, 0L, groups, tw);
+  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
   return hipGetLastError();
 }
 
 template <int M>
 static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
+                                double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s) {
   switch (K) {
-    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -765,11 +767,128 @@ int mls_modulus_blocks(int m, int k) {
 }
 
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
-                              const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, hipStream_t s) {
+                              const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
+                              hipStream_t s) {
+  if (c_modes < 1 || c_modes > K) return hipErrorInvalidValue;
   switch (mls_modulus_blocks(m, K) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
-    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, s);
+    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
+    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
+    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ------------------------------------------------------------ MLs probe modes 2..K
+// Workgroup (row tile t, mode k, chunk z): the inverse row pass of k_dmw_pass_c for the tile of
+// each frame j of the chunk in turn (plane k G + j), then at every pixel of the tile
+// acc += w_j chi conj(op_j), w_j = p / (p + fw) with p the illumination at the frame's ROI (the
+// arithmetic of k_pum_modes); the chunk's sums go to part[(z (K - 1) + k - 1) M^2 + pixel].
+// The chi_k planes are never written back: nothing else reads them.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_pum_rows(const float2* __restrict__ psi, const float2* __restrict__ op,
+                                                      const float* __restrict__ illum, const int2* __restrict__ roi,
+                                                      const int* __restrict__ gf, int K, int G, int cols, float fw,
+                                                      float2* __restrict__ part, const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
+  __shared__ float2 lds[C::ROWS * M];
+  __shared__ float4 acc[IO::PAIRS * PTYK_NT];  // per-thread sums (LDS keeps them out of the registers)
+  const int tid = threadIdx.x;
+  const int k = blockIdx.y + 1;
+  const int j0 = blockIdx.z * MLS_PUM_FPC, j1 = min(G, j0 + MLS_PUM_FPC);
+  const long mm = (long)M * M;
+  constexpr long NOLIM = 1L << 40;  // locate's bound: the tile always lies inside its plane
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) acc[i * PTYK_NT + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int j = j0; j < j1; ++j) {
+    const long plane = (long)k * G + j;
+    if (j > j0) __syncthreads();  // the previous frame's products have read the tile
+    float4 v[IO::PAIRS];
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      long f;
+      int r, c;
+      IO::locate(2 * (tid + i * PTYK_NT), plane, NOLIM, &f, &r, &c);
+      v[i] = *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c);
+    }
+    const int2 o = roi[gf[j]];
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = v[i];
+    }
+    __syncthreads();
+    cross_rows<M, true>(lds, tw, tid);
+    __syncthreads();
+    fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      long f;
+      int r, c;
+      IO::locate(e, plane, NOLIM, &f, &r, &c);
+      const float4 bq = *reinterpret_cast<const float4*>(op + (long)j * mm + r * M + c);
+      const float* ip = illum + (long)(o.x + r) * cols + o.y + c;
+      const float p0 = ip[0], p1 = ip[1];
+      const float4 u = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+      const float w0 = p0 / (p0 + fw), w1 = p1 / (p1 + fw);
+      const float2 d0 = make_float2(u.x * bq.x + u.y * bq.y, u.y * bq.x - u.x * bq.y);
+      const float2 d1 = make_float2(u.z * bq.z + u.w * bq.w, u.w * bq.z - u.z * bq.w);
+      float4 t = acc[i * PTYK_NT + tid];
+      t.x += w0 * d0.x;
+      t.y += w0 * d0.y;
+      t.z += w1 * d1.x;
+      t.w += w1 * d1.y;
+      acc[i * PTYK_NT + tid] = t;
+    }
+  }
+  float2* out = part + ((long)blockIdx.z * (K - 1) + (k - 1)) * mm;
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    long f;
+    int r, c;
+    IO::locate(2 * (tid + i * PTYK_NT), 0, NOLIM, &f, &r, &c);
+    *reinterpret_cast<float4*>(out + r * M + c) = acc[i * PTYK_NT + tid];
+  }
+}
+
+// P_k += (sum over chunks in order) / G * beta for modes 2..K.
+__global__ void k_pum_finish(float2* __restrict__ probe, const float2* __restrict__ part, int K, int chunks, int G,
+                             long mm, const double* __restrict__ beta) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm) return;
+  const int k = blockIdx.y + 1;
+  float2 a = make_float2(0.f, 0.f);
+  for (int z = 0; z < chunks; ++z) a = cadd(a, part[((long)z * (K - 1) + (k - 1)) * mm + q]);
+  const float inv = (float)(1.0 / G);
+  const float2 bp = make_float2((float)beta[0], (float)beta[1]);
+  const long i = (long)k * mm + q;
+  probe[i] = cadd(probe[i], cmul(cscale(a, inv), bp));
+}
+
+template <int M>
+static hipError_t mode_updates_m(float2* probe, const float2* psi, const float2* op, const float* illum,
+                                 const int2* roi, const int* gf, int K, int G, int cols, float fw,
+                                 const double* beta, float2* part, const float2* tw, hipStream_t s) {
+  using C = WaveCfg<M>;
+  const int chunks = (G + MLS_PUM_FPC - 1) / MLS_PUM_FPC;
+  hipLaunchKernelGGL((k_pum_rows<M>), dim3(C::L / C::GPW, K - 1, chunks), dim3(PTYK_NT), 0, s, psi, op, illum, roi,
+                     gf, K, G, cols, fw, part, tw);
+  const long mm = (long)M * M;
+  hipLaunchKernelGGL(k_pum_finish, dim3((unsigned)((mm + PTYK_NT - 1) / PTYK_NT), K - 1), dim3(PTYK_NT), 0, s, probe,
+                     part, K, chunks, G, mm, beta);
+  return hipGetLastError();
+}
+
+hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
+                            const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
+                            const float2* tw, hipStream_t s) {
+  if (K < 2 || G < 1) return hipErrorInvalidValue;
+  switch (m) {
+    case 128: return mode_updates_m<128>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
+    case 256: return mode_updates_m<256>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
+    case 512: return mode_updates_m<512>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

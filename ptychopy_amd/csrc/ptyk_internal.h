@@ -108,9 +108,22 @@ int mls_modulus_blocks(int m, int k);
 // the separable ramp applied as row shifts then column shifts (m in {128, 256, 512}; ptyk_dm_wave.hip).
 hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,
                             const int* gf, float sign, const float2* tw, hipStream_t s);
+// c_modes: mode planes the inverse row pass covers (K, or 1 when modes 2..K are left in their
+// column-transformed state for mls_mode_updates).
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
-                              hipStream_t s);
+                              int c_modes, hipStream_t s);
+// Probe updates of modes 2..K (P_k += mean_j w_j chi_k,j conj(op_j) * beta) straight from the
+// column-transformed planes: the inverse row pass and the frame sum in one kernel over chunks of
+// MLS_PUM_FPC frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame
+// order (m in {128, 256, 512}; ptyk_dm_wave.hip).  beta: the mean probe step (re, im).
+constexpr int MLS_PUM_FPC = 4;
+inline long mls_pum_part_len(int g, int k, long mm) {
+  return (long)((g + MLS_PUM_FPC - 1) / MLS_PUM_FPC) * (k > 1 ? k - 1 : 1) * mm;
+}
+hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
+                            const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
+                            const float2* tw, hipStream_t s);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {
@@ -173,6 +186,7 @@ struct MlsDev {
   float2* dxb = nullptr;        // column-derivative images, gmax frames
   double* pos_part = nullptr;   // partials, gmax x (m / 16) x 4
   double* pos_ftot = nullptr;   // per-frame totals, gmax x 4
+  float2* pum_part = nullptr;   // chunk partials of the probe-mode updates (mls_pum_part_len)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };

@@ -1321,6 +1321,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     // with the three-pass exit waves the object patches stay in op, and the position step's
     // derivative images come from one-dimensional transforms of them
     const bool pos_1d_step = pos_step && mod_blocks && has_pos_1d(m) && d.side;
+    // modes 2..K only feed the fused probe-mode update: their inverse row pass is left to it
+    const bool defer_modes = probe_rec && !d.apply_multimodal_update && K > 1 && mod_blocks && d.pum_part;
     if (subpix && mod_blocks) {
       // the shifted illumination probes' spectra formed in the inverse row pass itself
       TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
@@ -1335,7 +1337,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       // from O, P and vp; the position step's patch copy takes its own forward transform
       const bool spec2d = pos_step && !pos_1d_step;
       const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
-      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, s));
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, defer_modes ? 1 : K, s));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
       if (pos_1d_step) {
         // the position step reads only chi0, vp and op, which stay fixed until the group ends,
@@ -1458,7 +1460,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         }
       }
     }
-    if (fuse_modes) {
+    if (defer_modes) {
+      TRY(mls_mode_updates(d.probe, d.psi, d.op, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
+                           d.pum_part, d.tw, s));
+    } else if (fuse_modes) {
       LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, m,
              d.cols, fw, d.gscal);
       CHK();
