@@ -1379,7 +1379,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       if (probe_rec) {
         if (ll == 1) {
           if (subpix && mod_blocks) {
-            // chi0 conj(op), its transform, the shift and the inverse in three passes
+            // chi0 conj(op) with its sub-pixel shift in two passes (row shifts, column shifts)
             TRY(mls_shift_passes(d.pu, chi0, d.op, G, m, d.sub, gf, -1.f, d.tw, s));
           } else {
             LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
