@@ -79,6 +79,8 @@ CONFIGS = [
     # position refinement at 256^2 and 512^2 (derivative spectra formed in the inverse row pass)
     dict(k=2, m=256, n_side=3, cfg=dict(probe_pos_search=2)),
     dict(k=1, m=512, n_side=3, cfg=dict(probe_pos_search=2, variable_probe=0)),
+    # groups of 9 and 7 frames: the probe-mode update of modes 2..K sums over several chunks
+    dict(k=3, m=128, n_side=4, group=9, cfg=dict(probe_pos_search=2)),
 ]
 
 
@@ -91,7 +93,7 @@ def test_mls_iterations_match_oracle(gpu, orc, case):
     okw = {("pps" if key == "probe_pos_search" else key): v for key, v in c["cfg"].items()}
     rng = np.random.default_rng(100 + case)
     for it in range(1, 5):
-        groups = mls_groups(P["n"], 4, rng)  # six groups of 4 and one of 1
+        groups = mls_groups(P["n"], c.get("group", 4), rng)  # e.g. six groups of 4 and one of 1
         o = _oracle_from_engine(e, orc, amp)
         o.iteration(orc.mls_params(P["box"], iter=it, **okw), groups)
         ferr = e.mls_iteration(it, groups)
