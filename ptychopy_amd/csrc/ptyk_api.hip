@@ -108,7 +108,7 @@ static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
-                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part};  // d.tmp and d.tmp2 point into d.psi
+                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot};  // tmp, tmp2 in psi
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (d.ev_fork) hipEventDestroy(d.ev_fork);
@@ -779,11 +779,12 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, hipEventCreateWithFlags(&d.ev_join, hipEventDisableTiming));
   }
   if (g > d.gmax) {
-    void* ptrs[] = {d.op, d.vp, d.psi, d.pu, d.gscal, d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part};
+    void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,
+                    d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot};
     for (void* p : ptrs)  // tmp and tmp2 live inside psi
       if (p) hipFree(p);
     d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = d.dxb = d.pum_part = nullptr;
-    d.gscal = d.ftot = d.pos_part = d.pos_ftot = nullptr;
+    d.gscal = d.ftot = d.pos_part = d.pos_ftot = d.vp_part = d.vp_ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
     const long tcap = std::max<long>(g, d.k);
@@ -801,6 +802,8 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.pos_part, (size_t)4 * g * (d.m / 16)));
     HIPCHK(c, dalloc(&d.pos_ftot, (size_t)4 * g));
     HIPCHK(c, dalloc(&d.pum_part, (size_t)mls_pum_part_len(g, d.k, mm)));
+    HIPCHK(c, dalloc(&d.vp_part, (size_t)g * ch + 2 * nb));
+    HIPCHK(c, dalloc(&d.vp_ftot, (size_t)8 * g));
     d.gmax = g;
   }
   const long need = std::max<long>({6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L});

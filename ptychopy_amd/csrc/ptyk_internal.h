@@ -187,6 +187,8 @@ struct MlsDev {
   double* pos_part = nullptr;   // partials, gmax x (m / 16) x 4
   double* pos_ftot = nullptr;   // per-frame totals, gmax x 4
   float2* pum_part = nullptr;   // chunk partials of the probe-mode updates (mls_pum_part_len)
+  double* vp_part = nullptr;    // variable-probe update partials on the second stream
+  double* vp_ftot = nullptr;    // its per-frame totals, gmax
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };

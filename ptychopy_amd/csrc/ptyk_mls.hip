@@ -1371,6 +1371,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const int ll_range = std::max(K, d.n_objects);
     // modes 2..K take one fused pass when no object update runs for them
     const bool fuse_modes = probe_rec && !d.apply_multimodal_update && K > 1 && K <= 16;
+    // the variable-mode update on the second stream; the 2D position step (which rewrites tmp,
+    // the update's scratch) rules it out
+    const bool vp_side = var_update && d.variable_probe && probe_rec && d.side && d.vp_part && mod_blocks &&
+                         subpix && (!pos_step || pos_1d_step);
     for (int ll = 1; ll <= ll_range; ++ll) {
       if (fuse_modes && ll > 1) break;
       const bool obj_rec = obj_rec_iter && (d.apply_multimodal_update || ll <= d.n_objects);
@@ -1392,6 +1396,27 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
             }
           }
           LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
+          if (vp_side) {
+            // the variable-mode update needs only pu, pum, the illumination and V, and writes
+            // only V, its own scratch and the coefficients: it runs on the second stream beside
+            // the object direction, the LSQ steps and their application
+            CHK();
+            TRY(hipEventRecord(d.ev_fork, s));
+            TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
+            const int nb = (mm + NT - 1) / NT;
+            double* p2 = d.vp_part + (long)G * ch;  // |acc|^2 partials
+            double* p3 = p2 + nb;                   // |V|^2 partials
+            hipLaunchKernelGGL(k_vp_resid, gch, dim3(NT), 0, d.side, d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m,
+                               d.cols, d.vmode, d.gscal, d.vp_part);
+            hipLaunchKernelGGL(k_vp_coef, dim3(1), dim3(1024), 0, d.side, d.vp_part, ch, gf, G, mm, d.evo, d.n,
+                               d.vp_ftot, d.gscal);
+            hipLaunchKernelGGL(k_vp_acc, dim3(nb), dim3(NT), 0, d.side, d.acc, d.tmp, G, mm, d.gscal, p2);
+            hipLaunchKernelGGL(k_vp_apply, dim3(nb), dim3(NT), 0, d.side, d.vmode, d.acc, mm, p2, nb,
+                               (double)G / d.n, p3);
+            hipLaunchKernelGGL(k_vp_norm, dim3(nb), dim3(NT), 0, d.side, d.vmode, mm, p3, nb);
+            TRY(hipGetLastError());
+            TRY(hipEventRecord(d.ev_join, d.side));
+          }
         } else {
           LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, chi, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 1);
         }
@@ -1437,15 +1462,19 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       }
       if (var_update && ll == 1) {
         if (d.variable_probe) {
-          const int nb = (mm + NT - 1) / NT;
-          double* p2 = d.part + (long)G * ch;  // |acc|^2 partials
-          double* p3 = p2 + nb;                // |V|^2 partials
-          LAUNCH(k_vp_resid, gch, dim3(NT), d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m, d.cols, d.vmode, d.gscal,
-                 d.part);
-          LAUNCH(k_vp_coef, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, d.ftot, d.gscal);
-          LAUNCH(k_vp_acc, dim3(nb), dim3(NT), d.acc, d.tmp, G, mm, d.gscal, p2);
-          LAUNCH(k_vp_apply, dim3(nb), dim3(NT), d.vmode, d.acc, mm, p2, nb, (double)G / d.n, p3);
-          LAUNCH(k_vp_norm, dim3(nb), dim3(NT), d.vmode, mm, p3, nb);
+          if (vp_side) {
+            TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // V updated on the second stream
+          } else {
+            const int nb = (mm + NT - 1) / NT;
+            double* p2 = d.part + (long)G * ch;  // |acc|^2 partials
+            double* p3 = p2 + nb;                // |V|^2 partials
+            LAUNCH(k_vp_resid, gch, dim3(NT), d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m, d.cols, d.vmode, d.gscal,
+                   d.part);
+            LAUNCH(k_vp_coef, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, d.ftot, d.gscal);
+            LAUNCH(k_vp_acc, dim3(nb), dim3(NT), d.acc, d.tmp, G, mm, d.gscal, p2);
+            LAUNCH(k_vp_apply, dim3(nb), dim3(NT), d.vmode, d.acc, mm, p2, nb, (double)G / d.n, p3);
+            LAUNCH(k_vp_norm, dim3(nb), dim3(NT), d.vmode, mm, p3, nb);
+          }
           // with variable intensity too, both evolution updates take one pass (neither changes
           // op, chi, the variable mode or the probe, and they write disjoint halves of evo)
           const float2* page2 = d.variable_intensity ? d.probe : nullptr;
@@ -1468,7 +1497,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
              d.cols, fw, d.gscal);
       CHK();
     }
-    if (pos_1d_step) TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // before the next group rewrites op, vp, psi
+    if (pos_1d_step || vp_side) TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // before the next group's writes
   }
   if (K > d.n_rec) {
     LAUNCH(k_ortho, dim3(1), dim3(1024), d.probe, d.tmp, K, mm);
