@@ -1373,8 +1373,9 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const bool fuse_modes = probe_rec && !d.apply_multimodal_update && K > 1 && K <= 16;
     // the variable-mode update on the second stream; the 2D position step (which rewrites tmp,
     // the update's scratch) rules it out
+    // (and with later modes' own probe-update passes, which rewrite pum, ruled out too)
     const bool vp_side = var_update && d.variable_probe && probe_rec && d.side && d.vp_part && mod_blocks &&
-                         subpix && (!pos_step || pos_1d_step);
+                         subpix && (!pos_step || pos_1d_step) && (fuse_modes || std::max(K, d.n_objects) == 1);
     for (int ll = 1; ll <= ll_range; ++ll) {
       if (fuse_modes && ll > 1) break;
       const bool obj_rec = obj_rec_iter && (d.apply_multimodal_update || ll <= d.n_objects);
@@ -1461,10 +1462,20 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         CHK();
       }
       if (var_update && ll == 1) {
-        if (d.variable_probe) {
-          if (vp_side) {
-            TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // V updated on the second stream
-          } else {
+        if (d.variable_probe && vp_side) {
+          // the evolution dots need the updated V (second stream, in order) and, with variable
+          // intensity, the updated probe (this stream): they follow on the second stream, beside
+          // the probe-mode update, which touches neither evo, V nor probe mode 1
+          TRY(hipEventRecord(d.ev_fork, s));
+          TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
+          const float2* page2 = d.variable_intensity ? d.probe : nullptr;
+          hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, d.side, d.op, d.vmode, page2, chi0, m, d.part);
+          hipLaunchKernelGGL(k_evo_update, dim3(1), dim3(1024), 0, d.side, d.part, ch, gf, G, mm, d.evo, d.n,
+                             page2 ? 2 : 1, d.ftot);
+          TRY(hipGetLastError());
+          TRY(hipEventRecord(d.ev_join, d.side));
+        } else if (d.variable_probe) {
+          {
             const int nb = (mm + NT - 1) / NT;
             double* p2 = d.part + (long)G * ch;  // |acc|^2 partials
             double* p3 = p2 + nb;                // |V|^2 partials
