@@ -1,19 +1,18 @@
 #!/usr/bin/env python
 """Benchmark: reconstruction throughput, diffraction frames/s per iteration.
 
-Default workload `dm_c1` (the reference simulation shape): DM on a 30 x 30 raster scan,
-512 x 512 detector (lambda 0.248 nm, z 1 m, 172 um pixels => 2.82 nm object pixels, 50 nm =
-17.76 px steps), 1088 x 1088 random-field object (1027 x 1027 reconstruction canvas), 110 nm
-Gaussian probe, 1e7 photons/frame Poisson, one probe mode.  A step is one full DM iteration
-over all 900 frames with inputs resident in HBM.
+Default workload `mls_c2` (the configuration the headline metric is quoted on): MLs on a
+51 x 51 raster (100 nm = 14 px steps), 256 x 256 frames, five incoherent Hermite-Gaussian probe
+modes (power 0.6/0.2/0.1/0.06/0.04), 1e5 photons/frame, FP16 amplitudes, delta_p = 0.1; a step is
+one MLs iteration over all 2601 frames (101 groups of ~26 frames) with inputs resident in HBM.
+Every timed step runs the full iteration: probe and object updates, variable probe and
+intensity, sub-pixel shifts and position refinement (the timed iterations are numbered from the
+position-search iteration on, whatever --warmup is).
 
-`--workload mls_c2`: MLs on a 51 x 51 raster (100 nm = 14 px steps), 256 x 256 frames, five
-incoherent Hermite-Gaussian probe modes (power 0.6/0.2/0.1/0.06/0.04), 1e5 photons/frame,
-FP16 amplitudes; a step is one MLs iteration (101 groups of ~26 frames, position refinement
-on from iteration 20).
-
-Prints one JSON line.
-
+`--workload dm_c1`: DM on a 30 x 30 raster, 512 x 512 detector (lambda 0.248 nm, z 1 m, 172 um
+pixels => 2.82 nm object pixels, 50 nm = 17.76 px steps), 1088 x 1088 random-field object
+(1027 x 1027 reconstruction canvas), 110 nm Gaussian probe, 1e7 photons/frame, one mode; every
+timed step has the probe update on and the phase constraint off.
 `--workload epie_c0`: ePIE, 12 x 12 raster, 128 x 128 frames (positions/s; a dependent chain).
 `--workload batch_c3`: 16 DM scans of uneven size advanced in one captured HIP graph per
 iteration (and, beside it, one after another).
@@ -21,7 +20,9 @@ iteration (and, beside it, one after another).
 MLs on the mls_c2 workload from a uniform and from the stitched start to a target normalised
 object error (minutes; not a throughput line).
 
-  python bench.py [--gpus 1] [--workload dm_c1|mls_c2|epie_c0|batch_c3|c4] [--steps K] [--warmup W] [--no-cpu]
+Prints one JSON line.
+
+  python bench.py [--gpus 1] [--workload mls_c2|dm_c1|epie_c0|batch_c3|c4] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -43,15 +44,23 @@ KERNELS = ("row_pass", "column_pass", "inverse_row_pass", "probe_gather", "objec
            "object_apply", "finish")
 
 
-def dm_flags(i: int) -> int:
-    """IPhaser::phaseStep with the Python defaults phaseConstraint=1, updateProbe=10."""
+def dm_flags(i: int, timed: bool = False) -> int:
+    """IPhaser::phaseStep with the Python defaults phaseConstraint=1, updateProbe=10 for the
+    warm-up; every timed step has the probe update on and the phase constraint off, whatever
+    its iteration number."""
     from ptychopy_amd import _engine as E
+    if timed:
+        return E.FLAG_UPDATE_PROBE
     f = 0
     if i <= 1:
         f |= E.FLAG_PHASE_CONSTRAINT
     if i >= 10:
         f |= E.FLAG_UPDATE_PROBE
     return f
+
+
+def host_info() -> dict:
+    return {"nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def kernel_bytes(name: str, n: int, m: int, obj_px: int) -> float:
@@ -101,7 +110,7 @@ def cpu_baseline(wl: dict, frames: int, min_seconds: float = 10.0) -> dict:
             break
     return {"value": s * iters / dt, "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
             "sample": f"{iters} FP64 oracle DM iterations (probe update on) over the first {s} of {wl['n']} "
-                      f"frames, 512x512, same 1027x1027 canvas; {dt:.1f} s on {cpu_model()}"}
+                      f"frames, 512x512, same 1027x1027 canvas; {dt:.1f} s on {cpu_model()}", **host_info()}
 
 
 def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: int) -> dict:
@@ -117,7 +126,8 @@ def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: in
     dt = time.perf_counter() - t0
     return {"value": frames / dt, "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
             "sample": f"FP64 oracle MLs iteration {it} over its first 2 groups ({frames} of {W['n']} frames; "
-                      f"whole-canvas steps included), 256x256, 5 modes; {dt:.1f} s on {cpu_model()}"}
+                      f"whole-canvas steps included), 256x256, 5 modes; {dt:.1f} s on {cpu_model()}",
+            **host_info()}
 
 
 def _base_line(args, metric, value, unit, ms, dtype, data, config) -> dict:
@@ -153,9 +163,9 @@ def run_epie(args) -> dict:
     torch.cuda.synchronize()
     orders = [rng.permutation(n) for _ in range(args.steps)]
     t0 = time.perf_counter()
-    for o in orders:
+    for o in orders:  # every timed iteration updates the probe, whatever --warmup is
         it += 1
-        e.epie_iteration(o, flags(it))
+        e.epie_iteration(o, E.FLAG_UPDATE_PROBE)
     e.synchronize()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
@@ -179,7 +189,7 @@ def run_epie(args) -> dict:
                 break
         out["cpu_baseline"] = {"value": n * iters / dt, "unit": "positions/s", "cores": oracle.num_threads(),
                                "kind": "port", "sample": f"{iters} FP64 oracle ePIE iterations over all {n} positions; "
-                                                         f"{dt:.1f} s on {cpu_model()}"}
+                                                         f"{dt:.1f} s on {cpu_model()}", **host_info()}
     e.close()
     return out
 
@@ -276,7 +286,9 @@ def run_mls(args) -> dict:
     vmode = (rng.uniform(0, 1, (m, m)) * np.exp(2j * np.pi * rng.uniform(0, 1, (m, m)))).astype(np.complex64)
     evo1 = 1e-6 * rng.standard_normal(n)
     fp = (amp_deq.astype(np.float64) ** 2).sum(axis=(1, 2))
-    e.mls_begin(E.mls_config(W["box"]), W["pos"], vmode, evo1, fp)
+    cfg = E.mls_config(W["box"])
+    e.mls_begin(cfg, W["pos"], vmode, evo1, fp)
+    pps = int(cfg.probe_pos_search)
     parts = _driver.mls_partitions(n, 100, 1)
     n_groups = len(parts[0])
 
@@ -288,6 +300,9 @@ def run_mls(args) -> dict:
     for _ in range(args.warmup):
         it += 1
         e.mls_iteration(it, groups(it))
+    # timed iterations are numbered from the position-search iteration on: every one runs the
+    # position refinement, the variable-probe and intensity updates and no Fourier errors
+    it = max(it, pps - 1)
     e.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -297,6 +312,20 @@ def run_mls(args) -> dict:
     e.synchronize()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # per-kind launch times from HIP events on the engine stream (extra iterations of the same kind)
+    kt = {}
+    for _ in range(args.kernel_iters):
+        it += 1
+        for key, v in e.mls_iteration_timed(it, groups(it)).items():
+            kt[key] = kt.get(key, 0.0) + v / args.kernel_iters
+    streamed = {key: v for key, v in kt.items() if workloads.mls_kind_bytes(key, m, k, n, n_groups) > 0}
+    dom = max(streamed, key=streamed.get)
+    dom_bytes = workloads.mls_kind_bytes(dom, m, k, n, n_groups)
+    ach = dom_bytes / (kt[dom] * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "mls_c2_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("kinds", {}).get(dom, {}).get("traffic")
     bpf = workloads.mls_bytes_per_frame(m, rows * cols, n, n_groups)
     fpf = workloads.mls_flops_per_frame(m, k)
     gbs = bpf * n / (ms * 1e-3) / 1e9
@@ -315,17 +344,24 @@ def run_mls(args) -> dict:
         "dtype": "c64",
         "data": "synthetic (seeded random-field object, 5 Hermite-Gaussian modes, Poisson 1e5 photons/frame)",
         "config": {"workload": "MLs 51x51 raster, 256x256 frames, 5 probe modes, FP16 amplitudes, "
-                               f"{n_groups} groups, position refinement from iteration 20",
-                   "frames": n, "frame": m, "canvas": [rows, cols], "modes": k},
-        "roofline": {"bound": "hbm", "kernel": "iteration", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_frame": bpf},
+                               f"{n_groups} groups, probe/object/variable-probe updates and position refinement "
+                               "in every timed iteration",
+                   "frames": n, "frame": m, "canvas": [rows, cols], "modes": k, "groups": n_groups,
+                   "timed_iterations": [it - args.kernel_iters - args.steps + 1, it - args.kernel_iters],
+                   "position_refinement": True, "fourier_errors": False},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": dom_bytes,
+                     "launches": n_groups},
+        "iteration_roofline": {"bytes_per_frame": bpf, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": gbs / HBM_PEAK_GBS},
         "flop_roofline": {"flops_per_frame": fpf, "achieved": tfs, "peak": 157.3, "unit": "TFLOP/s",
                           "frac": tfs / 157.3},
+        "kernel_ms": kt,
         "setup_s": t_gen,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = mls_cpu_baseline(W, amp_deq, W["probe_true"], obj0, vmode, evo1, groups(it + 1),
-                                               max(it + 1, 21))
+                                               max(it + 1, pps + 1))
     e.close()
     return out
 
@@ -333,14 +369,14 @@ def run_mls(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("dm_c1", "mls_c2", "epie_c0", "batch_c3", "c4"), default="dm_c1")
+    ap.add_argument("--workload", choices=("mls_c2", "dm_c1", "epie_c0", "batch_c3", "c4"), default="mls_c2")
     ap.add_argument("--c4-train", type=int, default=50000, help="c4: training frames")
     ap.add_argument("--c4-held", type=int, default=5000, help="c4: held-out frames")
     ap.add_argument("--c4-epochs", type=int, default=2, help="c4: training epochs")
     ap.add_argument("--c4-target", type=float, default=1e-2, help="c4: normalised object error target")
     ap.add_argument("--c4-iters", type=int, default=100, help="c4: MLs iteration limit per start")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 for dm_c1, 10 for mls_c2)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 10 for dm_c1, 20 for mls_c2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 10 for dm_c1, 5 for mls_c2)")
     ap.add_argument("--cpu-frames", type=int, default=900)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
@@ -394,7 +430,7 @@ def main() -> int:
         return 0
     if args.workload == "mls_c2":
         args.steps = 10 if args.steps is None else args.steps
-        args.warmup = 20 if args.warmup is None else args.warmup
+        args.warmup = 5 if args.warmup is None else args.warmup
         print(json.dumps(run_mls(args)))
         return 0
     args.steps = 20 if args.steps is None else args.steps
@@ -429,7 +465,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for _ in range(args.steps):
         it += 1
-        e.dm_iteration(dm_flags(it))
+        e.dm_iteration(dm_flags(it, timed=True))
     e.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -439,7 +475,7 @@ def main() -> int:
     kt = np.zeros(len(KERNELS))
     for _ in range(args.kernel_iters):
         it += 1
-        kt += e.dm_iteration_timed(dm_flags(it))
+        kt += e.dm_iteration_timed(dm_flags(it, timed=True))
     kt /= args.kernel_iters
     obj_px = rows * cols
     dom = int(np.argmax(kt))
@@ -467,8 +503,9 @@ def main() -> int:
         "dtype": "c64",
         "data": "synthetic (seeded random-field object, Gaussian probe, Poisson 1e7 photons/frame)",
         "config": {"workload": "DM 30x30 raster, 512x512 frames, 1088x1088 object (1027x1027 canvas), "
-                               "1 probe mode, FP16 amplitudes, probe update on",
-                   "frames": n, "frame": m, "canvas": [rows, cols], "modes": 1},
+                               "1 probe mode, FP16 amplitudes, probe update on in every timed iteration",
+                   "frames": n, "frame": m, "canvas": [rows, cols], "modes": 1, "probe_update": True,
+                   "phase_constraint": False},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": kernel_bytes(name, n, m, obj_px)},

@@ -35,7 +35,7 @@ EXPORTS = (
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
     "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
-    "ptyk_mls_iteration", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
+    "ptyk_mls_iteration", "ptyk_mls_iteration_timed", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
     "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
     "ptyk_net_last_error", "ptyk_net_num_params", "ptyk_net_get_params", "ptyk_net_set_params",
@@ -133,6 +133,7 @@ def load():
     lib.ptyk_get_tuning.argtypes = [P, i32, C.POINTER(C.c_int32)]
     lib.ptyk_mls_begin.argtypes = [P, C.POINTER(MlsConfig), P, P, P, P]
     lib.ptyk_mls_iteration.argtypes = [P, i32, P, P, i32, P]
+    lib.ptyk_mls_iteration_timed.argtypes = [P, i32, P, P, i32, P, P]
     lib.ptyk_mls_get_state.argtypes = [P, C.POINTER(MlsState)]
     lib.ptyk_mls_set_state.argtypes = [P, C.POINTER(MlsState)]
     lib.ptyk_set_true_object.argtypes = [P, P]
@@ -170,6 +171,18 @@ def load():
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _group_tables(groups):
+    gf = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32) for g in groups]), dtype=np.int32)
+    go = np.zeros(len(groups) + 1, np.int32)
+    go[1:] = np.cumsum([len(g) for g in groups])
+    return gf, go
+
+
+# launch kinds of ptyk_mls_iteration_timed, in order (include/ptyk.h)
+MLS_KINDS = ("iteration_steps", "illumination_probes", "exit_row_pass", "modulus_column_pass", "inverse_row_pass",
+             "probe_update", "object_direction", "lsq", "step_application", "probe_modes", "side_stream_wait")
 
 
 class Engine:
@@ -353,12 +366,22 @@ class Engine:
 
     def mls_iteration(self, it: int, groups):
         """groups: sequence of frame-index arrays in processing order; returns Fourier errors."""
-        gf = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32) for g in groups]), dtype=np.int32)
-        go = np.zeros(len(groups) + 1, np.int32)
-        go[1:] = np.cumsum([len(g) for g in groups])
+        gf, go = _group_tables(groups)
         ferr = np.zeros(self.n, np.float32)
         self._chk(self.lib.ptyk_mls_iteration(self.h, int(it), _ptr(gf), _ptr(go), len(groups), _ptr(ferr)))
         return ferr
+
+    def mls_iteration_timed(self, it: int, groups):
+        """The same iteration with HIP-event timing of its main-stream launches: a dict of ms per
+        launch kind (MLS_KINDS) plus 'host_enqueue' (host time to enqueue the iteration)."""
+        gf, go = _group_tables(groups)
+        ferr = np.zeros(self.n, np.float32)
+        ms = np.zeros(len(MLS_KINDS) + 1, np.float32)
+        self._chk(self.lib.ptyk_mls_iteration_timed(self.h, int(it), _ptr(gf), _ptr(go), len(groups), _ptr(ferr),
+                                                    _ptr(ms)))
+        out = {k: float(v) for k, v in zip(MLS_KINDS, ms)}
+        out["host_enqueue"] = float(ms[-1])
+        return out
 
     def mls_get_state(self) -> dict:
         n, m = self.n, self.m

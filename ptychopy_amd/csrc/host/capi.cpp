@@ -163,15 +163,16 @@ int ptycho_phaser_get_errors(ptycho_phaser* h, double* out, int32_t n) {
 
 int ptycho_phaser_get_fourier_errors(ptycho_phaser* h, uint32_t iter, float* out, int64_t n) {
   if (!h || !out || iter == 0) return -1;
-  for (auto& pm : h->ph.methods) {
-    auto* ml = dynamic_cast<ptycho::MLSMethod*>(pm.method.get());
-    if (!ml) continue;
-    if (iter > ml->fourierErrors.size() || (int64_t)ml->fourierErrors[iter - 1].size() != n) return -1;
-    std::copy(ml->fourierErrors[iter - 1].begin(), ml->fourierErrors[iter - 1].end(), out);
-    return 0;
+  bool ml = false;
+  for (auto& pm : h->ph.methods) ml = ml || dynamic_cast<ptycho::MLSMethod*>(pm.method.get()) != nullptr;
+  if (!ml) {
+    h->ph.err = "no MLs method";
+    return -1;
   }
-  h->ph.err = "no MLs method";
-  return -1;
+  const auto& fe = h->ph.fourierErrors;
+  if (iter > fe.size() || (int64_t)fe[iter - 1].size() != n) return -1;
+  std::copy(fe[iter - 1].begin(), fe[iter - 1].end(), out);
+  return 0;
 }
 
 void* ptycho_phaser_engine(ptycho_phaser* h) { return h ? (void*)h->ph.eng.ctx : nullptr; }

@@ -335,14 +335,16 @@ bool SpiralScanMesh::generate(const Parameters& prm, std::string* err) {
 }
 
 // ---------------------------------------------------------------- methods
-int EPIE::iteration(Engine& eng, const ScanMesh&, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
-                    unsigned, bool rms, double* err) {
+real_t EPIE::iteration(Diffractions* diffs, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >&,
+                       const std::vector<float2>&, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
+                       unsigned int, bool RMS) {
   // random visiting order: repeatedly take a random one of the remaining positions
   // (ePIE.cpp:80-97, rand() % remaining; seeded here)
-  std::vector<int32_t> remaining(eng.n);
-  for (int i = 0; i < eng.n; ++i) remaining[i] = i;
-  order_.resize(eng.n);
-  for (int s = 0; s < eng.n; ++s) {
+  const int n = diffs->n;
+  std::vector<int32_t> remaining(n);
+  for (int i = 0; i < n; ++i) remaining[i] = i;
+  order_.resize(n);
+  for (int s = 0; s < n; ++s) {
     const size_t k = (size_t)(rng_() % remaining.size());
     order_[s] = remaining[k];
     remaining.erase(remaining.begin() + k);
@@ -351,30 +353,37 @@ int EPIE::iteration(Engine& eng, const ScanMesh&, bool phaseConstraint, bool upd
   if (phaseConstraint) flags |= PTYK_FLAG_PHASE_CONSTRAINT;
   if (updateProbe) flags |= PTYK_FLAG_UPDATE_PROBE;
   if (updateProbeModes) flags |= PTYK_FLAG_UPDATE_MODES;
-  if (rms) flags |= PTYK_FLAG_RMS;
-  return ptyk_epie_iteration(eng.ctx, order_.data(), flags, err);
+  if (RMS) flags |= PTYK_FLAG_RMS;
+  double err = 0.0;
+  status_ = ptyk_epie_iteration(diffs->ctx, order_.data(), flags, &err);
+  return (real_t)err;
 }
 
-int DMMethod::iteration(Engine& eng, const ScanMesh&, bool phaseConstraint, bool updateProbe, bool, unsigned,
-                        bool, double* err) {
+real_t DMMethod::iteration(Diffractions* diffs, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >&,
+                           const std::vector<float2>&, bool phaseConstraint, bool updateProbe, bool, unsigned int,
+                           bool) {
   uint32_t flags = 0;
   if (phaseConstraint) flags |= PTYK_FLAG_PHASE_CONSTRAINT;
   if (updateProbe) flags |= PTYK_FLAG_UPDATE_PROBE;
-  if (err) *err = 0.0;  // DM::iteration returns 0 (DM.cu:127)
-  return ptyk_dm_iteration(eng.ctx, flags);
+  status_ = ptyk_dm_iteration(diffs->ctx, flags);
+  return 0;  // DM::iteration returns 0 (DM.cu:127)
 }
 
-void MLSMethod::initMem(const ScanMesh& mesh, int) {
-  n_ = (int64_t)mesh.positions.size();
+void MLSMethod::initMem(IPtychoScanMesh* scanMesh, uint2) {
+  n_ = (int64_t)scanMesh->positions.size();
   ptycho_mls_partitions(n_, niter_, seed_ + 101, nullptr, nullptr, &groups_, &parts_);
   frames_.assign((size_t)parts_ * n_, 0);
   offsets_.assign((size_t)parts_ * (groups_ + 1), 0);
   ptycho_mls_partitions(n_, niter_, seed_ + 101, frames_.data(), offsets_.data(), &groups_, &parts_);
-  fourierErrors.clear();
 }
 
-int MLSMethod::iteration(Engine& eng, const ScanMesh&, bool, bool, bool, unsigned iter, bool, double* err) {
-  if (parts_ == 0) return PTYK_ESTATE;
+real_t MLSMethod::iteration(Diffractions* diffs, Probe*, Sample*, IPtychoScanMesh*,
+                            std::vector<std::vector<real_t> >& fourierErrors, const std::vector<float2>&, bool, bool,
+                            bool, unsigned int iter, bool) {
+  if (parts_ == 0) {
+    status_ = PTYK_ESTATE;
+    return 1;
+  }
   // randNoIndex = rand_gen_no(1, miniter) - 1 and random_shuffle of the group order (MLS.cpp:106-113)
   const int32_t part = (int32_t)(rng_() % (uint64_t)parts_);
   std::vector<int32_t> order(groups_);
@@ -388,17 +397,16 @@ int MLSMethod::iteration(Engine& eng, const ScanMesh&, bool, bool, bool, unsigne
     lastGroups.insert(lastGroups.end(), fr + of[g], fr + of[g + 1]);
     lastOffsets.push_back((int32_t)lastGroups.size());
   }
+  // fourierErrors[iterC].resize(diffSize) (MLS.cpp:84); the caller sizes the outer vector
   std::vector<float> fe((size_t)n_, 0.f);
-  const int rc = ptyk_mls_iteration(eng.ctx, (int32_t)iter, lastGroups.data(), lastOffsets.data(), groups_, fe.data());
+  status_ = ptyk_mls_iteration(diffs->ctx, (int32_t)iter, lastGroups.data(), lastOffsets.data(), groups_, fe.data());
   if (iter >= 1) {
     if (fourierErrors.size() < iter) fourierErrors.resize(iter);
-    fourierErrors[iter - 1] = fe;
+    fourierErrors[iter - 1].assign(fe.begin(), fe.end());
   }
-  if (err) *err = 1.0;  // MLS::iteration returns 1 (MLS.cpp:489)
-  return rc;
+  return 1;  // MLS::iteration returns 1 (MLS.cpp:489)
 }
 
-// ---------------------------------------------------------------- IPhaser
 bool IPhaser::simulate_frames(std::vector<float>* amp) {
   // Diffractions::simulate (Diffractions.cpp:80-116) over a seeded random-phase object of the
   // canvas size with the simulated probe (the reference reads lena/baboon PGM files here)
@@ -502,6 +510,7 @@ bool IPhaser::init() {
   eng.k = cfg.n_modes;
   eng.rows = cfg.obj_rows;
   eng.cols = cfg.obj_cols;
+  eng.bind();
   if (!mls) {  // MLs positions stay centred; their ROIs are set up by ptyk_mls_begin
     std::vector<float> pos(2 * (size_t)n);
     for (int q = 0; q < n; ++q) {
@@ -629,7 +638,8 @@ bool IPhaser::prePhase() {
     }
   }
   errors.assign(prm.r.iterations, 0.0);
-  for (auto& pm : methods) pm.method->initMem(*mesh, m);
+  fourierErrors.assign(prm.r.iterations, std::vector<real_t>());  // IPhaser.cpp:341
+  for (auto& pm : methods) pm.method->initMem(mesh.get(), uint2{(unsigned)m, (unsigned)m});
   return true;
 }
 
@@ -911,15 +921,17 @@ bool IPhaser::mls_prephase() {
     return false;
   }
   errors.assign(prm.r.iterations, 0.0);
-  for (auto& pm : methods) pm.method->initMem(*mesh, m);
+  fourierErrors.assign(prm.r.iterations, std::vector<real_t>());  // IPhaser.cpp:341
+  for (auto& pm : methods) pm.method->initMem(mesh.get(), uint2{(unsigned)m, (unsigned)m});
   return true;
 }
 
-bool IPhaser::phaseStep(IPhasingMethod* m, unsigned i, double* e) {
+bool IPhaser::phaseStep(IPhasingMethod* m, unsigned i, real_t* e) {
   // IPhaser::phaseStep (IPhaser.cpp:391-398)
-  const int rc = m->iteration(eng, *mesh, i <= prm.r.phaseConstraint, i >= prm.r.updateProbe,
-                              i >= prm.r.updateProbeModes, i, prm.r.calculateRMS, e);
-  if (rc != PTYK_OK) {
+  *e = m->iteration(&eng.diffractions, &eng.probe, &eng.sample, mesh.get(), fourierErrors, mesh->positions,
+                    i <= prm.r.phaseConstraint, i >= prm.r.updateProbe, i >= prm.r.updateProbeModes, i,
+                    prm.r.calculateRMS);
+  if (m->status() != PTYK_OK) {
     err = ptyk_last_error(eng.ctx);
     return false;
   }
@@ -941,7 +953,7 @@ bool IPhaser::phase() {
   // IPhaser::phaseLoop (IPhaser.cpp:367-389)
   for (auto& pm : methods)
     for (unsigned i = 1; i <= pm.iterations; ++i) {
-      double e = 0.0;
+      real_t e = 0;
       if (!phaseStep(pm.method.get(), i, &e)) return false;
       if (i - 1 < errors.size()) errors[i - 1] += e;
       if (i % prm.r.save == 0) writeResultsToDisk((int)(i / prm.r.save));
@@ -963,7 +975,7 @@ bool IPhaser::phaseinit() {
 
 bool IPhaser::phasestepvis(unsigned i) {
   for (auto& pm : methods) {
-    double e = 0.0;
+    real_t e = 0;
     if (!phaseStep(pm.method.get(), i, &e)) return false;
     if (i >= 1 && i - 1 < errors.size()) errors[i - 1] += e;
     if (prm.r.time >= 0 && now_s() - t0_ >= prm.r.time) break;

@@ -117,32 +117,80 @@ class SpiralScanMesh : public ScanMesh {
 };
 
 // ---------------------------------------------------------------- physics objects
-// Frames, probe and object live in the engine handle; these views keep the reference's
-// class roles (Diffractions.h, Probe.h, Sample.h).
+// The seam's value types: datatypes.h's real_t (float) and CUDA's float2 / uint2.
+typedef float real_t;
+typedef F2 float2;
+struct uint2 {
+  unsigned x, y;
+};
+typedef ScanMesh IPtychoScanMesh;
+
+// Views with the reference's class roles (Diffractions.h, Probe.h, Sample.h) over one engine
+// handle: the frame store, the probe modes and the object canvas are device arrays of the
+// ptyk_ctx, and a method reaches them through ctx.  The views own nothing.
+class Diffractions {
+ public:
+  ptyk_ctx* ctx = nullptr;
+  int n = 0, m = 0;  // N frames of M x M
+};
+class Probe {
+ public:
+  ptyk_ctx* ctx = nullptr;
+  int m = 0, modes = 1;
+};
+class Sample {
+ public:
+  ptyk_ctx* ctx = nullptr;
+  int rows = 0, cols = 0;
+};
+
 struct Engine {
   ptyk_ctx* ctx = nullptr;
   int n = 0, m = 0, k = 1, rows = 0, cols = 0;
+  Diffractions diffractions;
+  Probe probe;
+  Sample sample;
+  void bind() {
+    diffractions.ctx = ctx, diffractions.n = n, diffractions.m = m;
+    probe.ctx = ctx, probe.m = m, probe.modes = k;
+    sample.ctx = ctx, sample.rows = rows, sample.cols = cols;
+  }
   ~Engine() {
     if (ctx) ptyk_destroy(ctx);
   }
 };
 
+// IPhasingMethod (IPhasingMethod.h:53-69), pure virtuals with the reference's signatures.
+// initMem sees the scan mesh and the probe size once before the first iteration; iteration
+// returns what the reference's method returns (ePIE: the averaged Fourier error when RMS is
+// set, else 0; DM: 0; MLs: 1) and MLs writes fourierErrors[iter - 1] (MLS.cpp:84, 236).
+// The reference exit()s on a failed CUDA call; here a method records the engine's status
+// code instead (status(), PTYK_OK on success), which IPhaser::phaseStep checks.
 class IPhasingMethod {
  public:
-  virtual ~IPhasingMethod() = default;
-  virtual void initMem(const ScanMesh&, int probe_size) {}
-  // One iteration (IPhasingMethod::iteration, IPhasingMethod.h:53-69); returns the error
-  // the reference method returns (ePIE: averaged RMS when requested; DM: 0).
-  virtual int iteration(Engine& eng, const ScanMesh& mesh, bool phaseConstraint, bool updateProbe,
-                        bool updateProbeModes, unsigned iter, bool rms, double* err) = 0;
-  virtual void endPhasing() {}
+  IPhasingMethod() {}
+  virtual ~IPhasingMethod() {}
+  virtual void initMem(IPtychoScanMesh* scanMesh, uint2 probeSize) = 0;
+  virtual real_t iteration(Diffractions*, Probe*, Sample*, IPtychoScanMesh*,
+                           std::vector<std::vector<real_t> >& fourierErrors, const std::vector<float2>& scanPositions,
+                           bool phaseConstraint, bool updateProbe, bool updateProbeModes, unsigned int iter,
+                           bool RMS = false) = 0;
+  virtual void endPhasing() = 0;
+  int status() const { return status_; }
+
+ protected:
+  int status_ = PTYK_OK;
 };
 
+// ePIE (ePIE.h:55-70)
 class EPIE : public IPhasingMethod {
  public:
   explicit EPIE(uint64_t seed) : rng_(seed) {}
-  int iteration(Engine& eng, const ScanMesh& mesh, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
-                unsigned iter, bool rms, double* err) override;
+  void initMem(IPtychoScanMesh*, uint2) override {}
+  real_t iteration(Diffractions*, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >& fourierErrors,
+                   const std::vector<float2>& scanPositions, bool phaseConstraint, bool updateProbe,
+                   bool updateProbeModes, unsigned int iter, bool RMS = false) override;
+  void endPhasing() override {}
   const std::vector<int32_t>& lastOrder() const { return order_; }
 
  private:
@@ -150,10 +198,14 @@ class EPIE : public IPhasingMethod {
   std::vector<int32_t> order_;
 };
 
+// DM (DM.h:50-77)
 class DMMethod : public IPhasingMethod {
  public:
-  int iteration(Engine& eng, const ScanMesh& mesh, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
-                unsigned iter, bool rms, double* err) override;
+  void initMem(IPtychoScanMesh*, uint2) override {}
+  real_t iteration(Diffractions*, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >& fourierErrors,
+                   const std::vector<float2>& scanPositions, bool phaseConstraint, bool updateProbe,
+                   bool updateProbeModes, unsigned int iter, bool RMS = false) override;
+  void endPhasing() override {}
 };
 
 // MLS (MLS.h:48-69): the frame partitions of get_nonoverlapping_indices are drawn once; each
@@ -162,10 +214,11 @@ class DMMethod : public IPhasingMethod {
 class MLSMethod : public IPhasingMethod {
  public:
   MLSMethod(uint64_t seed, int niter) : rng_(seed + 202), seed_(seed), niter_(niter) {}
-  void initMem(const ScanMesh& mesh, int probe_size) override;
-  int iteration(Engine& eng, const ScanMesh& mesh, bool phaseConstraint, bool updateProbe, bool updateProbeModes,
-                unsigned iter, bool rms, double* err) override;
-  std::vector<std::vector<float>> fourierErrors;  // IPhaser::m_fourierErrors, per iteration
+  void initMem(IPtychoScanMesh* scanMesh, uint2 probeSize) override;
+  real_t iteration(Diffractions*, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >& fourierErrors,
+                   const std::vector<float2>& scanPositions, bool phaseConstraint, bool updateProbe,
+                   bool updateProbeModes, unsigned int iter, bool RMS = false) override;
+  void endPhasing() override {}
   std::vector<int32_t> lastGroups, lastOffsets;
 
  private:
@@ -200,6 +253,7 @@ class IPhaser {
   Engine eng;
   std::vector<CXPhasing> methods;
   std::vector<double> errors;
+  std::vector<std::vector<real_t> > fourierErrors;  // IPhaser::m_fourierErrors (IPhaser.cpp:341)
   std::string err;
   double phase_seconds = 0.0;
 
@@ -213,7 +267,7 @@ class IPhaser {
 
  private:
   bool prePhase();
-  bool phaseStep(IPhasingMethod* m, unsigned i, double* e);
+  bool phaseStep(IPhasingMethod* m, unsigned i, real_t* e);
   void postPhase();
   bool simulate_frames(std::vector<float>* amp);
   bool mls_prephase();

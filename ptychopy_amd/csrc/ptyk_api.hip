@@ -3,6 +3,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -58,6 +59,7 @@ struct ptyk_ctx {
   std::vector<int> wave_lo, wave_hi;
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> evpool;
+  std::vector<int> evcat;
   // object-error state
   float2* truth = nullptr;
   float* cov = nullptr;
@@ -653,14 +655,24 @@ static int dm_prepare(ptyk_ctx* c) {
 
 // The launches of one DM iteration on the context's stream (no host synchronisation, no
 // allocation: safe inside a stream capture once dm_prepare has run).
+// HIP events (and their category slots) of the launch timers, grown on demand.
+static int timer_reserve(ptyk_ctx* c, size_t n) {
+  while (c->evpool.size() < n) {
+    hipEvent_t e = nullptr;
+    HIPCHK(c, hipEventCreate(&e));
+    c->evpool.push_back(e);
+  }
+  c->evcat.resize(c->evpool.size());
+  return PTYK_OK;
+}
+
 static int dm_launch(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   DmTimer tm;
   if (kernel_ms) {
-    if (c->evpool.empty()) {
-      c->evpool.resize(1024);
-      for (auto& e : c->evpool) HIPCHK(c, hipEventCreate(&e));
-    }
+    int r = timer_reserve(c, 1024);
+    if (r) return r;
     tm.ev = c->evpool.data();
+    tm.cat = c->evcat.data();
     tm.cap = (int)c->evpool.size();
   }
   const int pc = (flags & PTYK_FLAG_PHASE_CONSTRAINT) ? 1 : 0;
@@ -806,7 +818,8 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.vp_ftot, (size_t)8 * g));
     d.gmax = g;
   }
-  const long need = std::max<long>({6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L});
+  const long need = std::max<long>(
+      {6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L, mls_part_min(d.k)});
   if (need > d.part_cap) {
     if (d.part) hipFree(d.part);
     d.part = nullptr;
@@ -914,8 +927,8 @@ int ptyk_mls_begin(ptyk_ctx* c, const ptyk_mls_config* cfg, const float* positio
   return PTYK_OK;
 }
 
-int ptyk_mls_iteration(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
-                       int32_t n_groups, float* fourier_errors) {
+static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
+                   int32_t n_groups, float* fourier_errors, float* kernel_ms) {
   if (!c || !group_frames || !group_offsets) return fail(c, PTYK_EINVAL, "null argument");
   if (!c->mls_ready) return fail(c, PTYK_ESTATE, "ptyk_mls_begin first");
   if (iter < 1 || n_groups < 0) return fail(c, PTYK_EINVAL, "iter must be >= 1");
@@ -941,18 +954,48 @@ int ptyk_mls_iteration(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, c
   if (r) return r;
   if (total > 0)
     HIPCHK(c, hipMemcpyAsync(d.gframes, group_frames, total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, mls_iteration(d, iter, group_offsets, n_groups, c->stream));
+  MlsTimer tm;
+  if (kernel_ms) {
+    r = timer_reserve(c, (size_t)16 * n_groups + 64);
+    if (r) return r;
+    tm.ev = c->evpool.data();
+    tm.cat = c->evcat.data();
+    tm.cap = (int)c->evpool.size();
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(c, mls_iteration(d, iter, group_offsets, n_groups, c->stream, kernel_ms ? &tm : nullptr));
+  const double host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   int flag = 0;
   HIPCHK(c, hipMemcpyAsync(&flag, d.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if (fourier_errors && iter < 10)
     HIPCHK(c, hipMemcpyAsync(fourier_errors, d.ferr, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));  // also keeps group_frames alive until the copy ran
+  if (kernel_ms) {
+    for (int i = 0; i <= MLS_T_COUNT; ++i) kernel_ms[i] = 0.f;
+    for (int i = 0; i + 1 < tm.n; ++i) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
+      if (tm.cat[i] >= 0 && tm.cat[i] < MLS_T_COUNT) kernel_ms[tm.cat[i]] += ms;
+    }
+    kernel_ms[MLS_T_COUNT] = (float)host_ms;
+  }
   if (flag) {
     HIPCHK(c, hipMemsetAsync(d.flag, 0, sizeof(int), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return fail(c, PTYK_ERANGE, "refined positions moved a ROI outside the object canvas (clamped)");
   }
   return PTYK_OK;
+}
+
+int ptyk_mls_iteration(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
+                       int32_t n_groups, float* fourier_errors) {
+  return mls_run(c, iter, group_frames, group_offsets, n_groups, fourier_errors, nullptr);
+}
+
+int ptyk_mls_iteration_timed(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
+                             int32_t n_groups, float* fourier_errors, float* kernel_ms) {
+  if (!kernel_ms) return fail(c, PTYK_EINVAL, "null argument");
+  return mls_run(c, iter, group_frames, group_offsets, n_groups, fourier_errors, kernel_ms);
 }
 
 static int mls_state_copy(ptyk_ctx* c, const ptyk_mls_state* st, bool out) {

@@ -724,39 +724,47 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
 template <int M, int K>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
                                 double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
-                                hipStream_t s) {
+                                hipStream_t s, MlsTimer* tm) {
   using C = WaveCfg<M>;
   constexpr int BPF = C::R0 * (M / PTYK_MLS_MOD_CB);
   const long groups = (long)K * G * C::L, groups_c = (long)c_modes * G * C::L;
   const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
   const unsigned gc = (unsigned)((groups_c + C::GPW - 1) / C::GPW);
   const size_t lds = (size_t)K * 64 * PTYK_MLS_MOD_CB * sizeof(float2);
-  hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
-  if (e != hipSuccess) return e;
+  static bool lds_set = false;  // the attribute is per kernel and process: set it once
+  if (!lds_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
+  if (tm) tm->mark(s, MLS_T_PASS_A);
   if (exit)
     hipLaunchKernelGGL((k_mls_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
   else
     hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
                        groups, tw);
+  if (tm) tm->mark(s, MLS_T_PASS_B);
   hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp, amp_scale,
                      gf, part, tw);
+  if (tm) tm->mark(s, MLS_T_PASS_C);
   hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
   return hipGetLastError();
 }
 
 template <int M>
 static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s) {
+                                double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s,
+                                MlsTimer* tm) {
   switch (K) {
-    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
-    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s);
+    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -768,12 +776,12 @@ int mls_modulus_blocks(int m, int k) {
 
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
-                              hipStream_t s) {
+                              hipStream_t s, MlsTimer* tm) {
   if (c_modes < 1 || c_modes > K) return hipErrorInvalidValue;
   switch (mls_modulus_blocks(m, K) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
-    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
-    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s);
+    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
     default: return hipErrorInvalidValue;
   }
 }

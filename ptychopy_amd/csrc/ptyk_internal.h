@@ -45,17 +45,18 @@ struct DmArgs {
 // the category given at mark i (category -1 closes the sequence).
 enum { DM_T_ROWS_FWD, DM_T_COLS, DM_T_ROWS_INV, DM_T_PROBE_GATHER, DM_T_OBJ_GATHER, DM_T_PROBE_APPLY,
        DM_T_OBJ_APPLY, DM_T_FINISH, DM_T_COUNT };
-struct DmTimer {
+struct LaunchTimer {
   hipEvent_t* ev = nullptr;
+  int* cat = nullptr;  // cap entries
   int cap = 0, n = 0;
-  int cat[1024];
   void mark(hipStream_t s, int category) {
-    if (n < cap && n < 1024) {
+    if (n < cap) {
       hipEventRecord(ev[n], s);
       cat[n++] = category;
     }
   }
 };
+typedef LaunchTimer DmTimer;
 
 hipError_t dm_iteration(const DmArgs& a, hipStream_t s, DmTimer* tm /* nullable */);
 int dm_obj_col_chunks(int obj_cols);
@@ -104,6 +105,11 @@ struct MlsExitArgs {
   float2* op2;  // nullable
 };
 int mls_modulus_blocks(int m, int k);
+// MLs launch categories of ptyk_mls_iteration_timed (main stream only; the second stream's
+// launches overlap them and are not charged).
+enum { MLS_T_ITER, MLS_T_PROBES, MLS_T_PASS_A, MLS_T_PASS_B, MLS_T_PASS_C, MLS_T_PROBE_UPDATE, MLS_T_PRECOND,
+       MLS_T_LSQ, MLS_T_STEP, MLS_T_MODES, MLS_T_JOIN, MLS_T_COUNT };
+typedef LaunchTimer MlsTimer;
 // pu_j = IFFT(exp(-2 pi i sign (sub f)) / M^2 FFT(chi_j conj(op_j))) for G planes in two passes,
 // the separable ramp applied as row shifts then column shifts (m in {128, 256, 512}; ptyk_dm_wave.hip).
 hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,
@@ -112,7 +118,7 @@ hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int
 // column-transformed state for mls_mode_updates).
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
-                              int c_modes, hipStream_t s);
+                              int c_modes, hipStream_t s, MlsTimer* tm = nullptr);
 // Probe updates of modes 2..K (P_k += mean_j w_j chi_k,j conj(op_j) * beta) straight from the
 // column-transformed planes: the inverse row pass and the frame sum in one kernel over chunks of
 // MLS_PUM_FPC frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame
@@ -196,7 +202,10 @@ int mls_chunks(int m);  // frame chunks of the per-frame reductions
 // Probe::initEvo: normalisations and the initial illumination (host-synchronous).
 hipError_t mls_init(MlsDev& d, const double* frame_power /* host, n */, hipStream_t s);
 // One MLs iteration; group g covers d.gframes[goffs[g] .. goffs[g+1]) (goffs on the host).
-hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s);
+hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s,
+                         MlsTimer* tm = nullptr);
+// Partial-sum capacity (doubles) the MLs kernels need beyond the per-group reductions.
+long mls_part_min(int k);
 hipError_t mls_roi(MlsDev& d, hipStream_t s);
 
 // ---- learned warm start: weighted average of (amplitude, phase) patches at the ROIs of the

@@ -1114,47 +1114,101 @@ __global__ __launch_bounds__(1024) void k_evo_update(const double* __restrict__ 
 // Probe::ortho_modes (Probe.cpp:244-294): modified Gram-Schmidt in mode order with
 // proj = sum(P_i conj P_j) / sum |P_j|^2, then modes 1..K-1 take the orthogonalised modes in
 // order of decreasing power measured before each mode's own orthogonalisation (mode 0 stays).
-__global__ __launch_bounds__(1024) void k_ortho(float2* __restrict__ probe, float2* __restrict__ copy, int K, int mm) {
-  __shared__ double red[16];
-  __shared__ double pw[16];
-  __shared__ int order[16];
-  for (int i = 0; i < K; ++i) {
-    double s = 0.0;
-    for (int q = threadIdx.x; q < mm; q += 1024) s += cabs2(probe[(long)i * mm + q]);
-    s = block_sum<1024>(s, red);
-    if (threadIdx.x == 0) pw[i] = s;
-    for (int j = 0; j < i; ++j) {
-      double sr = 0.0, slr = 0.0, sli = 0.0;
-      for (int q = threadIdx.x; q < mm; q += 1024) {
-        const float2 a = probe[(long)i * mm + q], b = probe[(long)j * mm + q];
-        sr += (double)b.x * b.x + (double)b.y * b.y;
-        slr += (double)a.x * b.x + (double)a.y * b.y;
-        sli += (double)a.y * b.x - (double)a.x * b.y;
+// Spread over ORTHO_NB workgroups, one launch per projection (i, j): every workgroup finishes the
+// coefficient from the previous launch's block partials (the same fixed-order sum in each),
+// subtracts the projection on its pixels and leaves the partials of the next pair (ni, nj) from
+// the values it has just written.  A pixel belongs to the same thread in every launch.  The first
+// launch (i < 0) leaves the power partials of every mode (mode i's sweep changes only mode i, so
+// they are the powers before each sweep); the last (ni < 0) also permutes its pixels' modes.
+constexpr int ORTHO_NB = 64;
+constexpr int ORTHO_KMAX = 16;
+__global__ __launch_bounds__(NT) void k_ortho_step(float2* __restrict__ probe, int mm, int K, int i, int j, int ni,
+                                                   int nj, const double* __restrict__ pin, double* __restrict__ pout,
+                                                   double* __restrict__ pw) {
+  __shared__ double red[NT / 64];
+  __shared__ double coef[2];
+  __shared__ int order[ORTHO_KMAX];
+  const int stride = gridDim.x * NT;
+  if (i >= 0) {
+    if (threadIdx.x < 64) {
+      double a = 0.0, b = 0.0, c = 0.0;
+      for (int t = threadIdx.x; t < (int)gridDim.x; t += 64) {
+        a += pin[3 * t];
+        b += pin[3 * t + 1];
+        c += pin[3 * t + 2];
       }
-      sr = block_sum<1024>(sr, red);
-      slr = block_sum<1024>(slr, red);
-      sli = block_sum<1024>(sli, red);
-      const float2 pr = c_of(slr / sr, sli / sr);
-      for (int q = threadIdx.x; q < mm; q += 1024) {
-        const long ii = (long)i * mm + q;
-        probe[ii] = csub(probe[ii], cmul(probe[(long)j * mm + q], pr));
+      a = wave_sum(a);
+      b = wave_sum(b);
+      c = wave_sum(c);
+      if (threadIdx.x == 0) {
+        coef[0] = b / a;
+        coef[1] = c / a;
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    const float2 pr = c_of(coef[0], coef[1]);
+    float2* pi = probe + (long)i * mm;
+    const float2* pj = probe + (long)j * mm;
+    for (int q = blockIdx.x * NT + threadIdx.x; q < mm; q += stride) pi[q] = csub(pi[q], cmul(pj[q], pr));
+  } else {
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int q = blockIdx.x * NT + threadIdx.x; q < mm; q += stride) s += cabs2(probe[(long)k * mm + q]);
+      s = block_sum<NT>(s, red);
+      if (threadIdx.x == 0) pw[(long)blockIdx.x * K + k] = s;
     }
   }
+  if (ni >= 0) {
+    const float2* a = probe + (long)ni * mm;
+    const float2* b = probe + (long)nj * mm;
+    double sr = 0.0, slr = 0.0, sli = 0.0;
+    for (int q = blockIdx.x * NT + threadIdx.x; q < mm; q += stride) {
+      const float2 u = a[q], v = b[q];
+      sr += (double)v.x * v.x + (double)v.y * v.y;
+      slr += (double)u.x * v.x + (double)u.y * v.y;
+      sli += (double)u.y * v.x - (double)u.x * v.y;
+    }
+    sr = block_sum<NT>(sr, red);
+    slr = block_sum<NT>(slr, red);
+    sli = block_sum<NT>(sli, red);
+    if (threadIdx.x == 0) {
+      pout[3 * blockIdx.x] = sr;
+      pout[3 * blockIdx.x + 1] = slr;
+      pout[3 * blockIdx.x + 2] = sli;
+    }
+    return;
+  }
+  // last launch: order modes 1..K-1 by (-power, index) and permute this thread's pixels
   if (threadIdx.x == 0) {
-    for (int i = 0; i < K; ++i) order[i] = i;
-    for (int a = 1; a < K; ++a)  // insertion sort by (-power, index)
-      for (int b = a; b > 0 && pw[order[b]] > pw[order[b - 1]]; --b) {
+    double pk[ORTHO_KMAX];
+    for (int k = 0; k < K; ++k) {
+      double t = 0.0;
+      for (int b = 0; b < (int)gridDim.x; ++b) t += pw[(long)b * K + k];
+      pk[k] = t;
+      order[k] = k;
+    }
+    for (int a = 1; a < K; ++a)  // insertion sort, stable
+      for (int b = a; b > 0 && pk[order[b]] > pk[order[b - 1]]; --b) {
         const int t = order[b];
         order[b] = order[b - 1];
         order[b - 1] = t;
       }
   }
-  for (long q = threadIdx.x; q < (long)K * mm; q += 1024) copy[q] = probe[q];
   __syncthreads();
-  for (int i = 1; i < K; ++i)
-    for (int q = threadIdx.x; q < mm; q += 1024) probe[(long)i * mm + q] = copy[(long)order[i] * mm + q];
+  for (int q = blockIdx.x * NT + threadIdx.x; q < mm; q += stride) {
+    float2 v[ORTHO_KMAX];
+#pragma unroll
+    for (int k = 1; k < ORTHO_KMAX; ++k)
+      if (k < K) v[k] = probe[(long)k * mm + q];
+#pragma unroll
+    for (int k = 1; k < ORTHO_KMAX; ++k)
+      if (k < K) {
+        const int src = order[k];
+        // mode 0 keeps its place; a mode >= 1 moves to slot k (order[0] is 0: power sorts
+        // include mode 0 but slot 0 is never rewritten, as in the reference's copy loop)
+        probe[(long)k * mm + q] = src == 0 ? probe[q] : v[src];
+      }
+  }
 }
 
 inline dim3 g1(long count) { return dim3((unsigned)((count + NT - 1) / NT)); }
@@ -1174,6 +1228,28 @@ int mls_chunks(int m) { return m * m / ppb_of(m); }
     hipError_t e_ = (x);                    \
     if (e_ != hipSuccess) return e_;        \
   } while (0)
+
+// Probe::ortho_modes as 1 + K (K - 1) / 2 launches of k_ortho_step (partials ping-pong in d.part).
+hipError_t mls_ortho(MlsDev& d, hipStream_t s) {
+  const int K = d.k, mm = d.m * d.m;
+  if (K < 2) return hipSuccess;
+  if (K > ORTHO_KMAX || d.part_cap < 6L * ORTHO_NB + (long)K * ORTHO_NB) return hipErrorInvalidValue;
+  double* ping = d.part;
+  double* pong = d.part + 3 * ORTHO_NB;
+  double* pw = d.part + 6 * ORTHO_NB;
+  // (1, 0), (2, 0), (2, 1), ..., (K - 1, K - 2); the first launch only prepares (1, 0)
+  std::vector<std::pair<int, int>> pairs;
+  for (int i = 1; i < K; ++i)
+    for (int j = 0; j < i; ++j) pairs.emplace_back(i, j);
+  for (size_t t = 0; t <= pairs.size(); ++t) {
+    const int ci = t == 0 ? -1 : pairs[t - 1].first, cj = t == 0 ? -1 : pairs[t - 1].second;
+    const int ni = t < pairs.size() ? pairs[t].first : -1, nj = t < pairs.size() ? pairs[t].second : -1;
+    LAUNCH(k_ortho_step, dim3(ORTHO_NB), dim3(NT), d.probe, mm, K, ci, cj, ni, nj, ping, pong, pw);
+    CHK();
+    std::swap(ping, pong);
+  }
+  return hipSuccess;
+}
 
 hipError_t mls_roi(MlsDev& d, hipStream_t s) {
   const float cx = (float)std::ceil(d.rows * 1.0 / 2 - d.m * 1.0 / 2);
@@ -1269,7 +1345,14 @@ hipError_t mls_init(MlsDev& d, const double* frame_power, hipStream_t s) {
   return hipStreamSynchronize(s);
 }
 
-hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s) {
+long mls_part_min(int k) { return (6L + std::max(k, ORTHO_KMAX)) * ORTHO_NB; }
+
+#define MARK(c) \
+  do {          \
+    if (tm) tm->mark(s, (c)); \
+  } while (0)
+
+hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s, MlsTimer* tm) {
   const int m = d.m, mm = m * m, K = d.k, ch = mls_chunks(m);
   const long canvas = (long)d.rows * d.cols;
   const dim3 tiles((d.cols + 63) / 64, (d.rows + 3) / 4);
@@ -1277,6 +1360,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   const bool var_any = d.variable_probe || d.variable_intensity;
   const int subpix = d.apply_subpix_shift ? 1 : 0;
 
+  MARK(MLS_T_ITER);
   // MLS.cpp:86-89
   if (iter == 1 || iter >= d.pps) TRY(mls_roi(d, s));
   // calc_object_norm over the precalculated box
@@ -1309,6 +1393,14 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   const bool obj_rec_iter = iter >= d.object_reconstruct;
   const bool probe_rec = iter >= d.probe_reconstruct;
 
+  const int mod_blocks = mls_modulus_blocks(m, K);
+  // the illumination probes of group g + 1 (spectra of P_0 and V, shifted probes) are formed on
+  // the second stream while the probe modes 2..K of group g are updated: they read only P_0, V,
+  // the evolution coefficients and sub-pixel shifts of group g + 1's frames, none of which the
+  // rest of group g writes, and they write only spec and vp, which nothing after that point in
+  // group g reads
+  const bool prefetch = subpix && mod_blocks && d.side;
+  bool vp_ready = false;
   for (int g = 0; g < n_groups; ++g) {
     const int G = goffs[g + 1] - goffs[g];
     const int* gf = d.gframes + goffs[g];
@@ -1316,8 +1408,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
     const bool pos_step = iter >= d.pps;
-    if (subpix) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
-    const int mod_blocks = mls_modulus_blocks(m, K);
+    MARK(MLS_T_PROBES);
+    if (subpix && !vp_ready) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     // with the three-pass exit waves the object patches stay in op, and the position step's
     // derivative images come from one-dimensional transforms of them
     const bool pos_1d_step = pos_step && mod_blocks && has_pos_1d(m) && d.side;
@@ -1325,8 +1417,10 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     const bool defer_modes = probe_rec && !d.apply_multimodal_update && K > 1 && mod_blocks && d.pum_part;
     if (subpix && mod_blocks) {
       // the shifted illumination probes' spectra formed in the inverse row pass itself
-      TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
-      TRY(fft_cols_batch(d.vp, G, m, true, d.tw, s));
+      if (!vp_ready) {
+        TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
+        TRY(fft_cols_batch(d.vp, G, m, true, d.tw, s));
+      }
     } else {
       LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
       CHK();
@@ -1337,7 +1431,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       // from O, P and vp; the position step's patch copy takes its own forward transform
       const bool spec2d = pos_step && !pos_1d_step;
       const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
-      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, defer_modes ? 1 : K, s));
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, defer_modes ? 1 : K, s,
+                             tm));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
       if (pos_1d_step) {
         // the position step reads only chi0, vp and op, which stay fixed until the group ends,
@@ -1362,6 +1457,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       CHK();
       TRY(fft2_batch(d.psi, (long)K * G, m, true, d.tw, s));
     }
+    MARK(MLS_T_PROBE_UPDATE);
     if (iter < 10) LAUNCH(k_ferr, g1(G), dim3(NT), d.part, mod_blocks ? mod_blocks : ch, gf, G, mm, d.ferr);
     if (!probe_rec) {  // probe_update / probe_update_m stay zero (MLS.cpp:146-153)
       LAUNCH(k_fill_c, g1(gm), dim3(NT), d.pu, gm, make_float2(0.f, 0.f));
@@ -1425,6 +1521,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         CHK();
       }
       const float2* pl = llp == 1 ? nullptr : d.probe + (long)(llp - 1) * mm;
+      MARK(MLS_T_PRECOND);
       if (obj_rec) {
         const int scatter = d.delta_p != 0 ? 1 : 0;
         LAUNCH(k_precond<PRE_RPT>, pre_tiles, dim3(NT), d.precond, d.illum, chi, d.vp, pl, d.roi, gf, G, m, d.rows, d.cols, fd,
@@ -1432,6 +1529,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         CHK();
       }
       const bool lsq = d.beta_lsq > 0 && ll == 1 && obj_rec && probe_rec;
+      MARK(MLS_T_LSQ);
       if (lsq) {
         // the object direction: the preconditioned update at each ROI, or chi conj(vp) unscaled
         const bool scatter = d.delta_p != 0;
@@ -1446,6 +1544,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
                d.beta_probe * d.beta_lsq, lsq ? 1 : 0, d.step_obj, d.step_probe, d.ftot, d.gscal);
         CHK();
       }
+      MARK(MLS_T_STEP);
       {  // object and probe steps in one launch (disjoint arrays, both after the step solve)
         const long no = obj_rec ? canvas : 0, np = probe_rec && ll <= K ? (long)mm : 0;
         if (no + np > 0)
@@ -1501,6 +1600,19 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
         }
       }
     }
+    vp_ready = false;
+    if (prefetch && g + 1 < n_groups) {
+      const int Gn = goffs[g + 2] - goffs[g + 1];
+      const int* gfn = d.gframes + goffs[g + 1];
+      TRY(hipEventRecord(d.ev_fork, s));
+      TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
+      TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, d.side));
+      TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gfn, d.n, var_any ? 1 : 0, Gn, m, d.tw, d.side));
+      TRY(fft_cols_batch(d.vp, Gn, m, true, d.tw, d.side));
+      TRY(hipEventRecord(d.ev_join, d.side));
+      vp_ready = true;
+    }
+    MARK(MLS_T_MODES);
     if (defer_modes) {
       TRY(mls_mode_updates(d.probe, d.psi, d.op, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
                            d.pum_part, d.tw, s));
@@ -1509,15 +1621,16 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
              d.cols, fw, d.gscal);
       CHK();
     }
-    if (pos_1d_step || vp_side) TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // before the next group's writes
+    MARK(MLS_T_JOIN);
+    if (pos_1d_step || vp_side || vp_ready) TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // before the next group's writes
   }
-  if (K > d.n_rec) {
-    LAUNCH(k_ortho, dim3(1), dim3(1024), d.probe, d.tmp, K, mm);
-    CHK();
-  }
+  MARK(MLS_T_ITER);
+  if (K > d.n_rec) TRY(mls_ortho(d, s));
+  MARK(-1);
   return hipSuccess;
 }
 
+#undef MARK
 #undef LAUNCH
 #undef CHK
 #undef TRY

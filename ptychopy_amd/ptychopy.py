@@ -247,8 +247,10 @@ def _make_phaser(algorithm: str, vals: dict, arrays: dict) -> Phaser:
 def _run(algorithm: str, args, kw, with_arrays: bool):
     vals, arrays = _collect(args, kw, with_arrays)
     if with_arrays and arrays.get("diffractionNP") is None:
+        # ptychopy_algorithmnp prints this and returns early; the entry point still returns
+        # True (ptychopy.c:366-367, 523-540)
         print("Please input a diffraction pattern numpy array")
-        return None
+        return True
     ph = _make_phaser(algorithm, vals, arrays)
     try:
         ph.init()
@@ -257,7 +259,7 @@ def _run(algorithm: str, args, kw, with_arrays: bool):
         ph.write_results(0)
     finally:
         ph.close()
-    return None if with_arrays else True
+    return True  # every whole-mode entry point returns Py_True (ptychopy.c:499-540)
 
 
 def helloworld(list1, list2, list3):
@@ -282,15 +284,18 @@ def mls(*args, **kw):
 
 
 def epienp(*args, **kw):
-    """ePIE on numpy inputs diffractionNP / positionNP / objectNP / probeNP (ptychopy.c:523-529)."""
+    """ePIE on numpy inputs diffractionNP / positionNP / objectNP / probeNP (ptychopy.c:523-529);
+    returns True."""
     return _run("ePIE", args, kw, True)
 
 
 def dmnp(*args, **kw):
+    """DM on numpy inputs (ptychopy.c:530-535); returns True."""
     return _run("DM", args, kw, True)
 
 
 def mlsnp(*args, **kw):
+    """MLs on numpy inputs (ptychopy.c:537-542); returns True."""
     return _run("MLs", args, kw, True)
 
 

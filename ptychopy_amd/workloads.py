@@ -6,7 +6,7 @@ dm_c1: plain DM at the reference simulation shape -- 30 x 30 raster, 512 x 512 d
 edge, 1e7 photons per frame with Poisson noise, one probe mode.
 mls_c2: MLs with mixed modes -- 51 x 51 raster, 256 x 256 frames, 100 nm steps = 14 px
 (dx_s = 100/14 nm), object on the MLs canvas (about 1146 x 1146), five orthogonalised
-Hermite-Gaussian modes (60 px fundamental FWHM) with power fractions 0.6/0.2/0.1/0.06/0.04,
+Hermite-Gaussian modes (48 px fundamental FWHM: the epie_c0 probe's 24 px at 128^2 scaled to 256^2) with power fractions 0.6/0.2/0.1/0.06/0.04,
 1e5 photons per frame with Poisson noise, delta_p = 0.1.
 batch_c3: 16 independent DM scans of uneven size -- raster sides drawn from 20..51, detectors
 from {128, 256, 512}^2, 1e5..1e7 photons (log-uniform); object/probe statistics of epie_c0
@@ -64,7 +64,7 @@ def dm_c1(n_side: int = 30, m: int = 512, photons: float = 1e7, sim_size: int = 
                 iterations=100)
 
 
-def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwhm: float = 60.0,
+def mls_c2(n_side: int = 51, m: int = 256, k: int = 5, photons: float = 1e5, fwhm: float = 48.0,
            config_id: int = 2, device: int = 0, phase_corr: float = 6.0) -> dict:
     dx_s = 100e-9 / 14.0
     step = 100e-9
@@ -122,6 +122,21 @@ def dm_bytes_per_frame(m: int, k: int, obj_px: int, n: int) -> float:
     """Compulsory HBM bytes per frame-iteration of the psi-resident DM form:
     M^2 (2 [FP16 A] + 16 K [psi read + write]) + (16 obj_px + 16 K M^2) / N."""
     return m * m * (2 + 16 * k) + (16 * obj_px + 16 * k * m * m) / n
+
+
+def mls_kind_bytes(kind: str, m: int, k: int, n: int, groups: int) -> float:
+    """Algorithmic bytes of one MLs iteration's launches of a main-stream kind (all groups), at
+    storage precision, for the three-pass form with probe modes 2..K deferred to their update:
+    exit_row_pass reads the object patch and mode 0's illumination probe (8 + 8 B/px) and writes
+    K exit-wave planes and the patch (8 K + 8); modulus_column_pass reads and writes the K planes
+    and reads FP16 A (16 K + 2); inverse_row_pass reads and writes mode 0's plane (16); probe_modes
+    reads planes 2..K, the patch and the illumination at the ROI (8 (K - 1) + 8 + 4)."""
+    px = float(n) * m * m
+    per_px = {"exit_row_pass": 8.0 * k + 24.0, "modulus_column_pass": 16.0 * k + 2.0, "inverse_row_pass": 16.0,
+              "probe_modes": 8.0 * (k - 1) + 12.0}
+    if kind not in per_px:
+        return 0.0
+    return per_px[kind] * px + (8.0 * k * m * m * groups if kind == "exit_row_pass" else 0.0)
 
 
 def mls_bytes_per_frame(m: int, obj_px: int, n: int, groups: int) -> float:

@@ -160,7 +160,9 @@ def test_python_surface_argument_handling(capsys):
         ptychopy.epie(notakeyword=1)
     with pytest.raises(TypeError):
         ptychopy.dm("job", iter="many")
-    assert ptychopy.epienp(jobID="x") is None  # no diffractionNP: message, None
+    # no diffractionNP: the message, and True as every whole-mode entry point returns
+    # (ptychopy.c:366-367, 523-540)
+    assert ptychopy.epienp(jobID="x") is True
     assert "Please input a diffraction pattern numpy array" in capsys.readouterr().out
     with pytest.raises(ValueError, match="2 dimensional object"):
         ptychopy.epienp(diffractionNP=np.zeros((1, 32, 32)), objectNP=np.zeros(3, complex))

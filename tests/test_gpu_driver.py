@@ -31,7 +31,7 @@ def test_dmnp_matches_engine_and_writes_results(gpu, tmp_path, monkeypatch):
     from ptychopy_amd._engine import Engine, FLAG_PHASE_CONSTRAINT, FLAG_UPDATE_PROBE
     monkeypatch.chdir(tmp_path)
     amp, pos, canvas, kw, dx_s = _dataset()
-    assert ptychopy.dmnp(jobID="t", diffractionNP=amp.astype(np.float64), iter=12, **kw) is None
+    assert ptychopy.dmnp(jobID="t", diffractionNP=amp.astype(np.float64), iter=12, **kw) is True
     obj = D.load_complex_csv("data/t_object_0.csv")
     prb = D.load_complex_csv("data/t_probes_0.csv")
     assert obj.shape == canvas and prb.shape == (64, 64)
