@@ -56,6 +56,15 @@ def lib():
         _lib.orc_mls_position_step.argtypes = [P, P, P, i, P, P, P]
         _lib.orc_mls_iteration.argtypes = [C.POINTER(MlsParams), i, i, i, i, i, P, P, P, P, P, d, P, P, P, P, P, P,
                                            P, P, P, i, P]
+        _lib.orc_mls_object_norm.argtypes = [P, i, i, P, P, i, i, P, P]
+        _lib.orc_mls_remove_extra.argtypes = [i, i, P, P, P]
+        _lib.orc_mls_frame_modulus.argtypes = [P, P, i, i]
+        _lib.orc_mls_frame_modulus.restype = d
+        _lib.orc_mls_variable_probe.argtypes = [i, i, i, P, P, P, i, i, P, P, P, P, P, P]
+        _lib.orc_mls_mode_directions.argtypes = [i, i, i, i, i, P, P, P, P, i, d, P, P, P, P, i, i, P, P, P]
+        _lib.orc_modulus_modes.argtypes = [P, P, i, i]
+        _lib.orc_orthogonalize_modes.argtypes = [P, i, i]
+        _lib.orc_init_probe_modes.argtypes = [P, P, i, i]
     return _lib
 
 
@@ -97,6 +106,29 @@ def modulus_projection(psi, amp) -> np.ndarray:
     a = _f64(amp)
     lib().orc_modulus_projection(_ptr(p), _ptr(a), p.shape[-1])
     return p
+
+
+def modulus_modes(psi, amp) -> np.ndarray:
+    """Modulus constraint on K exit waves (K x m x m): single-mode form for K = 1, modal for K > 1."""
+    p = _c128(psi).copy()
+    k, m = p.shape[0], p.shape[-1]
+    lib().orc_modulus_modes(_ptr(p), _ptr(_f64(amp)), k, m)
+    return p
+
+
+def orthogonalize_modes(probe) -> np.ndarray:
+    """Probe::orthogonalize (reference coefficient convention)."""
+    p = _c128(probe).copy()
+    lib().orc_orthogonalize_modes(_ptr(p), p.shape[0], p.shape[-1])
+    return p
+
+
+def init_probe_modes(probe):
+    """Probe::initProbeModes: (modes, intensities)."""
+    p = _c128(probe).copy()
+    pi = _f64(np.abs(p) ** 2).copy()
+    lib().orc_init_probe_modes(_ptr(p), _ptr(pi), p.shape[0], p.shape[-1])
+    return p, pi
 
 
 def probe_init(amp, k_total: int, beam_size: float, dx_s: float):
@@ -233,6 +265,66 @@ def mls_lsq(oup, vp, pum, op, chi) -> np.ndarray:
     x = np.zeros((g, 2), np.complex128)
     lib().orc_mls_lsq(*[_ptr(a) for a in arrs], g, m, _ptr(x))
     return x
+
+
+def mls_object_norm(box, obj, illum, probe, vmode):
+    """Probe::calc_object_norm: (obj, probe, vmode) after the renormalisation."""
+    o = _c128(obj).copy()
+    p = _c128(probe).copy()
+    v = _c128(vmode).copy()
+    b = np.ascontiguousarray(box, dtype=np.int32)
+    lib().orc_mls_object_norm(_ptr(b), o.shape[0], o.shape[1], _ptr(o), _ptr(_f64(illum)), p.shape[0], p.shape[-1],
+                              _ptr(p), _ptr(v))
+    return o, p, v
+
+
+def mls_remove_extra(probe, vmode, evo):
+    """Probe::remove_extra_degree: (probe, vmode, evo) after it; evo is 2 x n."""
+    p = _c128(probe).copy()
+    v = _c128(vmode).copy()
+    e = _f64(evo).copy()
+    lib().orc_mls_remove_extra(e.shape[1], p.shape[-1], _ptr(p), _ptr(v), _ptr(e))
+    return p, v, e
+
+
+def mls_frame_modulus(spectra, amp):
+    """(chi, error) of one frame: spectra K x m x m of the exit waves, amp m x m."""
+    p = _c128(spectra).copy()
+    err = lib().orc_mls_frame_modulus(_ptr(p), _ptr(_f64(amp)), p.shape[0], p.shape[-1])
+    return p, float(err)
+
+
+def mls_variable_probe(frames, roi, illum, pu, pum, op, chi0, vmode, evo1):
+    """Variable-mode update of one group: (vmode, evo1) after it."""
+    gi = np.ascontiguousarray(frames, dtype=np.int32)
+    r = np.ascontiguousarray(roi, dtype=np.int32)
+    il = _f64(illum)
+    v = _c128(vmode).copy()
+    e1 = _f64(evo1).copy()
+    g, m = len(gi), v.shape[-1]
+    lib().orc_mls_variable_probe(g, e1.shape[0], m, _ptr(gi), _ptr(r), _ptr(il), il.shape[0], il.shape[1],
+                                 _ptr(_c128(pu)), _ptr(_c128(pum)), _ptr(_c128(op)), _ptr(_c128(chi0)), _ptr(v),
+                                 _ptr(e1))
+    return v, e1
+
+
+def mls_mode_directions(ll, frames, roi, sub, illum, max_illum, chi, op, vp, probe, subpix=False,
+                        probe_rec=True, obj_rec=True):
+    """(pu, pum, oup) of mode ll (1-based) for one group (MLS.cpp:251-297)."""
+    gi = np.ascontiguousarray(frames, dtype=np.int32)
+    r = np.ascontiguousarray(roi, dtype=np.int32)
+    sb = np.ascontiguousarray(sub, dtype=np.float32)
+    il = _f64(illum)
+    p = _c128(probe)
+    k, m = p.shape[0], p.shape[-1]
+    g = len(gi)
+    pu = np.zeros((g, m, m), np.complex128)
+    pum = np.zeros((m, m), np.complex128)
+    oup = np.zeros((g, m, m), np.complex128)
+    lib().orc_mls_mode_directions(int(ll), min(k, int(ll)), g, m, int(subpix), _ptr(gi), _ptr(r), _ptr(sb), _ptr(il),
+                                  il.shape[1], float(max_illum), _ptr(_c128(chi)), _ptr(_c128(op)), _ptr(_c128(vp)),
+                                  _ptr(p), int(probe_rec), int(obj_rec), _ptr(pu), _ptr(pum), _ptr(oup))
+    return pu, pum, oup
 
 
 def mls_ortho_modes(probe) -> np.ndarray:

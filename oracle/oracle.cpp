@@ -16,7 +16,76 @@
 
 using namespace orc;
 
+namespace {
+
+// Fourier modulus constraint on the K spectra Phi_k at psi + k mm (PhaserUtil::applyModulusConstraint,
+// utilities.cpp:138-154): K = 1, Phi' = A exp(i atan2 Phi) / M^2 (d_adjustFFT,
+// utilitiesKernels.cu:619-653); K > 1, Phi'_k = Phi_k A / sqrt(sum_k |Phi_k|^2) / M^2
+// (d_adjustModalFFT, utilitiesKernels.cu:655-682).
+void modulus_spectra(cd* psi, const double* A, int k, int m) {
+  const long mm = (long)m * m;
+  const double nrm = 1.0 / (double)mm;
+  for (long q = 0; q < mm; ++q) {
+    if (k == 1) {
+      psi[q] = A[q] * unit_phase(psi[q]) * nrm;
+    } else {
+      double s2 = 0;
+      for (int kk = 0; kk < k; ++kk) s2 += std::norm(psi[(size_t)kk * mm + q]);
+      const double f = A[q] / std::sqrt(s2) * nrm;
+      for (int kk = 0; kk < k; ++kk) psi[(size_t)kk * mm + q] *= f;
+    }
+  }
+}
+
+}  // namespace
+
 extern "C" {
+
+// The modulus constraint on K exit waves (k x m x m, in place): FFT, modulus_spectra, unnormalised
+// inverse FFT.
+void orc_modulus_modes(double* psi, const double* amp, int k, int m) {
+  Fft1 p(m);
+  const long mm = (long)m * m;
+  for (int kk = 0; kk < k; ++kk) fft2(C(psi) + kk * mm, m, false, p);
+  modulus_spectra(C(psi), amp, k, m);
+  for (int kk = 0; kk < k; ++kk) fft2(C(psi) + kk * mm, m, true, p);
+}
+
+// Probe::orthogonalize (Probe.cpp:228-242): for i = 1..K-1 and j = i-1 down to 0,
+// P_i -= P_j (sum conj(v_i) P_j / sum |P_j|^2), with v_i a copy of mode i taken before the sweep
+// (h_projectUtoV with h_innerProduct(a, b) = sum conj(a) b, ProbeKernels.cu:509-515 and
+// utilitiesKernels.cu:854-867).  The coefficient conjugates the textbook projection
+// sum conj(P_j) v_i; for real modes the two agree.
+void orc_orthogonalize_modes(double* probe, int k, int m) {
+  const long mm = (long)m * m;
+  cd* P = C(probe);
+  const std::vector<cd> copy(P, P + (long)k * mm);
+  for (int i = 1; i < k; ++i)
+    for (int j = i - 1; j >= 0; --j) {
+      cd num = 0;
+      double den = 0;
+      for (long q = 0; q < mm; ++q) {
+        num += std::conj(copy[(long)i * mm + q]) * P[(long)j * mm + q];
+        den += std::norm(P[(long)j * mm + q]);
+      }
+      const cd f = num / den;
+      for (long q = 0; q < mm; ++q) P[(long)i * mm + q] -= P[(long)j * mm + q] * f;
+    }
+}
+
+// Probe::initProbeModes (Probe.cpp:320-330): modes 1..K-1 <- 0.05 P0 with intensities
+// |0.05 P0|^2 (d_initModesFromBase, ProbeKernels.cu:82-97), then orthogonalize.
+void orc_init_probe_modes(double* probe, double* probe_int, int k, int m) {
+  const long mm = (long)m * m;
+  cd* P = C(probe);
+  for (int kk = 1; kk < k; ++kk)
+    for (long q = 0; q < mm; ++q) {
+      P[(long)kk * mm + q] = P[q] * 0.05;
+      const double a = std::abs(P[(long)kk * mm + q]);
+      probe_int[(long)kk * mm + q] = a * a;
+    }
+  orc_orthogonalize_modes(probe, k, m);
+}
 
 int orc_num_threads() {
 #ifdef _OPENMP
@@ -122,7 +191,6 @@ void orc_epie_iteration(double* obj, double* obj_int, int rows, int cols, double
   cd* P = C(probe);
   std::vector<cd> psi_old((size_t)k_use * mm), psi((size_t)k_use * mm);
   double obj_max = scal[0], probe_max = scal[1], err = 0.0;
-  const double nrm = 1.0 / (double)mm;
   for (int64_t s = 0; s < n; ++s) {
     const int fi = order[s];
     const float px = pos[2 * fi], py = pos[2 * fi + 1];
@@ -135,16 +203,7 @@ void orc_epie_iteration(double* obj, double* obj_int, int rows, int cols, double
       }
     psi = psi_old;
     for (int k = 0; k < k_use; ++k) fft2(psi.data() + (size_t)k * mm, m, false, p);
-    for (long q = 0; q < mm; ++q) {
-      if (k_use == 1) {
-        psi[q] = A[q] * unit_phase(psi[q]) * nrm;
-      } else {
-        double s2 = 0;
-        for (int k = 0; k < k_use; ++k) s2 += std::norm(psi[(size_t)k * mm + q]);
-        const double f = A[q] / std::sqrt(s2) * nrm;
-        for (int k = 0; k < k_use; ++k) psi[(size_t)k * mm + q] *= f;
-      }
-    }
+    modulus_spectra(psi.data(), A, k_use, m);
     for (int k = 0; k < k_use; ++k) fft2(psi.data() + (size_t)k * mm, m, true, p);
     // object update at the rounded position (d_updateObjectArray(M), SampleKernels.cu:244-294)
     const int rx = (int)std::round(px), ry = (int)std::round(py);  // roundf: half away from zero

@@ -155,6 +155,127 @@ void lsq_steps(const cd* oup, const cd* vp, const cd* pum, const cd* op, const c
   }
 }
 
+
+// Diffractions::get_fourier_error and modulus_constraint for one frame (Diffractions.cpp:261-310;
+// d_modulus_amplitude, DiffractionsKernels.cu:127-137).  psi + kk * stride holds the spectrum of
+// mode kk; ap = sqrt(sum_k |Psi_k|^2); the error is sqrt(mean((A - ap)^2 / 0.25)) (the real
+// h_normalize divides by its factor, utilitiesKernels.cu:1672-1678); chi_k = IFFT_u((A / (ap +
+// 1e-9) - 1) Psi_k) / M^2 replaces each spectrum.  Returns the error.
+double frame_modulus(cd* psi, long stride, const double* A, int k, int m, const Fft1& fp) {
+  const long mm = (long)m * m;
+  double e2 = 0;
+  for (long q = 0; q < mm; ++q) {
+    double s2 = 0;
+    for (int kk = 0; kk < k; ++kk) s2 += std::norm(psi[kk * stride + q]);
+    const double ap = std::sqrt(s2);
+    e2 += (A[q] - ap) * (A[q] - ap) / 0.25;
+    const double R = A[q] / (ap + 1e-9) - 1.0;
+    for (int kk = 0; kk < k; ++kk) psi[kk * stride + q] *= R;
+  }
+  const double nrm = 1.0 / (double)mm;
+  for (int kk = 0; kk < k; ++kk) {
+    cd* ps = psi + kk * stride;
+    fft2(ps, m, true, fp);
+    for (long q = 0; q < mm; ++q) ps[q] *= nrm;
+  }
+  return std::sqrt(e2 / mm);
+}
+
+// Probe::update_variable_probe, variable-mode part (Probe.cpp:1366-1477): with w = p / max(p) at
+// each frame's ROI, img_j = w (pu_j - pum), c_j = (mean Re(V conj img_j) + e1_j) / sum_j e1_j^2,
+// U = sum_j c_j img_j / M^2, V <- V + U relax_U / max(1, rms U), relax_U = G / N, V <- V / rms V;
+// then e1_j += mean Re(chi_j conj(op_j V)) / (mean |op_j V|^2 + 0.1 mean_j mean |op_j V|^2).
+void variable_probe_update(int G, int n, int m, const int32_t* gi, const int32_t* roi, const double* illum, int rows,
+                           int cols, const cd* pu, const cd* pum, const cd* op, const cd* chi0, cd* V, double* evo1) {
+  const long mm = (long)m * m;
+  const double nrm = 1.0 / (double)mm;
+  const double relax_u = (double)G / n;
+  double evn = 0;
+  for (int j = 0; j < G; ++j) evn += evo1[gi[j]] * evo1[gi[j]];
+  double pmax = 0;
+  for (long q = 0; q < (long)rows * cols; ++q) pmax = std::max(pmax, illum[q]);
+  std::vector<cd> img((size_t)G * mm), acc(mm, cd(0));
+  for (int j = 0; j < G; ++j) {
+    const int f = gi[j];
+    double sre = 0;
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) {
+        const long q = (long)r * m + c;
+        const double w = illum[(long)(roi[2 * f] + r) * cols + roi[2 * f + 1] + c] / pmax;
+        img[j * mm + q] = w * (pu[j * mm + q] - pum[q]);
+        sre += (V[q] * std::conj(img[j * mm + q])).real();
+      }
+    const double cj = (sre / mm + evo1[f]) / evn;
+    for (long q = 0; q < mm; ++q) acc[q] += img[j * mm + q] * cj;
+  }
+  double s2 = 0;
+  for (long q = 0; q < mm; ++q) {
+    acc[q] *= nrm;
+    s2 += std::norm(acc[q]);
+  }
+  const double sc = relax_u / std::max(1.0, std::sqrt(s2 / mm));
+  double sv = 0;
+  for (long q = 0; q < mm; ++q) {
+    V[q] += acc[q] * sc;
+    sv += std::norm(V[q]);
+  }
+  const double vn = std::sqrt(sv / mm);
+  for (long q = 0; q < mm; ++q) V[q] *= 1.0 / vn;
+  std::vector<double> den(G), num(G);
+  double dsum = 0;
+  for (int j = 0; j < G; ++j) {
+    double d = 0, u = 0;
+    for (long q = 0; q < mm; ++q) {
+      const cd ov = op[j * mm + q] * V[q];
+      d += std::norm(ov);
+      u += (chi0[j * mm + q] * std::conj(ov)).real();
+    }
+    den[j] = d / mm;
+    num[j] = u / mm;
+    dsum += den[j];
+  }
+  const double m3 = dsum / G;
+  for (int j = 0; j < G; ++j) evo1[gi[j]] += num[j] / (den[j] + 0.1 * m3);
+}
+
+// The probe-update and object-update directions of mode ll for one group (MLS.cpp:251-297):
+// pu_j = chi_j conj(op_j) (shifted back by the frame's sub-pixel shift for ll = 1 with
+// apply_subpix_shift), pum = mean_j (p / (p + 0.01 MAX_ILLUM))_roi_j pu_j (d_addFactorDivide);
+// the object direction oup_j = chi_j conj(vp_j) for llp = 1 and P_llp chi_j (no conjugate,
+// MLS.cpp:294-297) for llp > 1.  pu/pum are formed when probe_rec, oup when obj_rec.
+void mode_directions(int ll, int llp, int G, int m, int subpix, const int32_t* gi, const int32_t* roi,
+                     const float* sub, const double* illum, int cols, double max_illum, const cd* chi, const cd* op,
+                     const cd* vp, const cd* P, bool probe_rec, bool obj_rec, cd* pu, cd* pum, cd* oup,
+                     const Fft1& fp) {
+  const long mm = (long)m * m;
+  std::fill(pum, pum + mm, cd(0));
+  if (probe_rec) {
+#pragma omp parallel for schedule(dynamic)
+    for (int j = 0; j < G; ++j) {
+      const int f = gi[j];
+      cd* u = pu + j * mm;
+      for (long q = 0; q < mm; ++q) u[q] = chi[j * mm + q] * std::conj(op[j * mm + q]);
+      if (ll == 1 && subpix) shift_frame(u, m, -sub[2 * f], -sub[2 * f + 1], fp);
+    }
+    const double fw = max_illum * 0.01;
+    for (int j = 0; j < G; ++j) {
+      const int f = gi[j];
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c) {
+          const double pv = illum[(long)(roi[2 * f] + r) * cols + roi[2 * f + 1] + c];
+          pum[(long)r * m + c] += pv / (pv + fw) * pu[j * mm + (long)r * m + c];
+        }
+    }
+    for (long q = 0; q < mm; ++q) pum[q] /= (double)G;
+  }
+  if (obj_rec)
+    for (int j = 0; j < G; ++j)
+      for (long q = 0; q < mm; ++q)
+        oup[j * mm + q] = (llp == 1) ? chi[j * mm + q] * std::conj(vp[j * mm + q])
+                                     : P[(llp - 1) * mm + q] * chi[j * mm + q];
+  else
+    std::fill(oup, oup + (long)G * mm, cd(0));
+}
 }  // namespace
 
 extern "C" {
@@ -178,6 +299,87 @@ void orc_mls_position_step(const double* op, const double* vp, const double* chi
   Fft1 p(m);
   position_step(C(op), vp ? C(vp) : nullptr, chi ? C(chi) : nullptr, m, p, out, out + 1, dX ? C(dX) : nullptr,
                 dY ? C(dY) : nullptr);
+}
+
+// Probe::calc_object_norm (Probe.cpp:671-699) over the precalculated 1-based inclusive box
+// (x0, x1, y0, y1): object_norm = rms(O W) / rms(W) with W the illumination; the probe modes and
+// the variable mode are multiplied by it and the object by its inverse (the complex h_normalize
+// multiplies, utilitiesKernels.cu:1646-1653).
+void orc_mls_object_norm(const int32_t* box, int rows, int cols, double* obj, const double* illum, int k, int m,
+                         double* probe, double* vmode) {
+  const long mm = (long)m * m, canvas = (long)rows * cols;
+  cd* O = C(obj);
+  const int x0 = box[0] - 1, y0 = box[2] - 1;
+  const int bx = box[1] - box[0] + 1, by = box[3] - box[2] + 1;
+  double sw = 0, so = 0;
+  for (int r = 0; r < bx; ++r)
+    for (int c = 0; c < by; ++c) {
+      const long q = (long)(x0 + r) * cols + y0 + c;
+      sw += illum[q] * illum[q];
+      so += std::norm(O[q] * illum[q]);
+    }
+  const double cnt = (double)bx * by;
+  const double onorm = std::sqrt(so / cnt) / std::sqrt(sw / cnt);
+  for (long q = 0; q < (long)k * mm; ++q) C(probe)[q] *= onorm;
+  for (long q = 0; q < mm; ++q) C(vmode)[q] *= onorm;
+  for (long q = 0; q < canvas; ++q) O[q] *= 1.0 / onorm;
+}
+
+// Probe::remove_extra_degree (Probe.cpp:615-669): V <- V / rms V, e1 <- e1 rms V; then over
+// len = number of evolution rows (2), i.e. the first two positions only: e0 <- (e0 - mean e0)
+// 0.99 + 1, e1 <- e1 - mean e1 (means kept in a float2), e0 <- e0 / rms(e0), P0 <- P0 rms(e0).
+// evo: 2 x n.
+void orc_mls_remove_extra(int n, int m, double* probe, double* vmode, double* evo) {
+  const long mm = (long)m * m;
+  cd* P = C(probe);
+  cd* V = C(vmode);
+  double* evo0 = evo;
+  double* evo1 = evo + n;
+  double s = 0;
+  for (long q = 0; q < mm; ++q) s += std::norm(V[q]);
+  const double vn = std::sqrt(s / mm);
+  for (long q = 0; q < mm; ++q) V[q] /= vn;
+  for (int i = 0; i < n; ++i) evo1[i] *= vn;
+  const int len = std::min(2, n);
+  double a = 0, b = 0;
+  for (int i = 0; i < len; ++i) {
+    a += evo0[i];
+    b += evo1[i];
+  }
+  const double ma = (float)(a / len), mb = (float)(b / len);  // stored in a float2
+  double s0 = 0;
+  for (int i = 0; i < len; ++i) {
+    evo0[i] = (evo0[i] - ma) * 0.99 + 1;
+    evo1[i] = evo1[i] - mb;
+    s0 += evo0[i] * evo0[i];
+  }
+  const double cn = std::sqrt(s0 / len);
+  for (int i = 0; i < len; ++i) evo0[i] /= cn;
+  for (long q = 0; q < mm; ++q) P[q] *= cn;
+}
+
+// frame_modulus on one frame's K spectra (k x m x m, replaced by chi); returns the error.
+double orc_mls_frame_modulus(double* psi, const double* amp, int k, int m) {
+  Fft1 p(m);
+  return frame_modulus(C(psi), (long)m * m, amp, k, m, p);
+}
+
+// variable_probe_update for one group; pu G x m x m, pum m x m, op / chi0 G x m x m, vmode m x m
+// (in/out), evo1 n (in/out), roi n x 2, illum rows x cols.
+void orc_mls_variable_probe(int G, int n, int m, const int32_t* gi, const int32_t* roi, const double* illum, int rows,
+                            int cols, const double* pu, const double* pum, const double* op, const double* chi0,
+                            double* vmode, double* evo1) {
+  variable_probe_update(G, n, m, gi, roi, illum, rows, cols, C(pu), C(pum), C(op), C(chi0), C(vmode), evo1);
+}
+
+// mode_directions for one group (pu, pum, oup out).
+void orc_mls_mode_directions(int ll, int llp, int G, int m, int subpix, const int32_t* gi, const int32_t* roi,
+                             const float* sub, const double* illum, int cols, double max_illum, const double* chi,
+                             const double* op, const double* vp, const double* probe, int probe_rec, int obj_rec,
+                             double* pu, double* pum, double* oup) {
+  Fft1 p(m);
+  mode_directions(ll, llp, G, m, subpix, gi, roi, sub, illum, cols, max_illum, C(chi), C(op), C(vp), C(probe),
+                  probe_rec != 0, obj_rec != 0, C(pu), C(pum), C(oup), p);
 }
 
 // Probe::initEvo (Probe.cpp:507-613) with h_preCalillum (ProbeKernels.cu:691-730).
@@ -266,7 +468,6 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
                        const int32_t* gframes, const int32_t* goffs, int n_groups, double* ferr) {
   const int iter = prm->iter;
   const long mm = (long)m * m, canvas = (long)rows * cols;
-  const double nrm = 1.0 / (double)mm;
   cd* O = C(obj);
   cd* P = C(probe);
   cd* V = C(vmode);
@@ -279,23 +480,7 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
   // MLS.cpp:86-89
   if (iter == 1 || iter >= prm->pps) roi_of(pos, n, m, rows, cols, roi, sub);
 
-  // Probe::calc_object_norm (Probe.cpp:671-699) over the precalculated box
-  {
-    const int x0 = prm->box[0] - 1, y0 = prm->box[2] - 1;
-    const int bx = prm->box[1] - prm->box[0] + 1, by = prm->box[3] - prm->box[2] + 1;
-    double sw = 0, so = 0;
-    for (int r = 0; r < bx; ++r)
-      for (int c = 0; c < by; ++c) {
-        const long q = (long)(x0 + r) * cols + y0 + c;
-        sw += illum[q] * illum[q];
-        so += std::norm(O[q] * illum[q]);
-      }
-    const double cnt = (double)bx * by;
-    const double onorm = std::sqrt(so / cnt) / std::sqrt(sw / cnt);
-    for (long q = 0; q < (long)k * mm; ++q) P[q] *= onorm;
-    for (long q = 0; q < mm; ++q) V[q] *= onorm;
-    for (long q = 0; q < canvas; ++q) O[q] *= 1.0 / onorm;
-  }
+  orc_mls_object_norm(prm->box, rows, cols, obj, illum, k, m, probe, vmode);
 
   // IPtychoScanMesh::updateScanPositionsDiff (float sums, IPtychoScanMesh.cpp:340-369)
   {
@@ -313,31 +498,7 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
     }
   }
 
-  // Probe::remove_extra_degree (Probe.cpp:615-669); its mean and renormalisation run over
-  // len = number of evolution rows (2), i.e. over the first two positions only.
-  if (prm->variable_probe_modes && iter > prm->probe_reconstruct) {
-    double s = 0;
-    for (long q = 0; q < mm; ++q) s += std::norm(V[q]);
-    const double vn = std::sqrt(s / mm);
-    for (long q = 0; q < mm; ++q) V[q] /= vn;
-    for (int i = 0; i < n; ++i) evo1[i] *= vn;
-    const int len = std::min(2, n);
-    double a = 0, b = 0;
-    for (int i = 0; i < len; ++i) {
-      a += evo0[i];
-      b += evo1[i];
-    }
-    const double ma = (float)(a / len), mb = (float)(b / len);  // stored in a float2
-    double s0 = 0;
-    for (int i = 0; i < len; ++i) {
-      evo0[i] = (evo0[i] - ma) * 0.99 + 1;
-      evo1[i] = evo1[i] - mb;
-      s0 += evo0[i] * evo0[i];
-    }
-    const double cn = std::sqrt(s0 / len);
-    for (int i = 0; i < len; ++i) evo0[i] /= cn;
-    for (long q = 0; q < mm; ++q) P[q] *= cn;
-  }
+  if (prm->variable_probe_modes && iter > prm->probe_reconstruct) orc_mls_remove_extra(n, m, probe, vmode, evo);
 
   // Probe::updated_cached_illumination (Probe.cpp:701-729): running average with the new
   // |P0|^2 projections.  MAX_ILLUM stays the initEvo value (this writes m_MAX_ILLUM).
@@ -379,22 +540,8 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
       // Fourier error (Diffractions.cpp:261-279; MLS.cpp:234-237: only for iter < 10, since
       // iter / min(20, 2^floor(2 + iter/50)) is a floating-point quotient) and
       // modulus constraint (Diffractions.cpp:281-310; DiffractionsKernels.cu:127-137)
-      const double* A = amp + (long)f * mm;
-      double e2 = 0;
-      for (long q = 0; q < mm; ++q) {
-        double s2 = 0;
-        for (int kk = 0; kk < k; ++kk) s2 += std::norm(psi[((size_t)kk * G + j) * mm + q]);
-        const double ap = std::sqrt(s2);
-        e2 += (A[q] - ap) * (A[q] - ap) / 0.25;
-        const double R = A[q] / (ap + 1e-9) - 1.0;
-        for (int kk = 0; kk < k; ++kk) psi[((size_t)kk * G + j) * mm + q] *= R;
-      }
-      if (iter < 10) ferr[f] = std::sqrt(e2 / mm);
-      for (int kk = 0; kk < k; ++kk) {
-        cd* ps = psi.data() + ((size_t)kk * G + j) * mm;
-        fft2(ps, m, true, fp);
-        for (long q = 0; q < mm; ++q) ps[q] *= nrm;
-      }
+      const double ev = frame_modulus(psi.data() + (size_t)j * mm, (long)G * mm, amp + (long)f * mm, k, m, fp);
+      if (iter < 10) ferr[f] = ev;
     }
     const cd* chi0 = psi.data();
 
@@ -404,33 +551,10 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
       const bool probe_rec = iter >= prm->probe_reconstruct;
       const int llp = std::min(k, ll);
       const cd* chi = psi.data() + (size_t)(ll - 1) * G * mm;
-      std::fill(pum.begin(), pum.end(), cd(0));
-      if (probe_rec) {
-#pragma omp parallel for schedule(dynamic)
-        for (int j = 0; j < G; ++j) {
-          const int f = gi[j];
-          cd* u = pu.data() + j * mm;
-          for (long q = 0; q < mm; ++q) u[q] = chi[j * mm + q] * std::conj(op[j * mm + q]);
-          if (ll == 1 && prm->apply_subpix_shift) shift_frame(u, m, -sub[2 * f], -sub[2 * f + 1], fp);
-        }
-        // weight p / (p + 0.01 MAX_ILLUM) (d_addFactorDivide), projected, frame mean
-        const double fw = max_illum * 0.01;
-        for (int j = 0; j < G; ++j) {
-          const int f = gi[j];
-          for (int r = 0; r < m; ++r)
-            for (int c = 0; c < m; ++c) {
-              const double pv = illum[(long)(roi[2 * f] + r) * cols + roi[2 * f + 1] + c];
-              pum[(long)r * m + c] += pv / (pv + fw) * pu[j * mm + (long)r * m + c];
-            }
-        }
-        for (long q = 0; q < mm; ++q) pum[q] /= (double)G;
-      }
+      mode_directions(ll, llp, G, m, prm->apply_subpix_shift, gi, roi, sub, illum, cols, max_illum, chi, op.data(),
+                      vp.data(), P, probe_rec, obj_rec, pu.data(), pum.data(), oup.data(), fp);
       std::vector<cd> upd;  // object_upd_sum after the preconditioner
       if (obj_rec) {
-        for (int j = 0; j < G; ++j)
-          for (long q = 0; q < mm; ++q)
-            oup[j * mm + q] = (llp == 1) ? chi[j * mm + q] * std::conj(vp[j * mm + q])
-                                         : P[(llp - 1) * mm + q] * chi[j * mm + q];  // no conj (MLS.cpp:294-297)
         upd.assign(canvas, cd(0.0, (double)(float)2.220446049250313e-16));  // (0, DBL_EPSILON) as float
         if (prm->delta_p != 0) {
           for (int j = 0; j < G; ++j) {
@@ -449,8 +573,6 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
                   oup[j * mm + (long)r * m + c] = upd[(long)(roi[2 * f] + r) * cols + roi[2 * f + 1] + c];
             }
         }
-      } else {
-        std::fill(oup.begin(), oup.end(), cd(0));
       }
       // LSQ step (MLS.cpp:349-416)
       if (prm->beta_lsq > 0 && ll == 1 && obj_rec && probe_rec) {
@@ -489,53 +611,7 @@ void orc_mls_iteration(const MlsPrm* prm, int n, int m, int k, int rows, int col
       // Probe::update_variable_probe (Probe.cpp:1366-1517)
       if (var_any && iter > prm->probe_reconstruct + 1 && ll == 1) {
         if (prm->variable_probe) {
-          const double relax_u = (double)G / n;
-          double evn = 0;
-          for (int j = 0; j < G; ++j) evn += evo1[gi[j]] * evo1[gi[j]];
-          double pmax = 0;
-          for (long q = 0; q < canvas; ++q) pmax = std::max(pmax, illum[q]);
-          std::vector<cd> img((size_t)G * mm), acc(mm, cd(0));
-          for (int j = 0; j < G; ++j) {
-            const int f = gi[j];
-            double sre = 0;
-            for (int r = 0; r < m; ++r)
-              for (int c = 0; c < m; ++c) {
-                const long q = (long)r * m + c;
-                const double w = illum[(long)(roi[2 * f] + r) * cols + roi[2 * f + 1] + c] / pmax;
-                img[j * mm + q] = w * (pu[j * mm + q] - pum[q]);
-                sre += (V[q] * std::conj(img[j * mm + q])).real();
-              }
-            const double cj = (sre / mm + evo1[f]) / evn;
-            for (long q = 0; q < mm; ++q) acc[q] += img[j * mm + q] * cj;
-          }
-          double s2 = 0;
-          for (long q = 0; q < mm; ++q) {
-            acc[q] *= nrm;
-            s2 += std::norm(acc[q]);
-          }
-          const double sc = relax_u / std::max(1.0, std::sqrt(s2 / mm));
-          double sv = 0;
-          for (long q = 0; q < mm; ++q) {
-            V[q] += acc[q] * sc;
-            sv += std::norm(V[q]);
-          }
-          const double vn = std::sqrt(sv / mm);
-          for (long q = 0; q < mm; ++q) V[q] *= 1.0 / vn;
-          std::vector<double> den(G), num(G);
-          double dsum = 0;
-          for (int j = 0; j < G; ++j) {
-            double d = 0, u = 0;
-            for (long q = 0; q < mm; ++q) {
-              const cd ov = op[j * mm + q] * V[q];
-              d += std::norm(ov);
-              u += (chi0[j * mm + q] * std::conj(ov)).real();
-            }
-            den[j] = d / mm;
-            num[j] = u / mm;
-            dsum += den[j];
-          }
-          const double m3 = dsum / G;
-          for (int j = 0; j < G; ++j) evo1[gi[j]] += num[j] / (den[j] + 0.1 * m3);
+          variable_probe_update(G, n, m, gi, roi, illum, rows, cols, pu.data(), pum.data(), op.data(), chi0, V, evo1);
         }
         if (prm->variable_intensity) {
           for (int j = 0; j < G; ++j) {
