@@ -52,6 +52,15 @@ int ptycho_mls_canvas(const float* pos, int64_t n, const float* extent, int32_t 
 int ptycho_mls_partitions(int64_t n, int32_t niter, uint64_t seed, int32_t* frames_out, int32_t* offsets_out,
                           int32_t* n_groups, int32_t* n_partitions);
 
+/* Initial MLs probe modes (Probe::initMLH, Probe.cpp:376-494) with its random draws given:
+ * out (K x M x M complex64) holds mode 0 on entry when it is non-zero (a guess), else all zeros
+ * and mode 0 is the Fresnel-zone-plate model; modes 1..K-1 are mode 0 tilted by
+ * exp(-2 pi i (draws[2i-2] (col/M - 0.5) + draws[2i-1] (row/M - 0.5))); all modes are then replaced
+ * by the eigenmodes of their overlap matrix, strongest first.  engine: a ptyk_ctx of frame size M
+ * (its FFT forms the zone-plate probe). */
+int ptycho_mls_probe_modes(void* engine, int32_t m, int32_t k, double lambda, double dx_s, double beam_size,
+                           const float* draws, float* out);
+
 /* Complex CSV ("a+bj", ", "-separated rows) or raw binary, as Cuda2DArray::save / load
  * (Cuda2DArray.cpp:46-108, Cuda2DArray.hpp:439-464). */
 int ptycho_save_complex_csv(const char* path, const float* c64, int32_t rows, int32_t cols, int32_t binary);

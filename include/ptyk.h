@@ -114,6 +114,10 @@ int ptyk_init_probe(ptyk_ctx* ctx, double beam_size, double dx_s);
  * endModalReconstruction, Probe.cpp:1768-1782; initProbeModes, Probe.cpp:320-330). */
 int ptyk_begin_modal(ptyk_ctx* ctx);
 int ptyk_end_modal(ptyk_ctx* ctx);
+/* Probe::orthogonalize (Probe.cpp:228-242) on the handle's probe modes: for i >= 1 and
+ * j = i-1 .. 0, P_i -= P_j sum(conj(v_i) P_j) / sum |P_j|^2 with v_i mode i before the sweep (the
+ * reference's h_projectUtoV coefficient, ProbeKernels.cu:509-515). */
+int ptyk_orthogonalize_modes(ptyk_ctx* ctx);
 
 /* One ePIE iteration over all positions in the given order (ePIE::iteration,
  * ePIE.cpp:65-129).  order: N frame indices (a permutation).  err (nullable)

@@ -383,6 +383,66 @@ class MlsState:
                                 _ptr(self.ferr))
 
 
+def mls_zone_plate_probe(m: int, lambd: float, dx_s: float, beam_size: float) -> np.ndarray:
+    """Mode 0 of Probe::initMLH without a guess (h_initProbeMLH / d_initProbeMLH / h_endProbeMLH,
+    ProbeKernels.cu:220-366, 592-641), FP64 with numpy's FFT.  Grid x_t = -dx M/2 + t dx M/(M-1)
+    (t along columns, the same along rows), aperture D_FZP/2 >= r >= D_H/2 (the host passes the
+    diameters halved), kernel exp(i pi fzp r^2) exp(i k r^2 / 2z) placed fftshifted; forward FFT;
+    times exp(i k z) exp(i k (u_x^2 + u_y^2) / 2z) with the frequency grid split at M/2; shifted back."""
+    rn, drn = 90e-6, 50e-9
+    fl = 2 * rn * drn / lambd
+    d_fzp, d_h = 180e-6, 60e-6
+    dx_fzp = lambd * fl / (m * dx_s)
+    fzp = -2 / (lambd * 2 * fl) * np.pi
+    idx = np.arange(m)
+    x = -dx_fzp * m / 2 + dx_fzp * m / (m - 1) * idx
+    xx, yy = np.meshgrid(x, x)  # xx varies along columns, yy along rows
+    r2 = xx ** 2 + yy ** 2
+    r = np.sqrt(r2)
+    k = 2 * np.pi / lambd
+    z = fl + beam_size
+    inside = (r <= d_fzp / 2) & (r >= d_h / 2)
+    kern = np.where(inside, np.exp(1j * (fzp * r2 + k / (2 * z) * r2)), 0)
+    fu = lambd * (fl + beam_size) / dx_fzp
+    step = fu / (m - 1)
+    u = np.where(idx < m // 2, -fu / 2 + step * (m // 2) + step * idx, -fu / 2 + step * (idx - m // 2))
+    ux, uy = np.meshgrid(u, u)
+    pf = np.exp(1j * k * z) * np.exp(1j * k / (2 * z) * (ux ** 2 + uy ** 2))
+    f = np.fft.fft2(np.fft.fftshift(kern))
+    return np.fft.ifftshift(f * pf)
+
+
+def mls_probe_modes(m: int, k: int, lambd: float, dx_s: float, beam_size: float, draws, mode0=None) -> np.ndarray:
+    """Probe::initMLH (Probe.cpp:376-494) in FP64: mode 0 (a guess or the zone-plate probe); modes
+    i >= 1 = mode 0 exp(-2 pi i (r1 (c/M - 0.5) + r2 (r/M - 0.5))) (h_initVarProbe with
+    d_imshift_fft, ProbeKernels.cu:756-772, utilitiesKernels.cu:784-826), (r1, r2) = draws[2i-2:2i];
+    then the eigenmodes of A_ij = sum conj(P_i) P_j (h_Orth / d_innerProductOne,
+    utilitiesKernels.cu:870-883), out_e = sum_r P_r v_re (d_innerProductModes,
+    utilitiesKernels.cu:888-907), strongest |eigenvalue| first, each eigenvector scaled so its
+    largest component is real positive (the reference leaves Eigen's order and phases)."""
+    p0 = mls_zone_plate_probe(m, lambd, dx_s, beam_size) if mode0 is None else _c128(mode0)
+    modes = np.zeros((k, m, m), np.complex128)
+    modes[0] = p0
+    d = np.asarray(draws, np.float64)
+    c = np.arange(m) / m - 0.5
+    for i in range(1, k):
+        r1, r2 = d[2 * i - 2], d[2 * i - 1]
+        modes[i] = p0 * np.exp(-2j * np.pi * (r1 * c[None, :] + r2 * c[:, None]))
+    if k == 1:
+        return modes
+    flat = modes.reshape(k, -1)
+    a = np.conj(flat) @ flat.T
+    w, v = np.linalg.eigh(a)
+    order = sorted(range(k), key=lambda e: -abs(w[e]))
+    out = np.zeros_like(flat)
+    for o, e in enumerate(order):
+        vec = v[:, e]
+        big = int(np.argmax(np.abs(vec)))
+        vec = vec * np.conj(vec[big]) / abs(vec[big])
+        out[o] = vec @ flat
+    return out.reshape(k, m, m)
+
+
 # ---------------------------------------------------------------------------- warm-start network
 def nn_num_params(c: int) -> int:
     return int(lib().orc_nn_num_params(c))

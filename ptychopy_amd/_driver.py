@@ -25,7 +25,7 @@ DRIVER_EXPORTS = (
     "ptycho_phaser_object_shape", "ptycho_phaser_get_object", "ptycho_phaser_probe_shape", "ptycho_phaser_get_probe",
     "ptycho_phaser_num_positions", "ptycho_phaser_get_positions", "ptycho_phaser_get_errors",
     "ptycho_phaser_phase_seconds", "ptycho_phaser_engine", "ptycho_mesh_cartesian_raw", "ptycho_mls_canvas",
-    "ptycho_mls_partitions", "ptycho_phaser_get_fourier_errors",
+    "ptycho_mls_partitions", "ptycho_phaser_get_fourier_errors", "ptycho_mls_probe_modes",
 )
 
 _bound = False
@@ -48,8 +48,24 @@ def lib():
         L.ptycho_mls_canvas.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ptycho_mls_partitions.argtypes = [C.c_int64, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p,
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ptycho_mls_probe_modes.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                             C.c_void_p, C.c_void_p]
         _bound = True
     return L
+
+
+def mls_probe_modes(engine, k: int, lambd: float, dx_s: float, beam_size: float, draws, mode0=None) -> np.ndarray:
+    """Initial MLs probe modes (Probe::initMLH) with the given draws (2 per extra mode); mode0:
+    optional guess for mode 0.  engine: an _engine.Engine of the frame size (its FFT is used)."""
+    m = engine.m
+    out = np.zeros((k, m, m), np.complex64)
+    if mode0 is not None:
+        out[0] = mode0
+    d = np.ascontiguousarray(draws, dtype=np.float32)
+    if lib().ptycho_mls_probe_modes(engine.h, m, k, lambd, dx_s, beam_size, d.ctypes.data_as(C.c_void_p),
+                                    out.ctypes.data_as(C.c_void_p)) != 0:
+        raise RuntimeError("ptycho_mls_probe_modes failed")
+    return out
 
 
 def sample_pixel_size(lambd: float, z: float, size: int, dx_d: float) -> float:

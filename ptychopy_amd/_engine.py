@@ -33,7 +33,7 @@ EXPORTS = (
     "ptyk_get_frames_f32", "ptyk_get_frame_scales", "ptyk_set_positions", "ptyk_get_roi", "ptyk_set_object",
     "ptyk_get_object", "ptyk_clear_object", "ptyk_set_probe", "ptyk_get_probe", "ptyk_set_psi", "ptyk_get_psi",
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
-    "ptyk_end_modal", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
+    "ptyk_end_modal", "ptyk_orthogonalize_modes", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
     "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
     "ptyk_mls_iteration", "ptyk_mls_iteration_timed", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
@@ -121,6 +121,7 @@ def load():
         getattr(lib, name).argtypes = [P, P]
     lib.ptyk_set_frames_f16.argtypes = [P, P, P]
     for name in ("ptyk_clear_object", "ptyk_refresh_maxima", "ptyk_begin_modal", "ptyk_end_modal",
+                 "ptyk_orthogonalize_modes",
                  "ptyk_synchronize"):
         getattr(lib, name).argtypes = [P]
     lib.ptyk_init_probe.argtypes = [P, C.c_double, C.c_double]
@@ -309,6 +310,10 @@ class Engine:
 
     def end_modal(self):
         self._chk(self.lib.ptyk_end_modal(self.h))
+
+    def orthogonalize_modes(self):
+        """Probe::orthogonalize on the handle's probe modes."""
+        self._chk(self.lib.ptyk_orthogonalize_modes(self.h))
 
     def epie_iteration(self, order, flags: int = 0) -> float:
         o = np.ascontiguousarray(order, dtype=np.int32)

@@ -236,6 +236,18 @@ int ptycho_mesh_spiral(const ptycho_mesh_params* p, float* pos_out, int32_t* can
   return mesh_out(m, pos_out, canvas_out, extent_out);
 }
 
+int ptycho_mls_probe_modes(void* engine, int32_t m, int32_t k, double lambda, double dx_s, double beam_size,
+                           const float* draws, float* out) {
+  if (!engine || !out || m < 2 || k < 1 || (k > 1 && !draws) || !(lambda > 0) || !(dx_s > 0)) return -1;
+  std::vector<float> modes(out, out + 2 * (size_t)k * m * m);
+  std::string err;
+  const int rc = ptycho::init_mlh_modes(static_cast<ptyk_ctx*>(engine), m, k, lambda, dx_s, beam_size, draws, true,
+                                        &modes, &err);
+  if (rc != PTYK_OK) return -1;
+  std::copy(modes.begin(), modes.end(), out);
+  return 0;
+}
+
 int ptycho_save_complex_csv(const char* path, const float* c64, int32_t rows, int32_t cols, int32_t binary) {
   return path && c64 && ptycho::save_complex(path, c64, rows, cols, binary != 0) ? 0 : -1;
 }

@@ -711,93 +711,80 @@ void hermitian_eig(std::vector<std::complex<double>>& a, int k, std::vector<doub
 }
 }  // namespace
 
-// Probe::initMLH (Probe.cpp:376-494): mode 0 from the guess or the Fresnel-zone-plate model;
-// modes 1..K-1 are mode 0 with random linear phase tilts (h_initVarProbe, ProbeKernels.cu:756-772,
-// d_imshift_fft utilitiesKernels.cu:784-826); then all modes are replaced by the eigenmodes of
-// their overlap matrix A_ij = sum conj(P_i) P_j (ComplexEigenSolver), strongest first.
-bool IPhaser::mls_probe_modes(std::vector<float>* modes) {
+// Probe::initMLH (Probe.cpp:376-494) given its random draws: mode 0 is the given one when it is
+// non-zero, else the
+// Fresnel-zone-plate model (h_initProbeMLH / d_initProbeMLH / h_endProbeMLH, ProbeKernels.cu:220-366,
+// 592-641): the aperture D_FZP/2 >= r >= D_H/2 times the lens chirp exp(i fzp pi r^2) and the
+// propagator exp(i k r^2 / 2z), placed fftshifted, forward FFT (on the engine), times the far-field
+// factor exp(i k z) exp(i k u^2 / 2z), shifted back.  The phase arguments are evaluated in double:
+// the reference forms them in float (up to ~3e-4 rad of rounding at the zone-plate edge, in an order
+// its compiler's contractions decide).  Modes 1..K-1 (unless all were given) are mode 0 tilted by
+// exp(-2 pi i (r1 (c/M - 0.5) + r2 (r/M - 0.5))) (h_initVarProbe, ProbeKernels.cu:756-772;
+// d_imshift_fft, utilitiesKernels.cu:784-826) with (r1, r2) = draws[2(i-1)], draws[2(i-1)+1]; then
+// all modes are replaced by the eigenmodes of A_ij = sum conj(P_i) P_j, strongest |eigenvalue| first,
+// each eigenvector's phase fixed so its largest component is real and positive.
+int init_mlh_modes(ptyk_ctx* ctx, int m, int k, double lambda, double dx_s, double beam_size, const float* draws,
+                   bool tilt, std::vector<float>* modes, std::string* err) {
   typedef std::complex<double> cd;
-  const int m = eng.m, k = eng.k;
   const size_t mm = (size_t)m * m;
-  modes->assign(2 * mm * k, 0.f);
-  if (!prm.r.probeGuess.empty()) {
-    std::vector<float> g;
-    int r = 0, c = 0;
-    if (!load_complex_csv(prm.r.probeGuess, &g, &r, &c) || r != m || c != m) {
-      err = "Probe guess file (" + prm.r.probeGuess + ") not found or not " + std::to_string(m) + "x" +
-            std::to_string(m);
-      return false;
-    }
-    std::copy(g.begin(), g.end(), modes->begin());
-  } else if (!probe_guess.empty()) {
-    // probeNP: one mode (size x size) as in the reference, or all K modes stacked (K*size x size)
-    if (probe_guess_cols != m || (probe_guess_rows != m && probe_guess_rows != k * m)) {
-      err = "probe guess must be size x size or (probeModes*size) x size";
-      return false;
-    }
-    std::copy(probe_guess.begin(), probe_guess.end(), modes->begin());
-  } else {
-    // Fresnel zone plate at the focal distance plus beamSize (h_initProbeMLH / d_initProbeMLH,
-    // ProbeKernels.cu:220-366, float arithmetic), then FFT and the far-field phase
-    const double lambda = prm.e.lambda, Rn = 90e-6, dRn = 50e-9, fl = 2 * Rn * dRn / lambda;
+  if (modes->size() != 2 * mm * k) modes->assign(2 * mm * k, 0.f);
+  if (!draws && tilt && k > 1) {
+    *err = "draws required";
+    return PTYK_EINVAL;
+  }
+  bool have_mode0 = false;
+  for (size_t q = 0; q < 2 * mm && !have_mode0; ++q) have_mode0 = (*modes)[q] != 0.f;
+  if (!have_mode0) {
+    const double Rn = 90e-6, dRn = 50e-9, fl = 2 * Rn * dRn / lambda;
     const double D_FZP = 180e-6, D_H = 60e-6;
-    const double dx_fzp = lambda * fl / (m * prm.e.dx_s);
-    const float fzp = (float)((-2 / (lambda * 2 * fl)) * M_PI);
-    const float step = (float)(dx_fzp * m / (m - 1)), start = (float)(-dx_fzp * m / 2);
-    const double kk = 2 * M_PI / lambda, z = fl + prm.e.beamSize;
-    const std::complex<float> kz((float)std::cos(kk * z), (float)std::sin(kk * z));
-    const float ekdz = (float)(kk / (2 * z));
-    const double fu = lambda * (fl + prm.e.beamSize) / dx_fzp;
-    const float fustep = (float)(fu / (m - 1)), startfu = (float)(-fu / 2 + fu / (m - 1) * (m / 2));
-    const float rf = (float)(D_FZP / 2), rh = (float)(D_H / 2), mfu2 = (float)(-fu / 2);
-    std::vector<std::complex<float>> kern(mm), pf(mm);
+    const double dx_fzp = lambda * fl / (m * dx_s);
+    const double fzp = (-2 / (lambda * 2 * fl)) * M_PI;
+    const double step = dx_fzp * m / (m - 1), start = -dx_fzp * m / 2;
+    const double kk = 2 * M_PI / lambda, z = fl + beam_size;
+    const cd kz = std::polar(1.0, kk * z);
+    const double ekdz = kk / (2 * z);
+    const double fu = lambda * (fl + beam_size) / dx_fzp;
+    const double fustep = fu / (m - 1), startfu = -fu / 2 + fu / (m - 1) * (m / 2);
+    const double rf = D_FZP / 2, rh = D_H / 2;
+    std::vector<cd> pf(mm);
+    std::vector<float> f(2 * mm);
     for (int b = 0; b < m; ++b)
       for (int t = 0; t < m; ++t) {
-        const float x = start + step * t, y = start + step * b;
-        const float r2 = x * x + y * y;
-        const float rr = std::sqrt(r2);
-        const std::complex<float> T(std::cos(fzp * r2), std::sin(fzp * r2));
-        const float C = rr <= rf ? 1.f : 0.f, H = rr >= rh ? 1.f : 0.f;
-        const float lx = t < m / 2 ? startfu + fustep * t : mfu2 + fustep * (t - m / 2);
-        const float ly = b < m / 2 ? startfu + fustep * b : mfu2 + fustep * (b - m / 2);
-        const float su = ekdz * (lx * lx + ly * ly), sx = ekdz * r2;
-        pf[(size_t)b * m + t] = kz * std::complex<float>(std::cos(su), std::sin(su));
-        const std::complex<float> kv = (C * T * H) * std::complex<float>(std::cos(sx), std::sin(sx));
-        kern[(size_t)((b + m / 2) % m) * m + (t + m / 2) % m] = kv;  // fftshift placement
+        const double x = start + step * t, y = start + step * b;
+        const double r2 = x * x + y * y, rr = std::sqrt(r2);
+        const double ap = (rr <= rf && rr >= rh) ? 1.0 : 0.0;
+        const double lx = t < m / 2 ? startfu + fustep * t : -fu / 2 + fustep * (t - m / 2);
+        const double ly = b < m / 2 ? startfu + fustep * b : -fu / 2 + fustep * (b - m / 2);
+        pf[(size_t)b * m + t] = kz * std::polar(1.0, ekdz * (lx * lx + ly * ly));
+        const cd kv = ap * std::polar(1.0, fzp * r2 + ekdz * r2);
+        const size_t d = (size_t)((b + m / 2) % m) * m + (t + m / 2) % m;  // fftshift placement
+        f[2 * d] = (float)kv.real();
+        f[2 * d + 1] = (float)kv.imag();
       }
-    std::vector<float> f(2 * mm);
-    for (size_t q = 0; q < mm; ++q) {
-      f[2 * q] = kern[q].real();
-      f[2 * q + 1] = kern[q].imag();
-    }
-    if (ptyk_fft2(eng.ctx, f.data(), 1, 0) != PTYK_OK) {
-      err = ptyk_last_error(eng.ctx);
-      return false;
+    const int rc = ptyk_fft2(ctx, f.data(), 1, 0);
+    if (rc != PTYK_OK) {
+      *err = ptyk_last_error(ctx);
+      return rc;
     }
     for (int b = 0; b < m; ++b)
       for (int t = 0; t < m; ++t) {
         const size_t q = (size_t)b * m + t;
-        const std::complex<float> v = std::complex<float>(f[2 * q], f[2 * q + 1]) * pf[q];
+        const cd v = cd(f[2 * q], f[2 * q + 1]) * pf[q];
         const size_t d = (size_t)((b + m / 2) % m) * m + (t + m / 2) % m;
-        (*modes)[2 * d] = v.real();
-        (*modes)[2 * d + 1] = v.imag();
+        (*modes)[2 * d] = (float)v.real();
+        (*modes)[2 * d + 1] = (float)v.imag();
       }
   }
-  const bool all_given = !probe_guess.empty() && prm.r.probeGuess.empty() && probe_guess_rows == k * m;
-  Normal nrm(prm.r.seed + 303);
-  if (!all_given)
+  if (tilt)
     for (int i = 1; i < k; ++i) {
-      const float r1 = (float)nrm(), r2 = (float)nrm();
+      const double r1 = draws[2 * (i - 1)], r2 = draws[2 * (i - 1) + 1];
       for (int r = 0; r < m; ++r)
         for (int c = 0; c < m; ++c) {
-          const float gx = r1 * ((float)c / m - 0.5f), gy = r2 * ((float)r / m - 0.5f);
-          const float ax = (float)(2 * M_PI * gx), ay = (float)(2 * M_PI * gy);
-          const std::complex<float> t = std::complex<float>(std::cos(ax), -std::sin(ax)) *
-                                        std::complex<float>(std::cos(ay), -std::sin(ay));
+          const double ph = -2 * M_PI * (r1 * ((double)c / m - 0.5) + r2 * ((double)r / m - 0.5));
           const size_t q = (size_t)r * m + c;
-          const std::complex<float> v = std::complex<float>((*modes)[2 * q], (*modes)[2 * q + 1]) * t;
-          (*modes)[2 * (i * mm + q)] = v.real();
-          (*modes)[2 * (i * mm + q) + 1] = v.imag();
+          const cd v = cd((*modes)[2 * q], (*modes)[2 * q + 1]) * std::polar(1.0, ph);
+          (*modes)[2 * (i * mm + q)] = (float)v.real();
+          (*modes)[2 * (i * mm + q) + 1] = (float)v.imag();
         }
     }
   if (k > 1) {
@@ -818,7 +805,6 @@ bool IPhaser::mls_probe_modes(std::vector<float>* modes) {
     std::vector<float> out(modes->size());
     for (int o = 0; o < k; ++o) {
       const int e = ord[o];
-      // eigenvector phase fixed by its largest component (real, positive)
       int big = 0;
       for (int r = 1; r < k; ++r)
         if (std::abs(v[(size_t)r * k + e]) > std::abs(v[(size_t)big * k + e])) big = r;
@@ -833,7 +819,45 @@ bool IPhaser::mls_probe_modes(std::vector<float>* modes) {
     }
     *modes = out;
   }
-  return true;
+  return PTYK_OK;
+}
+
+namespace {
+// the MLs initial-mode draws: two standard normals per extra mode (the reference's time-seeded
+// normalRandom, Probe.cpp:428-432)
+std::vector<float> mlh_draws(uint64_t seed, int k) {
+  Normal nrm(seed + 303);
+  std::vector<float> d(2 * (size_t)std::max(k - 1, 0));
+  for (auto& x : d) x = (float)nrm();
+  return d;
+}
+}  // namespace
+
+bool IPhaser::mls_probe_modes(std::vector<float>* modes) {
+  const int m = eng.m, k = eng.k;
+  const size_t mm = (size_t)m * m;
+  modes->assign(2 * mm * k, 0.f);
+  if (!prm.r.probeGuess.empty()) {
+    std::vector<float> g;
+    int r = 0, c = 0;
+    if (!load_complex_csv(prm.r.probeGuess, &g, &r, &c) || r != m || c != m) {
+      err = "Probe guess file (" + prm.r.probeGuess + ") not found or not " + std::to_string(m) + "x" +
+            std::to_string(m);
+      return false;
+    }
+    std::copy(g.begin(), g.end(), modes->begin());
+  } else if (!probe_guess.empty()) {
+    // probeNP: one mode (size x size) as in the reference, or all K modes stacked (K*size x size)
+    if (probe_guess_cols != m || (probe_guess_rows != m && probe_guess_rows != k * m)) {
+      err = "probe guess must be size x size or (probeModes*size) x size";
+      return false;
+    }
+    std::copy(probe_guess.begin(), probe_guess.end(), modes->begin());
+  }
+  const bool all_given = !probe_guess.empty() && prm.r.probeGuess.empty() && probe_guess_rows == k * m;
+  const std::vector<float> draws = mlh_draws(prm.r.seed, k);
+  return init_mlh_modes(eng.ctx, m, k, prm.e.lambda, prm.e.dx_s, prm.e.beamSize, draws.data(), !all_given, modes,
+                        &err) == PTYK_OK;
 }
 
 // IPhaser::prePhase, MLs branch (IPhaser.cpp:283-295, 318-358): probe modes, object (guess or

@@ -276,6 +276,11 @@ class IPhaser {
   double t0_ = 0.0;
 };
 
+// Probe::initMLH with explicit draws (2 per extra mode): modes (K x M x M complex64, mode 0 given
+// when non-zero, else the zone-plate model) are replaced by the initial MLs modes.
+int init_mlh_modes(ptyk_ctx* ctx, int m, int k, double lambda, double dx_s, double beam_size, const float* draws,
+                   bool tilt, std::vector<float>* modes, std::string* err);
+
 // CSV "a+bj" writer / reader of Cuda2DArray (Cuda2DArray.cpp:46-108, Cuda2DArray.hpp:439-464).
 bool save_complex(const std::string& path, const float* c64, int rows, int cols, bool binary);
 bool load_complex_csv(const std::string& path, std::vector<float>* c64, int* rows, int* cols);

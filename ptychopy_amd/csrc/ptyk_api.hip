@@ -499,6 +499,17 @@ int ptyk_begin_modal(ptyk_ctx* c) {
   return PTYK_OK;
 }
 
+int ptyk_orthogonalize_modes(ptyk_ctx* c) {
+  if (!c) return PTYK_EINVAL;
+  if (!c->have_probe) return fail(c, PTYK_ESTATE, "probe must be set");
+  hipSetDevice(c->cfg.device);
+  if (c->cfg.n_modes > 1) {
+    HIPCHK(c, orthogonalize_modes(c->probe, c->probe_tmp, c->cfg.n_modes, c->cfg.frame_size, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return PTYK_OK;
+}
+
 int ptyk_end_modal(ptyk_ctx* c) {
   if (!c) return PTYK_EINVAL;
   c->modal = false;

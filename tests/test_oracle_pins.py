@@ -267,3 +267,49 @@ def test_dm_phase_clamp_acts_on_the_increment(orc, theta):
     else:
         expect[2:2 + m, 1:1 + m] = roi + 2 * s_o * p2
     assert rel_err(o, expect) < 1e-12
+
+
+# ------------------------------------------------------------------------------- initMLH
+def _annulus_pixels(m, lambd, dx_s):
+    """Grid pixels with D_H/2 <= r <= D_FZP/2 (30 um .. 90 um) on the zone-plate grid."""
+    fl = 2 * 90e-6 * 50e-9 / lambd
+    dx = lambd * fl / (m * dx_s)
+    x = -dx * m / 2 + dx * m / (m - 1) * np.arange(m)
+    r = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2)
+    return int(np.count_nonzero((r <= 90e-6) & (r >= 30e-6)))
+
+
+@pytest.mark.parametrize("m", [64, 128])
+def test_mls_zone_plate_parseval(orc, m):
+    """The zone-plate kernel has unit modulus on the annulus and zero elsewhere, the far-field
+    factor has unit modulus, and the forward FFT is unnormalised: sum |P0|^2 = M^2 x (annulus pixels)."""
+    lambd, dx_s = 2.48e-10, 5.631549e-09
+    p0 = orc.mls_zone_plate_probe(m, lambd, dx_s, 1e-7)
+    n_ann = _annulus_pixels(m, lambd, dx_s)
+    assert n_ann > 10
+    assert np.sum(np.abs(p0) ** 2) == pytest.approx(m * m * n_ann, rel=1e-12)
+
+
+def test_mls_initial_modes_orthogonal_strongest_first(orc):
+    """The eigenmodes are mutually orthogonal with powers in decreasing order, and the unitary
+    mixing keeps the incoherent sum: sum_a |P_a|^2 = K |P0|^2 at every pixel (each tilted copy has
+    the modulus of mode 0)."""
+    m, k = 64, 4
+    draws = [0.3, -0.8, 1.1, 0.4, -1.7, 0.2]
+    p0 = orc.mls_zone_plate_probe(m, 2.48e-10, 5.631549e-09, 1e-7)
+    modes = orc.mls_probe_modes(m, k, 2.48e-10, 5.631549e-09, 1e-7, draws)
+    g = modes.reshape(k, -1).conj() @ modes.reshape(k, -1).T
+    off = g - np.diag(np.diag(g))
+    assert np.abs(off).max() < 1e-9 * np.abs(g).max()
+    pw = np.real(np.diag(g))
+    assert np.all(np.diff(pw) < 0)
+    assert np.allclose(np.sum(np.abs(modes) ** 2, axis=0), k * np.abs(p0) ** 2, rtol=1e-9, atol=1e-9 * pw[0])
+    # each eigenvector's largest component is real positive: the mode's overlap with the tilted
+    # copy it draws most from is real positive
+    tilts = np.stack([p0] + [p0 * np.exp(-2j * np.pi * (draws[2 * i - 2] * (np.arange(m) / m - 0.5)[None, :] +
+                                                        draws[2 * i - 1] * (np.arange(m) / m - 0.5)[:, None]))
+                             for i in range(1, k)])
+    for a in range(k):
+        ov = tilts.reshape(k, -1).conj() @ modes[a].ravel()
+        big = int(np.argmax(np.abs(ov)))
+        assert abs(ov[big].imag) < 1e-9 * abs(ov[big]) and ov[big].real > 0
