@@ -57,6 +57,9 @@ struct ptyk_ctx {
   int* wave_row_ptr = nullptr;
   int* wave_row_frames = nullptr;
   std::vector<int> wave_lo, wave_hi;
+  // bumped whenever the DM launch tables or shapes may change (positions, tuning, a rebuild):
+  // part of the batch graph signature, so a captured graph never replays stale row ranges
+  uint64_t dm_gen = 0;
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> evpool;
   std::vector<int> evcat;
@@ -287,6 +290,7 @@ static void free_dm(ptyk_ctx* c) {
   c->wave_row_ptr = c->wave_row_frames = nullptr;
   c->dm_wave = c->dm_waves = 0;
   c->dm_started = false;
+  ++c->dm_gen;
 }
 
 int ptyk_set_positions(ptyk_ctx* c, const float* p) {
@@ -313,6 +317,7 @@ int ptyk_set_positions(ptyk_ctx* c, const float* p) {
   c->pos = pos;
   c->roi = roi;
   c->have_pos = true;
+  ++c->dm_gen;
   // DM's frame lists depend on the positions; rebuild them on the next DM call.
   if (c->row_ptr) {
     hipFree(c->row_ptr);
@@ -598,6 +603,7 @@ static int dm_prepare(ptyk_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->row_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors die at the end of this block
+    ++c->dm_gen;
   }
   if (wave > 0 && c->dm_wave != wave) {
     // per wave: frames covering each object row (offsets into one frame-id array)
@@ -638,6 +644,7 @@ static int dm_prepare(ptyk_ctx* c) {
     c->wave_hi = hi;
     c->dm_wave = wave;
     c->dm_waves = nw;
+    ++c->dm_gen;
     if (!c->d_obj) HIPCHK(c, dalloc(&c->d_obj, (size_t)rows * c->cfg.obj_cols));
     if (!c->d_probe) HIPCHK(c, dalloc(&c->d_probe, (size_t)c->mm));
   }
@@ -775,6 +782,7 @@ int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
   if (key == PTYK_TUNE_DM_WAVE) {
     if (value < -1) return fail(c, PTYK_EINVAL, "dm wave must be -1, 0 or a frame count");
     c->dm_wave_cfg = value;
+    ++c->dm_gen;
     return PTYK_OK;
   }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
@@ -1079,7 +1087,8 @@ static std::vector<uintptr_t> dm_signature(const ptyk_ctx* c) {
   return {(uintptr_t)c->psi, (uintptr_t)c->row_ptr, (uintptr_t)c->row_frames, (uintptr_t)c->probe_partial,
           (uintptr_t)c->obj_partial, (uintptr_t)c->probe_partial_sum, (uintptr_t)c->d_obj, (uintptr_t)c->d_probe,
           (uintptr_t)c->wave_row_ptr, (uintptr_t)c->wave_row_frames, (uintptr_t)c->dm_wave, (uintptr_t)c->dm_waves,
-          (uintptr_t)c->probe_chunks, (uintptr_t)c->frames_per_chunk, (uintptr_t)dm_wave_frames(c)};
+          (uintptr_t)c->probe_chunks, (uintptr_t)c->frames_per_chunk, (uintptr_t)dm_wave_frames(c),
+          (uintptr_t)c->dm_gen};
 }
 
 static void batch_drop_graphs(ptyk_batch* b) {
@@ -1204,6 +1213,12 @@ int ptyk_batch_dm_iteration(ptyk_batch* b, uint32_t flags) {
       return PTYK_EDEVICE;
     }
     b->graphs.push_back({flags, exec});
+  }
+  // the replay follows whatever is already queued on each scan's own stream (a standalone
+  // iteration, a copy): the capture path synchronised them, a replay must order itself
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    BCHK(b, hipEventRecord(b->join[i], b->ctx[i]->stream));
+    BCHK(b, hipStreamWaitEvent(b->s0, b->join[i], 0));
   }
   BCHK(b, hipGraphLaunch(exec, b->s0));
   // later calls on any scan's stream see this iteration

@@ -91,3 +91,37 @@ def test_autotuner_persists_a_timing_table(gpu, tmp_path):
     assert set(int(c) for c in table[key]) == set(times)
     # a second call reads the table instead of timing again
     assert autotune.tune_dm_wave(P["pos"], 128, P["shape"], path=path) == (best, times)
+
+
+def test_batch_replays_after_positions_move(gpu):
+    """Moving one scan's positions between batch iterations rebuilds its frame lists (often at the
+    same device addresses); the replayed iterations must use the new row ranges: bitwise equal to a
+    standalone run with the same move."""
+    from ptychopy_amd._engine import Batch
+    P = make_problem(m=128, n_side=4, seed=17)
+    moved = P["pos"] + np.float32(1.0)
+    assert (np.ceil(moved.max(axis=0)) + 128 <= np.array(P["shape"])).all()
+
+    def standalone(move):
+        e = _engine(P)
+        for i in range(1, 7):
+            if move and i == 4:
+                e.set_positions(moved)
+            e.dm_iteration(0)
+        out = e.get_object(), e.get_psi()
+        e.close()
+        return out
+
+    ref_still, ref_moved = standalone(False), standalone(True)
+    e1, e2 = _engine(P), _engine(P)
+    with Batch([e1, e2]) as b:
+        for i in range(1, 4):
+            b.dm_iteration(0)
+        e2.set_positions(moved)
+        for i in range(4, 7):
+            b.dm_iteration(0)
+        b.synchronize()
+        assert np.array_equal(e1.get_object(), ref_still[0]) and np.array_equal(e1.get_psi(), ref_still[1])
+        assert np.array_equal(e2.get_object(), ref_moved[0]) and np.array_equal(e2.get_psi(), ref_moved[1])
+    e1.close()
+    e2.close()
