@@ -283,6 +283,19 @@ int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);
 int ptyk_simulate(ptyk_ctx* ctx, const float* obj, int32_t rows, int32_t cols, const float* probe,
                   const float* pos_rc, int64_t n, int32_t squared, float* out);
 
+/* Allocation-free simulator (no reference counterpart; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving or data-generation loops).  Same results as
+ * ptyk_simulate, but obj (rows x cols complex64), probe (M x M complex64), pos (n x (row, col)
+ * float32) and out (n x M x M float32) are DEVICE pointers, the call only enqueues on the handle's
+ * stream (ptyk_synchronize waits), and the exit waves are staged in a fixed device array of
+ * PTYK_NOALLOC_SCRATCH_BYTES per GPU instead of allocated memory: n x M x M x 8 bytes must fit in
+ * it, else PTYK_EINVAL.  No hipMalloc/hipFree or allocating call is made.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams or threads) race.  Positions are
+ * not validated (device memory); sample indices are clamped to the canvas for any value. */
+#define PTYK_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ptyk_simulate_noalloc(ptyk_ctx* ctx, const float* obj, int32_t rows, int32_t cols, const float* probe,
+                          const float* pos, int64_t n, int32_t squared, float* out);
+
 int ptyk_synchronize(ptyk_ctx* ctx);
 
 #ifdef __cplusplus

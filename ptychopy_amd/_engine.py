@@ -34,7 +34,7 @@ EXPORTS = (
     "ptyk_get_object", "ptyk_clear_object", "ptyk_set_probe", "ptyk_get_probe", "ptyk_set_psi", "ptyk_get_psi",
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_orthogonalize_modes", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
-    "ptyk_simulate", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
+    "ptyk_simulate", "ptyk_simulate_noalloc", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
     "ptyk_mls_iteration", "ptyk_mls_iteration_timed", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
     "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
@@ -130,6 +130,7 @@ def load():
     lib.ptyk_dm_iteration_timed.argtypes = [P, u32, P]
     lib.ptyk_fft2.argtypes = [P, P, i64, C.c_int]
     lib.ptyk_simulate.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
+    lib.ptyk_simulate_noalloc.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
     lib.ptyk_set_tuning.argtypes = [P, i32, i32]
     lib.ptyk_get_tuning.argtypes = [P, i32, C.POINTER(C.c_int32)]
     lib.ptyk_mls_begin.argtypes = [P, C.POINTER(MlsConfig), P, P, P, P]

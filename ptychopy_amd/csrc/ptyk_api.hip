@@ -167,6 +167,7 @@ int ptyk_create(const ptyk_config* cfg, ptyk_ctx** out) {
   };
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "stream");
+  if ((e = noalloc_setup(cfg->device)) != hipSuccess) return bail(e, "scratch array address");
   if ((e = dalloc(&c->tw, PTYK_TWN)) != hipSuccess) return bail(e, "twiddles");
   {
     std::vector<float2> tw(PTYK_TWN);
@@ -1306,6 +1307,28 @@ int ptyk_fft2(ptyk_ctx* c, float* data, int64_t n, int inverse) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(d);
   HIPCHK(c, e);
+  return PTYK_OK;
+}
+
+int ptyk_simulate_noalloc(ptyk_ctx* c, const float* obj, int32_t rows, int32_t cols, const float* probe,
+                          const float* pos, int64_t n, int32_t squared, float* out) {
+  if (!c) return PTYK_EINVAL;
+  if (!obj || !probe || !pos || !out || rows <= 0 || cols <= 0 || n < 0) return fail(c, PTYK_EINVAL, "bad argument");
+  const int m = c->cfg.frame_size;
+  const size_t need = (size_t)n * c->mm * sizeof(float2);
+  if (need > noalloc_scratch_bytes())
+    return fail(c, PTYK_EINVAL,
+                std::to_string(n) + " exit waves of " + std::to_string(m) + "^2 need " + std::to_string(need) +
+                    " bytes of scratch; the fixed array holds " + std::to_string(noalloc_scratch_bytes()));
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return fail(c, PTYK_EDEVICE, "hipSetDevice failed");
+  float2* psi = static_cast<float2*>(noalloc_scratch(c->cfg.device));
+  if (!psi) return fail(c, PTYK_ESTATE, "scratch array not set up on this device");
+  if (n == 0) return PTYK_OK;
+  HIPCHK(c, simulate_exit_waves(reinterpret_cast<const float2*>(obj), rows, cols,
+                                reinterpret_cast<const float2*>(probe), reinterpret_cast<const float2*>(pos), psi, n,
+                                m, c->cfg.tex_quantize, c->stream));
+  HIPCHK(c, fft2_batch(psi, n, m, false, c->tw, c->stream));
+  HIPCHK(c, abs_frames(psi, out, n * c->mm, squared, c->stream));
   return PTYK_OK;
 }
 

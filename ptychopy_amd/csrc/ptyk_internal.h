@@ -219,6 +219,12 @@ hipError_t object_error(const float2* obj, const float2* truth, const float2* pr
                         int n, int rows, int cols, double frac, float* cov, float* pint, double* part, int part_cap,
                         double* out, hipStream_t s);
 
+// ---- allocation-free entry points (ptyk_noalloc.hip): the per-device address of the fixed
+// scratch array, looked up by noalloc_setup (with the device current) at handle creation.
+hipError_t noalloc_setup(int device);
+void* noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+
 // ---- misc kernels (ptyk_misc.hip)
 hipError_t quantize_frames(const float* src_dev, __half* dst, float* scales, long n, int m, hipStream_t s);
 hipError_t dequantize_frames(const __half* src, const float* scales, float* dst, long n, int m, hipStream_t s);
