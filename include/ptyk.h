@@ -283,6 +283,28 @@ int ptyk_fft2(ptyk_ctx* ctx, float* data, int64_t n, int inverse);
 int ptyk_simulate(ptyk_ctx* ctx, const float* obj, int32_t rows, int32_t cols, const float* probe,
                   const float* pos_rc, int64_t n, int32_t squared, float* out);
 
+/* Preprocessing of on-disk diffraction frames (DiffractionLoader::run, DiffractionLoader.cpp:55-131;
+ * DiffractionsKernels.cu:100-118; utilitiesKernels.cu:841-851, 1945-1963) on the GPU: count raw
+ * frames of raw_rows x raw_cols float32 (host) become count frames of M x M float32 (host, M = the
+ * handle's frame size = symmetric_array_size): the S x S window centred at (center_row, center_col)
+ * (zeros beyond the raw frame; a raw frame that is already M x M is taken whole), rotated by 90
+ * degrees when rotate90 % 4 == 1 (the only rotation the reference implements), FFT-shifted when
+ * fft_shift, and, when threshold_sqrt, the d_preprocessDiffractions step (non-finite -> 0, values
+ * below threshold -> 0, else minus threshold, then the square root when square_root; the
+ * reference's load path skips this step, Diffractions.cpp:226-229). */
+typedef struct {
+  int32_t raw_rows, raw_cols;     /* rawFileSize (-nx, -ny) */
+  int32_t center_row, center_col; /* symmetric_array_center (-qx, -qy) */
+  int32_t rotate90;               /* rotate_90_times */
+  int32_t fft_shift;              /* FFT_SHIFT (-fftShiftData) */
+  int32_t threshold_sqrt;
+  int32_t square_root;            /* SQUARE_ROOT (-sqrtData) */
+  float threshold;                /* threshold_raw_data (-threshold) */
+  int32_t reserved[3];
+} ptyk_ingest_params;
+int ptyk_preprocess_frames(ptyk_ctx* ctx, const float* raw, int64_t count, const ptyk_ingest_params* params,
+                           float* out);
+
 /* Allocation-free simulator (no reference counterpart; for callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving or data-generation loops).  Same results as
  * ptyk_simulate, but obj (rows x cols complex64), probe (M x M complex64), pos (n x (row, col)

@@ -383,6 +383,40 @@ class MlsState:
                                 _ptr(self.ferr))
 
 
+def preprocess_frames(raw, size: int, center=None, rotate90: int = 0, fft_shift: bool = False,
+                      threshold_sqrt: bool = False, square_root: bool = False, threshold: float = 0.0):
+    """DiffractionLoader::run (DiffractionLoader.cpp:55-131) with numpy's own routines: the size x size
+    window of each raw frame centred at `center` (rows/cols outside the frame are zero; a frame that
+    is already size x size is kept whole), np.rot90 once when rotate90 % 4 == 1 (d_rot90,
+    utilitiesKernels.cu:841-851, 2031-2049), np.fft.fftshift when fft_shift (h_shiftFFT), then the
+    optional d_preprocessDiffractions step in float32 (DiffractionsKernels.cu:109-118)."""
+    r = np.asarray(raw, np.float32)
+    if r.ndim == 2:
+        r = r[None]
+    cnt, rows, cols = r.shape
+    cr, cc = center if center is not None else (rows // 2, cols // 2)
+    out = np.zeros((cnt, size, size), np.float32)
+    if rows == size and cols == size:
+        out[:] = r
+    else:
+        r0, c0 = cr - size // 2, cc - size // 2
+        a0, a1 = max(r0, 0), min(r0 + size, rows)
+        b0, b1 = max(c0, 0), min(c0 + size, cols)
+        if a1 > a0 and b1 > b0:
+            out[:, a0 - r0:a1 - r0, b0 - c0:b1 - c0] = r[:, a0:a1, b0:b1]
+    if rotate90 % 4 == 1:
+        out = np.rot90(out, k=1, axes=(1, 2))
+    if fft_shift:
+        out = np.fft.fftshift(out, axes=(1, 2))
+    out = np.ascontiguousarray(out)
+    if threshold_sqrt:
+        t = np.float32(threshold)
+        v = np.where(np.isfinite(out), out, np.float32(0))
+        v = np.where(v < t, np.float32(0), (v - t).astype(np.float32))
+        out = np.sqrt(v).astype(np.float32) if square_root else v.astype(np.float32)
+    return out
+
+
 def mls_zone_plate_probe(m: int, lambd: float, dx_s: float, beam_size: float) -> np.ndarray:
     """Mode 0 of Probe::initMLH without a guess (h_initProbeMLH / d_initProbeMLH / h_endProbeMLH,
     ProbeKernels.cu:220-366, 592-641), FP64 with numpy's FFT.  Grid x_t = -dx M/2 + t dx M/(M-1)

@@ -34,7 +34,7 @@ EXPORTS = (
     "ptyk_get_object", "ptyk_clear_object", "ptyk_set_probe", "ptyk_get_probe", "ptyk_set_psi", "ptyk_get_psi",
     "ptyk_get_scalars", "ptyk_set_scalars", "ptyk_refresh_maxima", "ptyk_init_probe", "ptyk_begin_modal",
     "ptyk_end_modal", "ptyk_orthogonalize_modes", "ptyk_epie_iteration", "ptyk_dm_iteration", "ptyk_dm_iteration_timed", "ptyk_fft2",
-    "ptyk_simulate", "ptyk_simulate_noalloc", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
+    "ptyk_simulate", "ptyk_simulate_noalloc", "ptyk_preprocess_frames", "ptyk_synchronize", "ptyk_set_tuning", "ptyk_get_tuning", "ptyk_mls_begin",
     "ptyk_mls_iteration", "ptyk_mls_iteration_timed", "ptyk_mls_get_state", "ptyk_mls_set_state", "ptyk_set_true_object", "ptyk_object_error",
     "ptyk_batch_create", "ptyk_batch_destroy", "ptyk_batch_last_error", "ptyk_batch_dm_iteration",
     "ptyk_batch_synchronize", "ptyk_batch_graph_count", "ptyk_net_create", "ptyk_net_destroy",
@@ -91,6 +91,13 @@ def mls_config(box, **kw) -> MlsConfig:
     return c
 
 
+class IngestParams(C.Structure):
+    _fields_ = [("raw_rows", C.c_int32), ("raw_cols", C.c_int32), ("center_row", C.c_int32),
+                ("center_col", C.c_int32), ("rotate90", C.c_int32), ("fft_shift", C.c_int32),
+                ("threshold_sqrt", C.c_int32), ("square_root", C.c_int32), ("threshold", C.c_float),
+                ("reserved", C.c_int32 * 3)]
+
+
 class MlsState(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("positions_start", C.c_void_p), ("roi", C.c_void_p),
                 ("subpixel", C.c_void_p), ("evolution", C.c_void_p), ("variable_mode", C.c_void_p),
@@ -131,6 +138,7 @@ def load():
     lib.ptyk_fft2.argtypes = [P, P, i64, C.c_int]
     lib.ptyk_simulate.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
     lib.ptyk_simulate_noalloc.argtypes = [P, P, i32, i32, P, P, i64, i32, P]
+    lib.ptyk_preprocess_frames.argtypes = [P, P, i64, C.POINTER(IngestParams), P]
     lib.ptyk_set_tuning.argtypes = [P, i32, i32]
     lib.ptyk_get_tuning.argtypes = [P, i32, C.POINTER(C.c_int32)]
     lib.ptyk_mls_begin.argtypes = [P, C.POINTER(MlsConfig), P, P, P, P]
@@ -358,6 +366,20 @@ class Engine:
         out = np.empty((ps.shape[0], self.m, self.m), np.float32)
         self._chk(self.lib.ptyk_simulate(self.h, _ptr(o), o.shape[0], o.shape[1], _ptr(p), _ptr(ps), ps.shape[0],
                                          int(squared), _ptr(out)))
+        return out
+
+    def preprocess_frames(self, raw, center=None, rotate90: int = 0, fft_shift: bool = False,
+                          threshold_sqrt: bool = False, square_root: bool = False, threshold: float = 0.0):
+        """Raw frames (count x rows x cols) -> preprocessed count x M x M frames (ptyk_preprocess_frames)."""
+        r = np.ascontiguousarray(raw, dtype=np.float32)
+        if r.ndim == 2:
+            r = r[None]
+        cnt, rows, cols = r.shape
+        cr, cc = center if center is not None else (rows // 2, cols // 2)
+        p = IngestParams(rows, cols, int(cr), int(cc), int(rotate90), int(bool(fft_shift)), int(bool(threshold_sqrt)),
+                         int(bool(square_root)), float(threshold))
+        out = np.empty((cnt, self.m, self.m), np.float32)
+        self._chk(self.lib.ptyk_preprocess_frames(self.h, _ptr(r), cnt, C.byref(p), _ptr(out)))
         return out
 
     # ---- MLs

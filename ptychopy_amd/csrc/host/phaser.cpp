@@ -407,6 +407,133 @@ real_t MLSMethod::iteration(Diffractions* diffs, Probe*, Sample*, IPtychoScanMes
   return 1;  // MLS::iteration returns 1 (MLS.cpp:489)
 }
 
+namespace {
+// Raw frame files (FileManager.cpp:130-214): ".bin" holds consecutive frames of raw_rows x
+// raw_cols float32 (BinFile: frame i at offset i x rows x cols), ".csv" comma-separated values
+// read in order (CSVFile); ".h5" needs HDF5, which this build does not link.
+struct FrameFile {
+  std::string path;
+  bool csv = false;
+  std::vector<float> values;  // csv: whole file
+  FILE* f = nullptr;
+  ~FrameFile() {
+    if (f) fclose(f);
+  }
+};
+
+bool open_frame_file(const std::string& path, FrameFile* ff, std::string* err) {
+  const size_t dot = path.find_last_of('.');
+  if (dot == std::string::npos || dot == 0) {
+    *err = "Error in -fp formatting parameter! No file extension found (" + path + ")";
+    return false;
+  }
+  std::string ext = path.substr(dot + 1);
+  std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
+  ff->path = path;
+  if (ext == "bin") {
+    ff->f = fopen(path.c_str(), "rb");
+    if (!ff->f) {
+      *err = "ERROR: Failed to open file (" + path + ")";
+      return false;
+    }
+    return true;
+  }
+  if (ext == "csv") {
+    std::ifstream in(path);
+    if (!in.is_open()) {
+      *err = "ERROR: Failed to open file (" + path + ")";
+      return false;
+    }
+    ff->csv = true;
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream iss(line);
+      std::string tok;
+      while (std::getline(iss, tok, ',')) {
+        const char* b = tok.c_str();
+        char* e = nullptr;
+        const float v = std::strtof(b, &e);
+        if (e != b) ff->values.push_back(v);
+      }
+    }
+    return true;
+  }
+  if (ext == "h5") {
+    *err = "HDF5 frame files (" + path + ") need an HDF5-enabled build; convert them to .bin or .csv";
+    return false;
+  }
+  *err = "unsupported frame file extension (" + path + "): use .bin or .csv";
+  return false;
+}
+
+bool read_frame(FrameFile* ff, unsigned section, int rows, int cols, float* out, std::string* err) {
+  const size_t cnt = (size_t)rows * cols;
+  if (ff->csv) {
+    if ((section + 1) * cnt > ff->values.size()) {
+      *err = "frame " + std::to_string(section) + " is beyond the end of " + ff->path;
+      return false;
+    }
+    std::copy(ff->values.begin() + section * cnt, ff->values.begin() + (section + 1) * cnt, out);
+    return true;
+  }
+  if (fseeko(ff->f, (off_t)(section * cnt * sizeof(float)), SEEK_SET) != 0 ||
+      fread(out, sizeof(float), cnt, ff->f) != cnt) {
+    *err = "frame " + std::to_string(section) + " is beyond the end of " + ff->path;
+    return false;
+  }
+  return true;
+}
+}  // namespace
+
+// Diffractions::load (Diffractions.cpp:188-232) and DiffractionLoader::run: frame q (scan order) is
+// section q % diffsPerFile of the file the pattern names for q / diffsPerFile + fileStartIndex
+// ('#' in the pattern starts a printf conversion, IPhaser.cpp:209); each raw frame is cropped,
+// rotated and shifted on the GPU (ptyk_preprocess_frames) in batches.
+bool IPhaser::read_frames(int n, int m, std::vector<float>* amp) {
+  std::string pat = prm.p.dataFilePattern;
+  const size_t hash = pat.find('#');
+  if (hash != std::string::npos) pat[hash] = '%';
+  const int rows = prm.p.raw_file_size_x, cols = prm.p.raw_file_size_y;
+  if (rows <= 0 || cols <= 0) {
+    err = "raw frame size (-nx, -ny) must be positive";
+    return false;
+  }
+  const unsigned dpf = std::max(1u, prm.p.diffsPerFile);
+  ptyk_ingest_params ip{};
+  ip.raw_rows = rows;
+  ip.raw_cols = cols;
+  ip.center_row = prm.p.symmetric_array_center_x;
+  ip.center_col = prm.p.symmetric_array_center_y;
+  ip.rotate90 = (int32_t)prm.p.rotate_90_times;
+  ip.fft_shift = (prm.p.flags & 4) ? 1 : 0;
+  ip.threshold_sqrt = 0;  // disabled in the reference's load path (Diffractions.cpp:226-229)
+  ip.square_root = (prm.p.flags & 1) ? 1 : 0;
+  ip.threshold = prm.p.threshold_raw_data;
+  const size_t mm = (size_t)m * m, raw_sz = (size_t)rows * cols;
+  amp->assign((size_t)n * mm, 0.f);
+  const int batch = 128;
+  std::vector<float> raw((size_t)batch * raw_sz);
+  std::unique_ptr<FrameFile> ff;
+  for (int q0 = 0; q0 < n; q0 += batch) {
+    const int cnt = std::min(batch, n - q0);
+    for (int b = 0; b < cnt; ++b) {
+      const unsigned idx = mesh->indices.empty() ? (unsigned)(q0 + b) : mesh->indices[q0 + b];
+      char path[4096];
+      snprintf(path, sizeof(path), pat.c_str(), (int)(idx / dpf + prm.p.fileStartIndex));
+      if (!ff || ff->path != path) {
+        ff.reset(new FrameFile());
+        if (!open_frame_file(path, ff.get(), &err)) return false;
+      }
+      if (!read_frame(ff.get(), idx % dpf, rows, cols, raw.data() + (size_t)b * raw_sz, &err)) return false;
+    }
+    if (ptyk_preprocess_frames(eng.ctx, raw.data(), cnt, &ip, amp->data() + (size_t)q0 * mm) != PTYK_OK) {
+      err = ptyk_last_error(eng.ctx);
+      return false;
+    }
+  }
+  return true;
+}
+
 bool IPhaser::simulate_frames(std::vector<float>* amp) {
   // Diffractions::simulate (Diffractions.cpp:80-116) over a seeded random-phase object of the
   // canvas size with the simulated probe (the reference reads lena/baboon PGM files here)
@@ -479,10 +606,8 @@ bool IPhaser::init() {
   mesh->ml = mls;  // generateMeshML + get_nonoverlapping_indices + precalculate_ROI for MLs
   if (!mesh->generate(prm, &err)) return false;
   const int n = (int)mesh->positions.size(), m = (int)prm.r.desiredShape;
-  if (!prm.r.simulated && diffraction.empty()) {
-    err = prm.p.dataFilePattern.empty()
-              ? "Could not find data path parameter! Did you forget to provide `-fp` argument?"
-              : "file input (" + prm.p.dataFilePattern + ") is not supported by this build; pass diffractionNP";
+  if (!prm.r.simulated && diffraction.empty() && prm.p.dataFilePattern.empty()) {
+    err = "Could not find data path parameter! Did you forget to provide `-fp` argument?";
     return false;
   }
   if (!diffraction.empty() && (diff_n != n || diff_rows != m || diff_cols != m)) {
@@ -525,6 +650,8 @@ bool IPhaser::init() {
   std::vector<float> amp;
   if (!diffraction.empty()) {
     amp = diffraction;
+  } else if (!prm.r.simulated) {
+    if (!read_frames(n, m, &amp)) return false;
   } else if (!simulate_frames(&amp)) {
     err = ptyk_last_error(eng.ctx);
     return false;

@@ -270,6 +270,7 @@ class IPhaser {
   bool phaseStep(IPhasingMethod* m, unsigned i, real_t* e);
   void postPhase();
   bool simulate_frames(std::vector<float>* amp);
+  bool read_frames(int n, int m, std::vector<float>* amp);  // -fp= frame files
   bool mls_prephase();
   bool mls_probe_modes(std::vector<float>* modes);
   std::vector<double> frame_power_;  // MLs: sum of each frame's intensities

@@ -1310,6 +1310,36 @@ int ptyk_fft2(ptyk_ctx* c, float* data, int64_t n, int inverse) {
   return PTYK_OK;
 }
 
+int ptyk_preprocess_frames(ptyk_ctx* c, const float* raw, int64_t count, const ptyk_ingest_params* p, float* out) {
+  if (!c) return PTYK_EINVAL;
+  if (!raw || !p || !out || count < 0 || p->raw_rows <= 0 || p->raw_cols <= 0) return fail(c, PTYK_EINVAL, "bad argument");
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return fail(c, PTYK_EDEVICE, "hipSetDevice failed");
+  if (count == 0) return PTYK_OK;
+  IngestArgs a{};
+  a.raw_rows = p->raw_rows;
+  a.raw_cols = p->raw_cols;
+  a.size = c->cfg.frame_size;
+  a.center_row = p->center_row;
+  a.center_col = p->center_col;
+  a.rotate = (p->rotate90 % 4 == 1) ? 1 : 0;
+  a.fft_shift = p->fft_shift ? 1 : 0;
+  a.threshold_sqrt = p->threshold_sqrt ? 1 : 0;
+  a.square_root = p->square_root ? 1 : 0;
+  a.threshold = p->threshold;
+  const size_t nraw = (size_t)count * p->raw_rows * p->raw_cols, nout = (size_t)count * c->mm;
+  float *d_raw = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc((void**)&d_raw, nraw * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_out, nout * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpyAsync(d_raw, raw, nraw * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = ingest_frames(d_raw, d_out, count, a, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, nout * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_raw) hipFree(d_raw);
+  if (d_out) hipFree(d_out);
+  HIPCHK(c, e);
+  return PTYK_OK;
+}
+
 int ptyk_simulate_noalloc(ptyk_ctx* c, const float* obj, int32_t rows, int32_t cols, const float* probe,
                           const float* pos, int64_t n, int32_t squared, float* out) {
   if (!c) return PTYK_EINVAL;

@@ -219,6 +219,15 @@ hipError_t object_error(const float2* obj, const float2* truth, const float2* pr
                         int n, int rows, int cols, double frac, float* cov, float* pint, double* part, int part_cap,
                         double* out, hipStream_t s);
 
+// ---- frame ingest (ptyk_ingest.hip): raw count x raw_rows x raw_cols frames (device) -> count x
+// size x size preprocessed frames (device).
+struct IngestArgs {
+  int raw_rows, raw_cols, size, center_row, center_col;
+  int rotate, fft_shift, threshold_sqrt, square_root;
+  float threshold;
+};
+hipError_t ingest_frames(const float* raw, float* out, long count, const IngestArgs& a, hipStream_t s);
+
 // ---- allocation-free entry points (ptyk_noalloc.hip): the per-device address of the fixed
 // scratch array, looked up by noalloc_setup (with the device current) at handle creation.
 hipError_t noalloc_setup(int device);
