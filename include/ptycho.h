@@ -61,6 +61,14 @@ int ptycho_mls_partitions(int64_t n, int32_t niter, uint64_t seed, int32_t* fram
 int ptycho_mls_probe_modes(void* engine, int32_t m, int32_t k, double lambda, double dx_s, double beam_size,
                            const float* draws, float* out);
 
+/* Compact MLs frame groups (IPtychoScanMesh::get_close_indices, IPtychoScanMesh.cpp:422-824, the
+ * "MLC" grouping): k-means clusters of the positions (n x 2) into ceil(n / max(1, n/100)) groups,
+ * balanced to at most two sizes and refined by size-preserving swaps toward the nearest median
+ * centre; groups by decreasing size, frames ascending.  frames_out: n, offsets_out: groups + 1;
+ * pass NULL to get the count. */
+int ptycho_mls_compact_groups(const float* pos, int64_t n, uint64_t seed, int32_t* frames_out, int32_t* offsets_out,
+                              int32_t* n_groups);
+
 /* Complex CSV ("a+bj", ", "-separated rows) or raw binary, as Cuda2DArray::save / load
  * (Cuda2DArray.cpp:46-108, Cuda2DArray.hpp:439-464). */
 int ptycho_save_complex_csv(const char* path, const float* c64, int32_t rows, int32_t cols, int32_t binary);

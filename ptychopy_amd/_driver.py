@@ -26,6 +26,7 @@ DRIVER_EXPORTS = (
     "ptycho_phaser_num_positions", "ptycho_phaser_get_positions", "ptycho_phaser_get_errors",
     "ptycho_phaser_phase_seconds", "ptycho_phaser_engine", "ptycho_mesh_cartesian_raw", "ptycho_mls_canvas",
     "ptycho_mls_partitions", "ptycho_phaser_get_fourier_errors", "ptycho_mls_probe_modes",
+    "ptycho_mls_compact_groups",
 )
 
 _bound = False
@@ -48,10 +49,28 @@ def lib():
         L.ptycho_mls_canvas.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.ptycho_mls_partitions.argtypes = [C.c_int64, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p,
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ptycho_mls_compact_groups.argtypes = [C.c_void_p, C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                C.POINTER(C.c_int32)]
         L.ptycho_mls_probe_modes.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                              C.c_void_p, C.c_void_p]
         _bound = True
     return L
+
+
+def mls_compact_groups(pos, seed: int = 0):
+    """Compact (k-means) MLs frame groups (IPtychoScanMesh::get_close_indices): list of frame arrays."""
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    n = p.shape[0]
+    g = C.c_int32()
+    L = lib()
+    if L.ptycho_mls_compact_groups(p.ctypes.data_as(C.c_void_p), n, seed, None, None, C.byref(g)) != 0:
+        raise ValueError("bad positions")
+    fr = np.zeros(n, np.int32)
+    of = np.zeros(g.value + 1, np.int32)
+    if L.ptycho_mls_compact_groups(p.ctypes.data_as(C.c_void_p), n, seed, fr.ctypes.data_as(C.c_void_p),
+                                   of.ctypes.data_as(C.c_void_p), C.byref(g)) != 0:
+        raise ValueError("bad positions")
+    return [fr[of[i]:of[i + 1]] for i in range(g.value)]
 
 
 def mls_probe_modes(engine, k: int, lambd: float, dx_s: float, beam_size: float, draws, mode0=None) -> np.ndarray:

@@ -116,6 +116,7 @@ bool Parameters::parseCommandLine(const std::vector<std::string>& args, std::str
     else if (key == "wsServerUrl") r.wsServerUrl = val;
     else if (key == "delta_p") r.delta_p = atof(val.c_str());
     else if (key == "seed") r.seed = strtoull(val.c_str(), nullptr, 10);
+    else if (key == "MLC") r.compactGroups = iv > 0;
     else {
       if (err) *err = "unknown option -" + key;
       return false;
@@ -175,6 +176,7 @@ bool Parameters::setKeyword(const std::string& key, const std::string& val, std:
   else if (key == "algorithm") { if (!val.empty()) r.algorithm = val; }
   else if (key == "RMS") { r.calculateRMS = iv > 0; }
   else if (key == "seed") { r.seed = strtoull(val.c_str(), nullptr, 10); }
+  else if (key == "MLC") { r.compactGroups = iv > 0; }
   else {
     if (err) *err = "unknown keyword " + key;
     return false;
@@ -371,6 +373,20 @@ real_t DMMethod::iteration(Diffractions* diffs, Probe*, Sample*, IPtychoScanMesh
 
 void MLSMethod::initMem(IPtychoScanMesh* scanMesh, uint2) {
   n_ = (int64_t)scanMesh->positions.size();
+  if (compact_) {
+    // get_close_indices: one compact partition, its group order shuffled every iteration
+    std::vector<float> pos(2 * (size_t)n_);
+    for (int64_t q = 0; q < n_; ++q) {
+      pos[2 * q] = scanMesh->positions[q].x;
+      pos[2 * q + 1] = scanMesh->positions[q].y;
+    }
+    ptycho_mls_compact_groups(pos.data(), n_, seed_ + 101, nullptr, nullptr, &groups_);
+    frames_.assign((size_t)n_, 0);
+    offsets_.assign((size_t)groups_ + 1, 0);
+    ptycho_mls_compact_groups(pos.data(), n_, seed_ + 101, frames_.data(), offsets_.data(), &groups_);
+    parts_ = 1;
+    return;
+  }
   ptycho_mls_partitions(n_, niter_, seed_ + 101, nullptr, nullptr, &groups_, &parts_);
   frames_.assign((size_t)parts_ * n_, 0);
   offsets_.assign((size_t)parts_ * (groups_ + 1), 0);
@@ -693,7 +709,7 @@ bool IPhaser::addPhasingMethod(const std::string& name, unsigned iterations) {
     methods.push_back(CXPhasing{name, std::unique_ptr<IPhasingMethod>(new DMMethod()), iterations});
   else if (name == "MLs")
     methods.push_back(
-        CXPhasing{name, std::unique_ptr<IPhasingMethod>(new MLSMethod(prm.r.seed, prm.r.Niter)), iterations});
+        CXPhasing{name, std::unique_ptr<IPhasingMethod>(new MLSMethod(prm.r.seed, prm.r.Niter, prm.r.compactGroups)), iterations});
   else {
     err = "Reconstruction algorithm [" + name + "] not implemented";
     return false;

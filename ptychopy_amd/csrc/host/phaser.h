@@ -63,6 +63,7 @@ struct ReconstructionParams {
   bool calculateRMS = false, binaryOutput = false;
   std::string positionsFilename, probeGuess, objectGuess, wsServerUrl;
   uint64_t seed = 0;  // replaces the reference's time-seeded srand / curand streams
+  bool compactGroups = false;  // MLs groups from get_close_indices ("MLC") instead of the random partitions
 };
 
 class Parameters {
@@ -213,7 +214,8 @@ class DMMethod : public IPhasingMethod {
 // the whole iteration (ptyk_mls_iteration).
 class MLSMethod : public IPhasingMethod {
  public:
-  MLSMethod(uint64_t seed, int niter) : rng_(seed + 202), seed_(seed), niter_(niter) {}
+  MLSMethod(uint64_t seed, int niter, bool compact = false)
+      : rng_(seed + 202), seed_(seed), niter_(niter), compact_(compact) {}
   void initMem(IPtychoScanMesh* scanMesh, uint2 probeSize) override;
   real_t iteration(Diffractions*, Probe*, Sample*, IPtychoScanMesh*, std::vector<std::vector<real_t> >& fourierErrors,
                    const std::vector<float2>& scanPositions, bool phaseConstraint, bool updateProbe,
@@ -225,6 +227,7 @@ class MLSMethod : public IPhasingMethod {
   std::mt19937_64 rng_;
   uint64_t seed_;
   int niter_;
+  bool compact_;
   std::vector<int32_t> frames_, offsets_;
   int32_t groups_ = 0, parts_ = 0;
   int64_t n_ = 0;

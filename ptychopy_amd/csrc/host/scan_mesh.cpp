@@ -182,3 +182,224 @@ extern "C" int ptycho_mls_partitions(int64_t n, int32_t niter, uint64_t seed, in
   }
   return 0;
 }
+
+// IPtychoScanMesh::get_close_indices (IPtychoScanMesh.cpp:422-824; the "MLC" alternative to
+// get_nonoverlapping_indices that IPhaser.cpp:159 leaves commented out): spatially compact groups
+// of about floor(N/100) frames.  The reference's body reads its starting groups and distances from
+// fixed files on its authors' disk; this follows the steps it takes around them:
+//  1. k-means of the positions into ceil(N/grouping) clusters: k-means++ seeding (utilities.h:225-251)
+//     and Lloyd iterations until at most N/1024 points change (utilities.h:253-285), ten seeded
+//     restarts, keeping the one whose cluster sizes have the smallest standard deviation
+//     (scanPositionSD, IPtychoScanMesh.cpp:320-333);
+//  2. balancing: while a cluster holds more than `grouping` frames, the frame of an over-full cluster
+//     closest to the centre of the smallest cluster moves there (IPtychoScanMesh.cpp:564-628), until
+//     at most two sizes remain;
+//  3. centres become the per-cluster medians of x and y (CalcMHWScore, IPtychoScanMesh.cpp:399-420)
+//     and, for up to ten passes while the share of frames outside their nearest cluster falls, the
+//     frame farthest from its centre swaps clusters with the frame of its nearest cluster closest to
+//     the first one's centre (sizes are kept; IPtychoScanMesh.cpp:656-754);
+//  4. groups in order of decreasing size (ties by cluster index), frames ascending
+//     (IPtychoScanMesh.cpp:756-822).
+// pos: n x 2 (any unit).  Null outputs: only the count.
+extern "C" int ptycho_mls_compact_groups(const float* pos, int64_t n, uint64_t seed, int32_t* frames_out,
+                                         int32_t* offsets_out, int32_t* n_groups) {
+  if (!pos || n <= 0 || !n_groups) return -1;
+  const int64_t grouping = std::max<int64_t>(1, n / 100);
+  const int K = (int)((n + grouping - 1) / grouping);
+  *n_groups = K;
+  if (!frames_out || !offsets_out) return 0;
+  auto d2 = [&](int64_t i, double cx, double cy) {
+    const double dx = pos[2 * i] - cx, dy = pos[2 * i + 1] - cy;
+    return dx * dx + dy * dy;
+  };
+  std::mt19937_64 g(seed);
+  std::vector<int> best_group;
+  std::vector<double> best_cx, best_cy;
+  double best_sd = HUGE_VAL;
+  std::vector<int> grp(n);
+  std::vector<double> cx(K), cy(K), dmin(n);
+  for (int rep = 0; rep < 10; ++rep) {
+    // k-means++ seeding
+    int64_t first = (int64_t)(g() % (uint64_t)n);
+    cx[0] = pos[2 * first];
+    cy[0] = pos[2 * first + 1];
+    for (int64_t i = 0; i < n; ++i) dmin[i] = d2(i, cx[0], cy[0]);
+    for (int k = 1; k < K; ++k) {
+      double sum = 0.0;
+      for (int64_t i = 0; i < n; ++i) sum += dmin[i];
+      double r = (double)(g() >> 11) * (1.0 / 9007199254740992.0) * sum;
+      int64_t pick = n - 1;
+      for (int64_t i = 0; i < n; ++i) {
+        r -= dmin[i];
+        if (r <= 0) {
+          pick = i;
+          break;
+        }
+      }
+      cx[k] = pos[2 * pick];
+      cy[k] = pos[2 * pick + 1];
+      for (int64_t i = 0; i < n; ++i) dmin[i] = std::min(dmin[i], d2(i, cx[k], cy[k]));
+    }
+    auto nearest = [&](int64_t i) {
+      int bi = 0;
+      double bd = HUGE_VAL;
+      for (int k = 0; k < K; ++k) {
+        const double d = d2(i, cx[k], cy[k]);
+        if (d < bd) {
+          bd = d;
+          bi = k;
+        }
+      }
+      return bi;
+    };
+    for (int64_t i = 0; i < n; ++i) grp[i] = nearest(i);
+    // Lloyd iterations
+    for (int it = 0; it < 1000; ++it) {
+      std::vector<double> sx(K, 0.0), sy(K, 0.0);
+      std::vector<int64_t> cnt(K, 0);
+      for (int64_t i = 0; i < n; ++i) {
+        sx[grp[i]] += pos[2 * i];
+        sy[grp[i]] += pos[2 * i + 1];
+        cnt[grp[i]]++;
+      }
+      for (int k = 0; k < K; ++k)
+        if (cnt[k]) {
+          cx[k] = sx[k] / cnt[k];
+          cy[k] = sy[k] / cnt[k];
+        }
+      int64_t changed = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        const int b = nearest(i);
+        if (b != grp[i]) {
+          grp[i] = b;
+          ++changed;
+        }
+      }
+      if (changed <= (n >> 10)) break;
+    }
+    std::vector<int64_t> cnt(K, 0);
+    for (int64_t i = 0; i < n; ++i) cnt[grp[i]]++;
+    double mean = (double)n / K, var = 0.0;
+    for (int k = 0; k < K; ++k) var += ((double)cnt[k] - mean) * ((double)cnt[k] - mean);
+    const double sd = std::sqrt(var / K);
+    if (sd < best_sd) {
+      best_sd = sd;
+      best_group = grp;
+      best_cx = cx;
+      best_cy = cy;
+    }
+  }
+  grp = best_group;
+  cx = best_cx;
+  cy = best_cy;
+  std::vector<int64_t> cnt(K, 0);
+  for (int64_t i = 0; i < n; ++i) cnt[grp[i]]++;
+  // balancing
+  for (int64_t it = 1; it <= 4 * n + 1000; ++it) {
+    std::vector<int64_t> sizes(cnt);
+    std::sort(sizes.begin(), sizes.end());
+    const int64_t distinct = std::unique(sizes.begin(), sizes.end()) - sizes.begin();
+    if (distinct <= 2 && ((int64_t)K * grouping != n || it > 1000)) break;
+    if (distinct == 1) break;
+    const int smallest = (int)(std::min_element(cnt.begin(), cnt.end()) - cnt.begin());
+    int64_t pick = -1;
+    double pd = HUGE_VAL;
+    for (int64_t i = 0; i < n; ++i)
+      if (cnt[grp[i]] > grouping) {
+        const double d = d2(i, cx[smallest], cy[smallest]);
+        if (d < pd) {
+          pd = d;
+          pick = i;
+        }
+      }
+    if (pick < 0) break;
+    cnt[grp[pick]]--;
+    grp[pick] = smallest;
+    cnt[smallest]++;
+  }
+  // median centres, then size-preserving swaps toward the nearest centre
+  for (int k = 0; k < K; ++k) {
+    std::vector<double> xs, ys;
+    for (int64_t i = 0; i < n; ++i)
+      if (grp[i] == k) {
+        xs.push_back(pos[2 * i]);
+        ys.push_back(pos[2 * i + 1]);
+      }
+    if (xs.empty()) continue;
+    std::sort(xs.begin(), xs.end());
+    std::sort(ys.begin(), ys.end());
+    const size_t s = xs.size();
+    cx[k] = s % 2 ? xs[s / 2] : 0.5 * (xs[s / 2 - 1] + xs[s / 2]);
+    cy[k] = s % 2 ? ys[s / 2] : 0.5 * (ys[s / 2 - 1] + ys[s / 2]);
+  }
+  std::vector<int> opt(n);
+  for (int64_t i = 0; i < n; ++i) {
+    int bi = 0;
+    double bd = HUGE_VAL;
+    for (int k = 0; k < K; ++k) {
+      const double d = d2(i, cx[k], cy[k]);
+      if (d < bd) {
+        bd = d;
+        bi = k;
+      }
+    }
+    opt[i] = bi;
+  }
+  double ratio0 = 1.0;
+  for (int pass = 0; pass < 10; ++pass) {
+    std::vector<int64_t> sw;
+    for (int64_t i = 0; i < n; ++i)
+      if (grp[i] != opt[i]) sw.push_back(i);
+    const double ratio = (double)sw.size() / n;
+    if (ratio >= ratio0) break;
+    ratio0 = ratio;
+    std::vector<char> done(sw.size(), 0);
+    for (size_t step = 0; step < sw.size(); ++step) {
+      int64_t worst = -1;
+      size_t wi = 0;
+      double wd = -1.0;
+      for (size_t s = 0; s < sw.size(); ++s)
+        if (!done[s]) {
+          const double d = d2(sw[s], cx[grp[sw[s]]], cy[grp[sw[s]]]);
+          if (d > wd) {
+            wd = d;
+            worst = sw[s];
+            wi = s;
+          }
+        }
+      if (worst < 0) break;
+      done[wi] = 1;
+      const int g_old = grp[worst], g_new = opt[worst];
+      if (g_old == g_new) continue;
+      int64_t partner = -1;
+      double bd = HUGE_VAL;
+      for (int64_t i = 0; i < n; ++i)
+        if (grp[i] == g_new) {
+          const double d = d2(i, cx[g_old], cy[g_old]);
+          if (d < bd) {
+            bd = d;
+            partner = i;
+          }
+        }
+      if (partner < 0) continue;
+      grp[worst] = g_new;
+      grp[partner] = g_old;
+      for (size_t s = 0; s < sw.size(); ++s)
+        if (sw[s] == partner) done[s] = 1;
+    }
+  }
+  // output: groups by decreasing size (ties by index), frames ascending
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int64_t i = 0; i < n; ++i) cnt[grp[i]]++;
+  std::vector<int> order(K);
+  for (int k = 0; k < K; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+  int64_t off = 0;
+  offsets_out[0] = 0;
+  for (int o = 0; o < K; ++o) {
+    for (int64_t i = 0; i < n; ++i)
+      if (grp[i] == order[o]) frames_out[off++] = (int32_t)i;
+    offsets_out[o + 1] = (int32_t)off;
+  }
+  return 0;
+}

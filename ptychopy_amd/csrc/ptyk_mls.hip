@@ -729,15 +729,30 @@ __global__ __launch_bounds__(1024) void k_lsq_solve(const double* __restrict__ p
 
 // x += y * beta, beta from gscal (object: O += precond * mean step; probe: P_ll += pum * mean step);
 // the second update (x2, y2, count2, beta2), when count2 > 0, runs in the same launch after the
-// first's range.
+// first's range.  Each thread takes two consecutive elements of one range with 16-byte loads and
+// stores (both bases are 16-byte aligned: device allocations and whole probe modes).
+__device__ __forceinline__ void axpy_pair(float2* __restrict__ x, const float2* __restrict__ y, long e, long count,
+                                          float2 b) {
+  if (e + 1 < count) {
+    const float4 xv = *reinterpret_cast<const float4*>(x + e);
+    const float4 yv = *reinterpret_cast<const float4*>(y + e);
+    const float2 r0 = cadd(make_float2(xv.x, xv.y), cmul(make_float2(yv.x, yv.y), b));
+    const float2 r1 = cadd(make_float2(xv.z, xv.w), cmul(make_float2(yv.z, yv.w), b));
+    *reinterpret_cast<float4*>(x + e) = make_float4(r0.x, r0.y, r1.x, r1.y);
+  } else if (e < count) {
+    x[e] = cadd(x[e], cmul(y[e], b));
+  }
+}
+
 __global__ void k_axpy_beta(float2* __restrict__ x, const float2* __restrict__ y, long count,
                             const double* __restrict__ beta, float2* __restrict__ x2, const float2* __restrict__ y2,
                             long count2, const double* __restrict__ beta2) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count)
-    x[i] = cadd(x[i], cmul(y[i], c_of(beta[0], beta[1])));
-  else if (i - count < count2)
-    x2[i - count] = cadd(x2[i - count], cmul(y2[i - count], c_of(beta2[0], beta2[1])));
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long h1 = (count + 1) / 2;
+  if (t < h1)
+    axpy_pair(x, y, 2 * t, count, c_of(beta[0], beta[1]));
+  else
+    axpy_pair(x2, y2, 2 * (t - h1), count2, c_of(beta2[0], beta2[1]));
 }
 
 // Gradient position solver spectra (Probe.cpp:956-1118): t = FFT(obj_proj) * 2 pi i;
@@ -1548,7 +1563,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       {  // object and probe steps in one launch (disjoint arrays, both after the step solve)
         const long no = obj_rec ? canvas : 0, np = probe_rec && ll <= K ? (long)mm : 0;
         if (no + np > 0)
-          LAUNCH(k_axpy_beta, g1(no + np), dim3(NT), d.obj, d.precond, no, d.gscal + MLS_S_BO,
+          LAUNCH(k_axpy_beta, g1((no + 1) / 2 + (np + 1) / 2), dim3(NT), d.obj, d.precond, no, d.gscal + MLS_S_BO,
                  d.probe + (long)(ll - 1) * mm, d.pum, np, d.gscal + MLS_S_BP);
       }
       CHK();

@@ -39,7 +39,7 @@ KWARGS: tuple[tuple[str, type, Any], ...] = (
     ("lf", str, ""), ("PPS", int, 20),
 )
 NP_ARRAYS = ("diffractionNP", "positionNP", "objectNP", "probeNP")
-EXTRA = (("seed", int, 0), ("RMS", int, 0))
+EXTRA = (("seed", int, 0), ("RMS", int, 0), ("MLC", int, 0))
 
 _bound = False
 _phaser = None  # step-mode state (ptychopy.c:113)

@@ -222,3 +222,53 @@ def test_mls_partitions():
     assert all(np.array_equal(a, b) for a, b in zip(P[5], Q[5]))
     small = D.mls_partitions(50, 5, seed=1)  # fewer than 100 frames: groups of one
     assert len(small) == 5 and all(len(g) == 1 for g in small[0]) and len(small[0]) == 50
+
+
+# ---------------------------------------------------------------- compact (MLC) groups
+def _raster(side, step=14.0, jitter=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    r, c = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    pos = np.stack([r.ravel(), c.ravel()], axis=1).astype(np.float64) * step
+    return (pos + rng.uniform(-jitter, jitter, pos.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("side", [20, 51])
+def test_compact_groups_partition_and_sizes(side):
+    """Every frame once; ceil(N / max(1, N/100)) groups by decreasing size, at most two sizes."""
+    from ptychopy_amd import _driver as D
+    pos = _raster(side, jitter=1.0)
+    n = len(pos)
+    groups = D.mls_compact_groups(pos, seed=3)
+    grouping = max(1, n // 100)
+    assert len(groups) == -(-n // grouping)
+    allf = np.concatenate(groups)
+    assert np.array_equal(np.sort(allf), np.arange(n))
+    sizes = [len(g) for g in groups]
+    assert sizes == sorted(sizes, reverse=True) and len(set(sizes)) <= 2
+    assert all(np.array_equal(g, np.sort(g)) for g in groups)
+
+
+@pytest.mark.parametrize("side,typical_steps", [(20, 1.5), (51, 4.0)])
+def test_compact_groups_are_spatially_compact_and_seeded(side, typical_steps):
+    """k-means groups of a raster: the typical (median) group extends a few steps from its centroid
+    (groups of 4 on 20 x 20, of 26 on 51 x 51), the mean spread is well below that of the random
+    partitions (balancing moves a few frames into far groups), and the result depends only on
+    the seed."""
+    from ptychopy_amd import _driver as D
+    step = 14.0
+    pos = _raster(side, step=step, jitter=1.0)
+    groups = D.mls_compact_groups(pos, seed=5)
+    spread = np.array([np.abs(pos[g] - pos[g].mean(axis=0)).max() for g in groups])
+    parts = D.mls_partitions(len(pos), 1, 5)[0]
+    rand_spread = np.mean([np.abs(pos[g] - pos[g].mean(axis=0)).max() for g in parts])
+    assert np.median(spread) <= typical_steps * step
+    assert spread.mean() < 0.5 * rand_spread
+    again = D.mls_compact_groups(pos, seed=5)
+    assert all(np.array_equal(a, b) for a, b in zip(groups, again))
+
+
+def test_compact_groups_small_scan_is_one_frame_per_group():
+    from ptychopy_amd import _driver as D
+    pos = _raster(7)
+    groups = D.mls_compact_groups(pos, seed=0)
+    assert len(groups) == 49 and all(len(g) == 1 for g in groups)

@@ -167,3 +167,20 @@ def test_mlsnp_with_all_probe_modes_given(gpu, tmp_path, monkeypatch):
     assert ph.get_object().shape == (rows, cols)
     assert ph.get_fourier_errors(8).mean() < 0.5 * ph.get_fourier_errors(1).mean()
     ph.close()
+
+
+def test_mls_compact_groups_run_lowers_the_fourier_error(gpu, tmp_path, monkeypatch):
+    """MLs over the compact (k-means) groups of get_close_indices (-MLC=1): the solver runs on the
+    one compact partition and reduces the Fourier error as with the random partitions."""
+    from ptychopy_amd import ptychopy
+    monkeypatch.chdir(tmp_path)
+    ph = ptychopy.Phaser()
+    ph.parse_cmdline(MLS_CMD + " -i=9 -MLC=1")
+    ph.set_param("algorithm", "MLs")
+    ph.init()
+    ph.add_method("MLs", 9)
+    ph.phase()
+    e1, e9 = ph.get_fourier_errors(1), ph.get_fourier_errors(9)
+    assert np.isfinite(ph.get_object()).all() and np.isfinite(e9).all()
+    assert e9.mean() < 0.7 * e1.mean()
+    ph.close()
