@@ -60,6 +60,7 @@ int ptycho_phaser_set_object_guess(ptycho_phaser* h, const float* c64, int32_t r
   h->ph.object_guess.assign(c64, c64 + 2 * (size_t)rows * cols);
   h->ph.obj_guess_rows = rows;
   h->ph.obj_guess_cols = cols;
+  h->ph.obj_guess_centred = false;
   return 0;
 }
 

@@ -604,7 +604,7 @@ bool IPhaser::init() {
     auto* lm = new ListScanMesh();
     const size_t n = (size_t)prm.e.scanDims_x * prm.e.scanDims_y;
     for (size_t i = 0; i < n; ++i) {
-      double x, y;
+      float x, y;  // parsed straight into float, as ListScanMesh::loadList does (ListScanMesh.cpp:64-89)
       char comma;
       if (!(in >> x >> comma >> y) || comma != ',') {
         err = "Invalid raster positions file format [" + prm.r.positionsFilename + "]";
@@ -717,6 +717,29 @@ bool IPhaser::addPhasingMethod(const std::string& name, unsigned iterations) {
   return true;
 }
 
+namespace {
+// The object guess into the canvas.  A guess read from a file (Sample::loadGuess ->
+// h_setObjectArray, Sample.cpp:164-182, SampleKernels.cu:311-326 and 626-640) is centred: its
+// element (i, j) lands at (i + rows/2 - r/2, j + cols/2 - c/2), halves in integer division.  A
+// guess given as an array (objectNP, Sample::loadarr) fills the canvas from its origin.  Elements
+// that would land outside the canvas are dropped (the reference's unsigned offsets write outside
+// it for a guess larger than the canvas).
+void place_object_guess(std::vector<float>* canvas, int rows, int cols, const std::vector<float>& g, int r, int c,
+                        bool centred) {
+  const int oi = centred ? rows / 2 - r / 2 : 0, oj = centred ? cols / 2 - c / 2 : 0;
+  for (int i = 0; i < r; ++i) {
+    const int ti = i + oi;
+    if (ti < 0 || ti >= rows) continue;
+    for (int j = 0; j < c; ++j) {
+      const int tj = j + oj;
+      if (tj < 0 || tj >= cols) continue;
+      (*canvas)[2 * ((size_t)ti * cols + tj)] = g[2 * ((size_t)i * c + j)];
+      (*canvas)[2 * ((size_t)ti * cols + tj) + 1] = g[2 * ((size_t)i * c + j) + 1];
+    }
+  }
+}
+}  // namespace
+
 bool IPhaser::prePhase() {
   if (!eng.ctx) {
     err = "init() must be called before phase()";
@@ -766,15 +789,11 @@ bool IPhaser::prePhase() {
     object_guess = g;
     obj_guess_rows = r;
     obj_guess_cols = c;
+    obj_guess_centred = true;
   }
   if (!object_guess.empty()) {
-    // Sample::loadarr / loadGuess: the guess fills the canvas from its origin
     std::vector<float> canvas((size_t)2 * eng.rows * eng.cols, 0.f);
-    for (int i = 0; i < std::min(eng.rows, obj_guess_rows); ++i)
-      for (int j = 0; j < std::min(eng.cols, obj_guess_cols); ++j) {
-        canvas[2 * ((size_t)i * eng.cols + j)] = object_guess[2 * ((size_t)i * obj_guess_cols + j)];
-        canvas[2 * ((size_t)i * eng.cols + j) + 1] = object_guess[2 * ((size_t)i * obj_guess_cols + j) + 1];
-      }
+    place_object_guess(&canvas, eng.rows, eng.cols, object_guess, obj_guess_rows, obj_guess_cols, obj_guess_centred);
     if (ptyk_set_object(eng.ctx, canvas.data()) != PTYK_OK) {
       err = ptyk_last_error(eng.ctx);
       return false;
@@ -1024,14 +1043,11 @@ bool IPhaser::mls_prephase() {
     object_guess = g;
     obj_guess_rows = r;
     obj_guess_cols = c;
+    obj_guess_centred = true;
   }
   std::vector<float> canvas((size_t)2 * eng.rows * eng.cols, 0.f);
   if (!object_guess.empty()) {
-    for (int i = 0; i < std::min(eng.rows, obj_guess_rows); ++i)
-      for (int j = 0; j < std::min(eng.cols, obj_guess_cols); ++j) {
-        canvas[2 * ((size_t)i * eng.cols + j)] = object_guess[2 * ((size_t)i * obj_guess_cols + j)];
-        canvas[2 * ((size_t)i * eng.cols + j) + 1] = object_guess[2 * ((size_t)i * obj_guess_cols + j) + 1];
-      }
+    place_object_guess(&canvas, eng.rows, eng.cols, object_guess, obj_guess_rows, obj_guess_cols, obj_guess_centred);
   } else {
     // d_initRandObjectArray (SampleKernels.cu:328-339): |O| = u1, arg O = 2 pi u2 / 10.  The
     // simulated case starts here too (the reference starts it from O = 0, which its own

@@ -249,6 +249,7 @@ class IPhaser {
   std::vector<double> positions_m;  // n x 2 metres
   std::vector<float> object_guess;  // complex64
   int obj_guess_rows = 0, obj_guess_cols = 0;
+  bool obj_guess_centred = false;  // read from an objectGuess file: placed at the canvas centre
   std::vector<float> probe_guess;  // complex64 m x m
   int probe_guess_rows = 0, probe_guess_cols = 0;
 

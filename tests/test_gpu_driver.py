@@ -184,3 +184,77 @@ def test_mls_compact_groups_run_lowers_the_fourier_error(gpu, tmp_path, monkeypa
     assert np.isfinite(ph.get_object()).all() and np.isfinite(e9).all()
     assert e9.mean() < 0.7 * e1.mean()
     ph.close()
+
+
+# ---------------------------------------------------------------- input files (lf=, objectGuess, probeGuess)
+def test_positions_file_equals_position_list(gpu, tmp_path, monkeypatch):
+    """`lf=` (ListScanMesh::loadList, ListScanMesh.cpp:64-89): `x,y` lines in metres, parsed into
+    float, give the same list-mesh geometry as the same values passed as positionNP."""
+    from ptychopy_amd import _driver as D, ptychopy
+    monkeypatch.chdir(tmp_path)
+    amp, pos, canvas, kw, dx_s = _dataset(seed=4)
+    metres = ((pos.astype(np.float64) - pos.mean(axis=0)) * dx_s).astype(np.float32)
+    with open("pos.txt", "w") as f:
+        for x, y in metres:
+            f.write(f"{float(x):.9g},{float(y):.9g}\n")
+    ph_file = ptychopy._make_phaser("DM", *ptychopy._collect((), dict(diffractionNP=amp, lf="pos.txt", iter=2, **kw),
+                                                               True))
+    ph_file.init()
+    ph_list = ptychopy._make_phaser("DM", *ptychopy._collect(
+        (), dict(diffractionNP=amp, positionNP=metres.astype(np.float64), iter=2, **kw), True))
+    ph_list.init()
+    lpos, lcanvas, _ = D.list_mesh(metres.astype(np.float64), dx_s, 64)
+    assert np.array_equal(ph_file.get_positions(), ph_list.get_positions())
+    assert np.array_equal(ph_file.get_positions(), lpos)
+    for ph in (ph_file, ph_list):
+        ph.add_method("DM", 2)
+        ph.phase()
+    assert np.array_equal(ph_file.get_object(), ph_list.get_object())
+    ph_file.close()
+    ph_list.close()
+    # a malformed line is an error, not a silent partial mesh
+    with open("bad.txt", "w") as f:
+        f.write("1e-7;2e-7\n")
+    ph = ptychopy._make_phaser("DM", *ptychopy._collect((), dict(diffractionNP=amp, lf="bad.txt", iter=1, **kw), True))
+    with pytest.raises(RuntimeError, match="positions file"):
+        ph.init()
+    ph.close()
+
+
+@pytest.mark.parametrize("algorithm", ["DM", "MLs"])
+def test_object_and_probe_guess_files(gpu, tmp_path, monkeypatch, algorithm):
+    """`objectGuess=` is centred in the canvas (Sample::loadGuess -> h_setObjectArray,
+    SampleKernels.cu:626-640: offset rows/2 - r/2, cols/2 - c/2); `probeGuess=` becomes mode 0
+    (Probe::init, Probe.cpp:342-352).  MLs then rescales the object by one real factor
+    (Probe::initEvo's normalisation), so its placement is compared up to that factor."""
+    from ptychopy_amd import _driver as D, ptychopy
+    monkeypatch.chdir(tmp_path)
+    amp, pos, canvas, kw, dx_s = _dataset(seed=5)
+    rng = np.random.default_rng(9)
+    gr, gc = canvas[0] - 9, canvas[1] - 6
+    guess = (rng.uniform(0.5, 1, (gr, gc)) * np.exp(1j * rng.uniform(-1, 1, (gr, gc)))).astype(np.complex64)
+    yy, xx = np.mgrid[0:64, 0:64] - 32
+    pguess = (np.exp(-(xx ** 2 + yy ** 2) / 200.0) * np.exp(0.2j * xx)).astype(np.complex64)
+    assert D.save_complex("obj.csv", guess) and D.save_complex("probe.csv", pguess)
+    gtext, ptext = D.load_complex_csv("obj.csv"), D.load_complex_csv("probe.csv")  # the values as parsed
+    vals = dict(diffractionNP=amp ** 2 if algorithm == "MLs" else amp, objectGuess="obj.csv", probeGuess="probe.csv",
+                iter=2, **kw)
+    ph = ptychopy._make_phaser(algorithm, *ptychopy._collect((), vals, True))
+    ph.init()
+    ph.add_method(algorithm, 2)
+    ph.phaseinit()
+    o = ph.get_object().astype(np.complex128)
+    want = np.zeros(o.shape, np.complex128)
+    oi, oj = o.shape[0] // 2 - gr // 2, o.shape[1] // 2 - gc // 2
+    want[oi:oi + gr, oj:oj + gc] = gtext
+    if algorithm == "DM":
+        assert np.array_equal(o, want)
+        assert np.array_equal(ph.get_probe(), ptext)
+    else:
+        f = np.vdot(want, o) / np.vdot(want, want)
+        assert abs(f.imag) < 1e-6 * abs(f) and f.real > 0
+        assert rel_err(o, f * want) < 1e-6
+        p0 = ph.get_probe()[:64].astype(np.complex128)
+        fp = np.vdot(ptext, p0) / np.vdot(ptext, ptext)
+        assert rel_err(p0, fp * ptext) < 1e-5  # mode 0 = the guess up to the power normalisation
+    ph.close()

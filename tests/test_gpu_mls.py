@@ -81,6 +81,9 @@ CONFIGS = [
     dict(k=1, m=512, n_side=3, cfg=dict(probe_pos_search=2, variable_probe=0)),
     # groups of 9 and 7 frames: the probe-mode update of modes 2..K sums over several chunks
     dict(k=3, m=128, n_side=4, group=9, cfg=dict(probe_pos_search=2)),
+    # multimodal object updates at 128^2: every mode's own inverse pass and object step, no
+    # second-stream windows for the variable mode
+    dict(k=3, m=128, n_side=3, cfg=dict(apply_multimodal_update=1, probe_pos_search=2)),
 ]
 
 
@@ -127,6 +130,28 @@ def test_mls_deterministic_and_converging(gpu):
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
     errs = runs[0][2]
     assert errs[-1] < errs[0]
+
+
+@pytest.mark.parametrize("m,k", [(128, 1), (128, 3), (256, 3)])
+def test_mls_forked_schedule_is_bitwise_repeatable(gpu, m, k):
+    """At 128^2 and 256^2 the position step, the variable-mode update, the evolution dots and the
+    next group's illumination probes run on a second stream beside the main one (position search
+    from iteration 2, variable probe and intensity from iteration 3): two runs in one process
+    agree bit for bit in every state array."""
+    P = _problem(k=k, m=m, n_side=4, seed=11)
+    runs = []
+    for _ in range(2):
+        e = _engine_for(P, P["state"].amp, probe_pos_search=2, variable_probe=1, variable_intensity=1)
+        rng = np.random.default_rng(5)
+        for it in range(1, 6):
+            e.mls_iteration(it, mls_groups(P["n"], 4, rng))
+        g = e.mls_get_state()
+        runs.append((e.get_object(), e.get_probe(), g["variable_mode"], g["positions"], g["evolution"],
+                     g["beta_object"], g["beta_probe"]))
+        e.close()
+    for name, a, b in zip(("object", "probe", "variable mode", "positions", "evolution", "beta_object",
+                           "beta_probe"), *runs):
+        assert np.array_equal(a, b), name
 
 
 def test_mls_rejects_bad_groups(gpu):
