@@ -277,7 +277,11 @@ def run_mls(args) -> dict:
     n, m, k = W["n"], W["m"], W["k"]
     rows, cols = W["canvas"]
     rng = np.random.default_rng(20261204)
+    from ptychopy_amd import autotune
+    (cb, fpc), tune_ms = autotune.tune_mls(W["pos"], m, k, (rows, cols), W["box"])
     e = E.Engine(n, m, (rows, cols), n_modes=k)
+    e.set_tuning(E.TUNE_MLS_MOD_CB, cb)
+    e.set_tuning(E.TUNE_MLS_PUM_FPC, fpc)
     e.set_frames(W["amp"])
     amp_deq = e.get_frames()
     obj0 = np.ones((rows, cols), np.complex64)
@@ -357,6 +361,7 @@ def run_mls(args) -> dict:
         "flop_roofline": {"flops_per_frame": fpf, "achieved": tfs, "peak": 157.3, "unit": "TFLOP/s",
                           "frac": tfs / 157.3},
         "kernel_ms": kt,
+        "tuning": {"mls_mod_cb": cb, "mls_pum_fpc": fpc, "table_ms": tune_ms},
         "setup_s": t_gen,
     }
     if not args.no_cpu:
@@ -384,8 +389,8 @@ def main() -> int:
                     help="first copy 512 MiB device-to-device twice from a cache-cold source, with the runtime's "
                          "copy kernel and with an elementwise kernel (kernels of known traffic that scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
-    ap.add_argument("--dm-wave", type=int, default=64,
-                    help="frames per DM wave (0 automatic, -1 one pass over all frames)")
+    ap.add_argument("--dm-wave", type=int, default=None,
+                    help="frames per DM wave (default: the autotuner's choice; 0 automatic, -1 one pass)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -445,10 +450,13 @@ def main() -> int:
     t_gen = time.perf_counter() - t_gen
     n, m = wl["n"], wl["m"]
     rows, cols = wl["canvas"]
+    from ptychopy_amd._engine import TUNE_DM_WAVE
+    tune_ms = None
+    if args.dm_wave is None:
+        from ptychopy_amd import autotune
+        args.dm_wave, tune_ms = autotune.tune_dm_wave(wl["pos"], m, (rows, cols))
     e = Engine(n, m, (rows, cols))
-    if args.dm_wave is not None:
-        from ptychopy_amd._engine import TUNE_DM_WAVE
-        e.set_tuning(TUNE_DM_WAVE, args.dm_wave)
+    e.set_tuning(TUNE_DM_WAVE, args.dm_wave)
     e.set_frames(wl["amp"])
     e.set_positions(wl["pos"])
     e.clear_object()
@@ -513,6 +521,7 @@ def main() -> int:
                                "frac": it_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: float(v) for k, v in zip(KERNELS, kt)},
         "dm_wave_frames": e.get_tuning(1),
+        "tuning": {"dm_wave": args.dm_wave, "table_ms": tune_ms},
         "setup_s": t_gen,
     }
     if not args.no_cpu:

@@ -132,11 +132,15 @@ int ptyk_dm_iteration(ptyk_ctx* ctx, uint32_t flags);
  *                 probe apply, object apply, finish}. */
 int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
 
-/* Tuning knobs (no reference counterpart; chosen by the autotuner).
+/* Tuning knobs (no reference counterpart; chosen by the autotuner).  They change launch shapes
+ * only: every value computes the same iteration.
  * PTYK_TUNE_DM_WAVE: frames per DM wave for M in {128, 256, 512}; 0 = automatic (about
  * 128 MiB of exit waves per wave), -1 = one pass over all frames.  get returns the value in
- * effect (0 when the single-pass form runs). */
-enum { PTYK_TUNE_DM_WAVE = 1 };
+ * effect (0 when the single-pass form runs).
+ * PTYK_TUNE_MLS_MOD_CB: column width (8, 16 or 32; default 16) of the 64-row blocks of the MLs
+ * modulus pass for M in {128, 256, 512}.
+ * PTYK_TUNE_MLS_PUM_FPC: frames per chunk (2..64; default 4) of the MLs probe-mode update. */
+enum { PTYK_TUNE_DM_WAVE = 1, PTYK_TUNE_MLS_MOD_CB = 2, PTYK_TUNE_MLS_PUM_FPC = 3 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
 int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);
 

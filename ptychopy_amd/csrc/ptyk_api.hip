@@ -51,6 +51,7 @@ struct ptyk_ctx {
   int probe_chunks = 1, frames_per_chunk = 1;
   // wave-scheduled DM
   int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
+  int mls_mod_cb = 16, mls_pum_fpc = 4;  // tuning: MLs modulus block width, probe-mode chunk frames
   int dm_wave = 0, dm_waves = 0;
   double2* d_obj = nullptr;
   double2* d_probe = nullptr;
@@ -786,6 +787,17 @@ int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
     ++c->dm_gen;
     return PTYK_OK;
   }
+  if (key == PTYK_TUNE_MLS_MOD_CB) {
+    if (value != 8 && value != 16 && value != 32) return fail(c, PTYK_EINVAL, "MLs modulus block width: 8, 16 or 32");
+    c->mls_mod_cb = value;
+    return PTYK_OK;
+  }
+  if (key == PTYK_TUNE_MLS_PUM_FPC) {
+    if (value < MLS_PUM_FPC_MIN || value > 64)
+      return fail(c, PTYK_EINVAL, "MLs probe-mode chunk: 2 to 64 frames");
+    c->mls_pum_fpc = value;
+    return PTYK_OK;
+  }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
 }
 
@@ -793,6 +805,10 @@ int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
   if (!c || !value) return PTYK_EINVAL;
   if (key == PTYK_TUNE_DM_WAVE) {
     *value = dm_wave_frames(c);
+    return PTYK_OK;
+  }
+  if (key == PTYK_TUNE_MLS_MOD_CB || key == PTYK_TUNE_MLS_PUM_FPC) {
+    *value = key == PTYK_TUNE_MLS_MOD_CB ? c->mls_mod_cb : c->mls_pum_fpc;
     return PTYK_OK;
   }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
@@ -974,6 +990,8 @@ static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const
   if (r) return r;
   if (total > 0)
     HIPCHK(c, hipMemcpyAsync(d.gframes, group_frames, total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  d.mod_cb = c->mls_mod_cb;
+  d.pum_fpc = c->mls_pum_fpc;
   MlsTimer tm;
   if (kernel_ms) {
     r = timer_reserve(c, (size_t)16 * n_groups + 64);

@@ -647,15 +647,14 @@ struct ModeColLayout {
 // frequency rows R0 k1 + k2), s = |sum_k |Psi_k|^2|^(1/2), error partial 4 (A - s)^2, Psi_k *=
 // (A / (s + 1e-9) - 1) / M^2 (k_modulus's arithmetic), inverse columns. One wave per mode (64 K
 // threads) over 64 x 16 blocks: 16 KiB of LDS per mode, so four workgroups share a CU at K = 5.
-// One error partial per block: part[j * BPF + block].
-#define PTYK_MLS_MOD_CB 16
-template <int M, int K>
+// One error partial per block: part[j * BPF + block].  CB: the block's column width.
+template <int M, int K, int CB>
 __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi, int G, const __half* __restrict__ amp,
                                                        const float* __restrict__ amp_scale, const int* __restrict__ gf,
                                                        double* __restrict__ part, const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
   constexpr int BT = 64 * K;
-  constexpr int CB = PTYK_MLS_MOD_CB, NCB = M / CB, BPF = C::R0 * NCB;
+  constexpr int NCB = M / CB, BPF = C::R0 * NCB;
   constexpr int BLK = 64 * CB;         // elements per mode
   constexpr int PAIRS = BLK / 2 / 64;  // float4 per lane of the mode's wave
   extern __shared__ float2 lds[];      // K x 64 x CB (mode, row, column)
@@ -721,20 +720,20 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
   if (tid == 0) part[(long)j * BPF + blk] = err;
 }
 
-template <int M, int K>
+template <int M, int K, int CB>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
                                 double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
                                 hipStream_t s, MlsTimer* tm) {
   using C = WaveCfg<M>;
-  constexpr int BPF = C::R0 * (M / PTYK_MLS_MOD_CB);
+  constexpr int BPF = C::R0 * (M / CB);
   const long groups = (long)K * G * C::L, groups_c = (long)c_modes * G * C::L;
   const unsigned ga = (unsigned)((groups + C::GPW - 1) / C::GPW);
   const unsigned gc = (unsigned)((groups_c + C::GPW - 1) / C::GPW);
-  const size_t lds = (size_t)K * 64 * PTYK_MLS_MOD_CB * sizeof(float2);
+  const size_t lds = (size_t)K * 64 * CB * sizeof(float2);
   static bool lds_set = false;  // the attribute is per kernel and process: set it once
   if (!lds_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void*)k_mls_pass_b<M, K, CB>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     lds_set = true;
   }
@@ -745,43 +744,55 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
     hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
                        groups, tw);
   if (tm) tm->mark(s, MLS_T_PASS_B);
-  hipLaunchKernelGGL((k_mls_pass_b<M, K>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp, amp_scale,
-                     gf, part, tw);
+  hipLaunchKernelGGL((k_mls_pass_b<M, K, CB>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp,
+                     amp_scale, gf, part, tw);
   if (tm) tm->mark(s, MLS_T_PASS_C);
   hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
   return hipGetLastError();
 }
 
-template <int M>
-static hipError_t mls_modulus_m(float2* psi, int K, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s,
-                                MlsTimer* tm) {
-  switch (K) {
-    case 1: return mls_modulus_k<M, 1>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 2: return mls_modulus_k<M, 2>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 3: return mls_modulus_k<M, 3>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 4: return mls_modulus_k<M, 4>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 5: return mls_modulus_k<M, 5>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 6: return mls_modulus_k<M, 6>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 7: return mls_modulus_k<M, 7>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 8: return mls_modulus_k<M, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+template <int M, int K>
+static hipError_t mls_modulus_kc(float2* psi, int G, int cb, const __half* amp, const float* amp_scale, const int* gf,
+                                 double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s,
+                                 MlsTimer* tm) {
+  switch (cb) {
+    case 8: return mls_modulus_k<M, K, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 16: return mls_modulus_k<M, K, 16>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 32: return mls_modulus_k<M, K, 32>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
 
-int mls_modulus_blocks(int m, int k) {
-  if ((m != 128 && m != 256 && m != 512) || k < 1 || k > 8) return 0;
-  return (m / 64) * (m / PTYK_MLS_MOD_CB);
+template <int M>
+static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, const __half* amp, const float* amp_scale,
+                                const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int cm,
+                                hipStream_t s, MlsTimer* tm) {
+  switch (K) {
+    case 1: return mls_modulus_kc<M, 1>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 2: return mls_modulus_kc<M, 2>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 3: return mls_modulus_kc<M, 3>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 4: return mls_modulus_kc<M, 4>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 5: return mls_modulus_kc<M, 5>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 6: return mls_modulus_kc<M, 6>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 7: return mls_modulus_kc<M, 7>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 8: return mls_modulus_kc<M, 8>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    default: return hipErrorInvalidValue;
+  }
 }
 
-hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
+int mls_modulus_blocks(int m, int k, int cb) {
+  if ((m != 128 && m != 256 && m != 512) || k < 1 || k > 8 || (cb != 8 && cb != 16 && cb != 32)) return 0;
+  return (m / 64) * (m / cb);
+}
+
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
                               hipStream_t s, MlsTimer* tm) {
   if (c_modes < 1 || c_modes > K) return hipErrorInvalidValue;
-  switch (mls_modulus_blocks(m, K) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
-    case 256: return mls_modulus_m<256>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
-    case 512: return mls_modulus_m<512>(psi, K, G, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+  switch (mls_modulus_blocks(m, K, cb) ? m : 0) {
+    case 128: return mls_modulus_m<128>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+    case 256: return mls_modulus_m<256>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+    case 512: return mls_modulus_m<512>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -796,7 +807,8 @@ template <int M>
 __global__ __launch_bounds__(PTYK_NT) void k_pum_rows(const float2* __restrict__ psi, const float2* __restrict__ op,
                                                       const float* __restrict__ illum, const int2* __restrict__ roi,
                                                       const int* __restrict__ gf, int K, int G, int cols, float fw,
-                                                      float2* __restrict__ part, const float2* __restrict__ tw) {
+                                                      float2* __restrict__ part, int fpc,
+                                                      const float2* __restrict__ tw) {
   using C = WaveCfg<M>;
   using IO = RowIO<M>;
   using L = SwzRowLayout<M, M>;
@@ -804,7 +816,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_pum_rows(const float2* __restrict__
   __shared__ float4 acc[IO::PAIRS * PTYK_NT];  // per-thread sums (LDS keeps them out of the registers)
   const int tid = threadIdx.x;
   const int k = blockIdx.y + 1;
-  const int j0 = blockIdx.z * MLS_PUM_FPC, j1 = min(G, j0 + MLS_PUM_FPC);
+  const int j0 = blockIdx.z * fpc, j1 = min(G, j0 + fpc);
   const long mm = (long)M * M;
   constexpr long NOLIM = 1L << 40;  // locate's bound: the tile always lies inside its plane
 #pragma unroll
@@ -878,11 +890,11 @@ __global__ void k_pum_finish(float2* __restrict__ probe, const float2* __restric
 template <int M>
 static hipError_t mode_updates_m(float2* probe, const float2* psi, const float2* op, const float* illum,
                                  const int2* roi, const int* gf, int K, int G, int cols, float fw,
-                                 const double* beta, float2* part, const float2* tw, hipStream_t s) {
+                                 const double* beta, float2* part, int fpc, const float2* tw, hipStream_t s) {
   using C = WaveCfg<M>;
-  const int chunks = (G + MLS_PUM_FPC - 1) / MLS_PUM_FPC;
+  const int chunks = (G + fpc - 1) / fpc;
   hipLaunchKernelGGL((k_pum_rows<M>), dim3(C::L / C::GPW, K - 1, chunks), dim3(PTYK_NT), 0, s, psi, op, illum, roi,
-                     gf, K, G, cols, fw, part, tw);
+                     gf, K, G, cols, fw, part, fpc, tw);
   const long mm = (long)M * M;
   hipLaunchKernelGGL(k_pum_finish, dim3((unsigned)((mm + PTYK_NT - 1) / PTYK_NT), K - 1), dim3(PTYK_NT), 0, s, probe,
                      part, K, chunks, G, mm, beta);
@@ -891,12 +903,12 @@ static hipError_t mode_updates_m(float2* probe, const float2* psi, const float2*
 
 hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
                             const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
-                            const float2* tw, hipStream_t s) {
-  if (K < 2 || G < 1) return hipErrorInvalidValue;
+                            int fpc, const float2* tw, hipStream_t s) {
+  if (K < 2 || G < 1 || fpc < MLS_PUM_FPC_MIN) return hipErrorInvalidValue;
   switch (m) {
-    case 128: return mode_updates_m<128>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
-    case 256: return mode_updates_m<256>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
-    case 512: return mode_updates_m<512>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, tw, s);
+    case 128: return mode_updates_m<128>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
+    case 256: return mode_updates_m<256>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
+    case 512: return mode_updates_m<512>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -88,8 +88,9 @@ bool dm_wave_supported(int m);
 int dm_wave_probe_blocks(int m);
 hipError_t dm_wave_iteration(const DmWaveArgs& a, hipStream_t s, DmTimer* tm /* nullable */);
 // MLs forward FFT + multi-mode modulus step + inverse FFT of K mode-major planes of G frames in
-// three passes (ptyk_dm_wave.hip). mls_modulus_blocks: error partials per frame, 0 when (m, K)
-// is not supported (m in {128, 256, 512}, K <= 8).
+// three passes (ptyk_dm_wave.hip). mls_modulus_blocks: error partials per frame, 0 when (m, K, cb)
+// is not supported (m in {128, 256, 512}, K <= 8, cb in {8, 16, 32}: the column width of the
+// middle pass's 64-row blocks).
 // exit: when given, the first pass forms the exit waves itself (psi_k = P_k O_r, or O_r vp_j for
 // mode 0 with the sub-pixel shift) and writes the object patches op (and op2); otherwise psi
 // already holds them.
@@ -104,7 +105,7 @@ struct MlsExitArgs {
   float2* op;
   float2* op2;  // nullable
 };
-int mls_modulus_blocks(int m, int k);
+int mls_modulus_blocks(int m, int k, int cb);
 // MLs launch categories of ptyk_mls_iteration_timed (main stream only; the second stream's
 // launches overlap them and are not charged).
 enum { MLS_T_ITER, MLS_T_PROBES, MLS_T_PASS_A, MLS_T_PASS_B, MLS_T_PASS_C, MLS_T_PROBE_UPDATE, MLS_T_PRECOND,
@@ -116,20 +117,21 @@ hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int
                             const int* gf, float sign, const float2* tw, hipStream_t s);
 // c_modes: mode planes the inverse row pass covers (K, or 1 when modes 2..K are left in their
 // column-transformed state for mls_mode_updates).
-hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, const __half* amp, const float* amp_scale,
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
                               int c_modes, hipStream_t s, MlsTimer* tm = nullptr);
 // Probe updates of modes 2..K (P_k += mean_j w_j chi_k,j conj(op_j) * beta) straight from the
 // column-transformed planes: the inverse row pass and the frame sum in one kernel over chunks of
-// MLS_PUM_FPC frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame
-// order (m in {128, 256, 512}; ptyk_dm_wave.hip).  beta: the mean probe step (re, im).
-constexpr int MLS_PUM_FPC = 4;
-inline long mls_pum_part_len(int g, int k, long mm) {
-  return (long)((g + MLS_PUM_FPC - 1) / MLS_PUM_FPC) * (k > 1 ? k - 1 : 1) * mm;
+// fpc frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame order
+// (m in {128, 256, 512}, fpc >= MLS_PUM_FPC_MIN; ptyk_dm_wave.hip).  beta: the mean probe step
+// (re, im).
+constexpr int MLS_PUM_FPC_MIN = 2;
+inline long mls_pum_part_len(int g, int k, long mm) {  // room for any fpc >= MLS_PUM_FPC_MIN
+  return (long)((g + MLS_PUM_FPC_MIN - 1) / MLS_PUM_FPC_MIN) * (k > 1 ? k - 1 : 1) * mm;
 }
 hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
                             const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
-                            const float2* tw, hipStream_t s);
+                            int fpc, const float2* tw, hipStream_t s);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {
@@ -161,6 +163,9 @@ struct MlsDev {
   int box[4] = {0, 0, 0, 0};
   double delta_p = 0.1, beta_lsq = 0.9, beta_object = 1.0, beta_probe = 1.0;
   float max_illum = 0.f;  // Probe::MAX_ILLUM, fixed by initEvo
+  // launch shapes (tuning knobs, same results): column width of the modulus pass's blocks, frames
+  // per chunk of the probe-mode update
+  int mod_cb = 16, pum_fpc = 4;
   // context arrays (not owned)
   float2* obj = nullptr;
   float2* probe = nullptr;

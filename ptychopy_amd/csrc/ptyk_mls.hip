@@ -1367,7 +1367,8 @@ long mls_part_min(int k) { return (6L + std::max(k, ORTHO_KMAX)) * ORTHO_NB; }
     if (tm) tm->mark(s, (c)); \
   } while (0)
 
-hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s, MlsTimer* tm) {
+static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s,
+                                     MlsTimer* tm) {
   const int m = d.m, mm = m * m, K = d.k, ch = mls_chunks(m);
   const long canvas = (long)d.rows * d.cols;
   const dim3 tiles((d.cols + 63) / 64, (d.rows + 3) / 4);
@@ -1408,7 +1409,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   const bool obj_rec_iter = iter >= d.object_reconstruct;
   const bool probe_rec = iter >= d.probe_reconstruct;
 
-  const int mod_blocks = mls_modulus_blocks(m, K);
+  const int mod_blocks = mls_modulus_blocks(m, K, d.mod_cb);
   // the illumination probes of group g + 1 (spectra of P_0 and V, shifted probes) are formed on
   // the second stream while the probe modes 2..K of group g are updated: they read only P_0, V,
   // the evolution coefficients and sub-pixel shifts of group g + 1's frames, none of which the
@@ -1446,8 +1447,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
       // from O, P and vp; the position step's patch copy takes its own forward transform
       const bool spec2d = pos_step && !pos_1d_step;
       const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
-      TRY(mls_modulus_passes(d.psi, K, G, m, d.amp, d.amp_scale, gf, d.part, d.tw, &ex, defer_modes ? 1 : K, s,
-                             tm));
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
+                             defer_modes ? 1 : K, s, tm));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
       if (pos_1d_step) {
         // the position step reads only chi0, vp and op, which stay fixed until the group ends,
@@ -1630,7 +1631,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
     MARK(MLS_T_MODES);
     if (defer_modes) {
       TRY(mls_mode_updates(d.probe, d.psi, d.op, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
-                           d.pum_part, d.tw, s));
+                           d.pum_part, d.pum_fpc, d.tw, s));
     } else if (fuse_modes) {
       LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, m,
              d.cols, fw, d.gscal);
@@ -1643,6 +1644,15 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   if (K > d.n_rec) TRY(mls_ortho(d, s));
   MARK(-1);
   return hipSuccess;
+}
+
+hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s, MlsTimer* tm) {
+  const hipError_t e = mls_iteration_body(d, iter, goffs, n_groups, s, tm);
+  // an error between a fork and its join leaves work on the second stream that still reads the
+  // group buffers and writes the positions, V and the evolution: it finishes before the error is
+  // reported, so that no later call overwrites what it reads
+  if (e != hipSuccess && d.side) (void)hipStreamSynchronize(d.side);
+  return e;
 }
 
 #undef MARK

@@ -164,3 +164,57 @@ def test_mls_rejects_bad_groups(gpu):
         e.mls_iteration(1, [np.array([P["n"]])])
     with pytest.raises(_engine.PtykError):
         e.mls_iteration(0, [np.array([0])])
+
+
+@pytest.mark.parametrize("m", [128, 256])
+def test_mls_launch_shapes_give_the_same_iteration(gpu, orc, m):
+    """The tuning knobs change launch shapes only: every modulus block width gives the same object
+    and probe bit for bit (the per-column transforms and per-pixel modulus are the same arithmetic),
+    every probe-mode chunk size the same within FP32 rounding of the chunk sums; each stays within
+    1e-5 of the oracle."""
+    from ptychopy_amd import _engine as E
+    P = _problem(k=3, m=m, n_side=4, seed=21)
+    rng0 = np.random.default_rng(8)
+    groups = [mls_groups(P["n"], 9, rng0) for _ in range(3)]
+
+    def run(cb, fpc):
+        e = _engine_for(P, P["state"].amp, probe_pos_search=2)
+        e.set_tuning(E.TUNE_MLS_MOD_CB, cb)
+        e.set_tuning(E.TUNE_MLS_PUM_FPC, fpc)
+        assert (e.get_tuning(E.TUNE_MLS_MOD_CB), e.get_tuning(E.TUNE_MLS_PUM_FPC)) == (cb, fpc)
+        for it in (1, 2):
+            e.mls_iteration(it, groups[it - 1])
+        amp = e.get_frames().astype(np.float64)
+        o = _oracle_from_engine(e, orc, amp)
+        e.mls_iteration(3, groups[2])
+        o.iteration(orc.mls_params(P["box"], iter=3, pps=2), groups[2])
+        out = (e.get_object(), e.get_probe())
+        assert rel_err(out[0], o.obj) < 1e-5 and rel_err(out[1], o.probe) < 1e-5
+        e.close()
+        return out
+
+    base = run(16, 4)
+    for cb in (8, 32):
+        o, p = run(cb, 4)
+        assert np.array_equal(o, base[0]) and np.array_equal(p, base[1]), cb
+    for fpc in (2, 8):
+        o, p = run(16, fpc)
+        assert rel_err(o, base[0]) < 1e-6 and rel_err(p, base[1]) < 1e-6, fpc
+    from ptychopy_amd._engine import PtykError
+    e = _engine_for(P, P["state"].amp)
+    for key, bad in ((E.TUNE_MLS_MOD_CB, 12), (E.TUNE_MLS_PUM_FPC, 1)):
+        with pytest.raises(PtykError):
+            e.set_tuning(key, bad)
+    e.close()
+
+
+def test_mls_autotuner_persists_a_timing_table(gpu, tmp_path):
+    import json
+    from ptychopy_amd import autotune
+    P = _problem(k=2, m=128, n_side=4, seed=22)
+    path = tmp_path / "tune.json"
+    best, times = autotune.tune_mls(P["pos"], 128, 2, P["shape"], P["box"], iters=1, path=path)
+    assert f"{best[0]},{best[1]}" in times and len(times) == 9
+    table = json.loads(path.read_text())
+    assert table[autotune.mls_key(P["n"], 128, 2, P["shape"], max(1, P["n"] // 100))] == times
+    assert autotune.tune_mls(P["pos"], 128, 2, P["shape"], P["box"], path=path) == (best, times)
