@@ -278,10 +278,15 @@ def run_mls(args) -> dict:
     rows, cols = W["canvas"]
     rng = np.random.default_rng(20261204)
     from ptychopy_amd import autotune
-    (cb, fpc), tune_ms = autotune.tune_mls(W["pos"], m, k, (rows, cols), W["box"])
+    if args.mls_shape:
+        cb, fpc, graphs = (int(v) for v in args.mls_shape.split(","))
+        tune_ms = None
+    else:
+        (cb, fpc, graphs), tune_ms = autotune.tune_mls(W["pos"], m, k, (rows, cols), W["box"])
     e = E.Engine(n, m, (rows, cols), n_modes=k)
     e.set_tuning(E.TUNE_MLS_MOD_CB, cb)
     e.set_tuning(E.TUNE_MLS_PUM_FPC, fpc)
+    e.set_tuning(E.TUNE_MLS_GRAPHS, graphs)
     e.set_frames(W["amp"])
     amp_deq = e.get_frames()
     obj0 = np.ones((rows, cols), np.complex64)
@@ -361,7 +366,7 @@ def run_mls(args) -> dict:
         "flop_roofline": {"flops_per_frame": fpf, "achieved": tfs, "peak": 157.3, "unit": "TFLOP/s",
                           "frac": tfs / 157.3},
         "kernel_ms": kt,
-        "tuning": {"mls_mod_cb": cb, "mls_pum_fpc": fpc, "table_ms": tune_ms},
+        "tuning": {"mls_mod_cb": cb, "mls_pum_fpc": fpc, "mls_group_graphs": graphs, "table_ms": tune_ms},
         "setup_s": t_gen,
     }
     if not args.no_cpu:
@@ -389,6 +394,9 @@ def main() -> int:
                     help="first copy 512 MiB device-to-device twice from a cache-cold source, with the runtime's "
                          "copy kernel and with an elementwise kernel (kernels of known traffic that scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    ap.add_argument("--mls-shape", default=None,
+                    help="MLs launch shape 'CB,FPC,GRAPHS' (modulus block width, probe-mode chunk frames, "
+                         "group graphs 0/1); default: the autotuner's choice")
     ap.add_argument("--dm-wave", type=int, default=None,
                     help="frames per DM wave (default: the autotuner's choice; 0 automatic, -1 one pass)")
     args = ap.parse_args()

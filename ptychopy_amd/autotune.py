@@ -92,11 +92,13 @@ def mls_key(n: int, m: int, k: int, canvas, group: int) -> str:
 
 def tune_mls(pos, m: int, k: int, canvas, box, iters: int = 2, cb_candidates=MLS_CB_CANDIDATES,
              fpc_candidates=MLS_FPC_CANDIDATES, path: Path | None = None, force: bool = False, device: int = 0):
-    """Best (modulus block width, probe-mode chunk frames) for an MLs problem with these positions,
-    timed on a scratch handle over full iterations (position refinement and the variable probe on,
-    the reference's max(1, N/100)-frame groups); returns ((cb, fpc), {"cb,fpc": ms})."""
+    """Best (modulus block width, probe-mode chunk frames, group graphs on/off) for an MLs problem
+    with these positions, timed on a scratch handle over full iterations (position refinement and
+    the variable probe on, the reference's max(1, N/100)-frame groups): every (cb, fpc) pair with
+    direct launches, then the best pair replayed from group graphs.  Returns ((cb, fpc, graphs),
+    {"cb,fpc,graphs": ms})."""
     from . import _driver
-    from ._engine import TUNE_MLS_MOD_CB, TUNE_MLS_PUM_FPC, mls_config
+    from ._engine import TUNE_MLS_GRAPHS, TUNE_MLS_MOD_CB, TUNE_MLS_PUM_FPC, mls_config
     pos = np.ascontiguousarray(pos, np.float32)
     n = pos.shape[0]
     path = path or table_path()
@@ -106,8 +108,7 @@ def tune_mls(pos, m: int, k: int, canvas, box, iters: int = 2, cb_candidates=MLS
     if not force and key in table:
         t = table[key]
         best = min(t, key=t.get)
-        cb, fpc = (int(v) for v in best.split(","))
-        return (cb, fpc), t
+        return tuple(int(v) for v in best.split(",")), t
     rng = np.random.default_rng(0)
     times = {}
     with Engine(n, m, canvas, n_modes=k, device=device) as e:
@@ -121,19 +122,24 @@ def tune_mls(pos, m: int, k: int, canvas, box, iters: int = 2, cb_candidates=MLS
         e.mls_begin(cfg, pos, vmode, 1e-6 * rng.standard_normal(n))
         part = _driver.mls_partitions(n, 1, 0)[0]
         it = 3  # position refinement, variable probe and intensity all on
+
+        def timed(cb, fpc, graphs):
+            e.set_tuning(TUNE_MLS_MOD_CB, cb)
+            e.set_tuning(TUNE_MLS_PUM_FPC, fpc)
+            e.set_tuning(TUNE_MLS_GRAPHS, graphs)
+            e.mls_iteration(it, part)  # warm-up of this shape (and the graphs' capture)
+            ts = []
+            for _ in range(iters):
+                t0 = time.perf_counter()
+                e.mls_iteration(it, part)
+                ts.append(time.perf_counter() - t0)
+            times[f"{cb},{fpc},{graphs}"] = float(np.median(ts) * 1e3)
+
         for cb in cb_candidates:
             for fpc in fpc_candidates:
-                e.set_tuning(TUNE_MLS_MOD_CB, cb)
-                e.set_tuning(TUNE_MLS_PUM_FPC, fpc)
-                e.mls_iteration(it, part)  # warm-up of this shape
-                ts = []
-                for _ in range(iters):
-                    t0 = time.perf_counter()
-                    e.mls_iteration(it, part)
-                    ts.append(time.perf_counter() - t0)
-                times[f"{cb},{fpc}"] = float(np.median(ts) * 1e3)
+                timed(cb, fpc, 0)
+        cb, fpc, _ = (int(v) for v in min(times, key=times.get).split(","))
+        timed(cb, fpc, 1)
     table[key] = times
     _store(path, table)
-    best = min(times, key=times.get)
-    cb, fpc = (int(v) for v in best.split(","))
-    return (cb, fpc), times
+    return tuple(int(v) for v in min(times, key=times.get).split(",")), times

@@ -52,6 +52,7 @@ struct ptyk_ctx {
   // wave-scheduled DM
   int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
   int mls_mod_cb = 16, mls_pum_fpc = 4;  // tuning: MLs modulus block width, probe-mode chunk frames
+  int mls_graphs = 1;                    // tuning: MLs groups replayed from captured graphs
   int dm_wave = 0, dm_waves = 0;
   double2* d_obj = nullptr;
   double2* d_probe = nullptr;
@@ -114,12 +115,18 @@ static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
-                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot};  // tmp, tmp2 in psi
+                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot,
+                  d.goffs_dev, d.gsel};  // tmp, tmp2 in psi
+  mls_drop_graphs(d);
   for (void* p : ptrs)
     if (p) hipFree(p);
-  if (d.ev_fork) hipEventDestroy(d.ev_fork);
-  if (d.ev_join) hipEventDestroy(d.ev_join);
+  for (hipEvent_t ev : d.ev_side)
+    if (ev) hipEventDestroy(ev);
+  for (hipEvent_t ev : d.ev_join)
+    if (ev) hipEventDestroy(ev);
+  if (d.ev_vpdone) hipEventDestroy(d.ev_vpdone);
   if (d.side) hipStreamDestroy(d.side);
+  if (d.side2) hipStreamDestroy(d.side2);
   d = MlsDev();
   c->mls_ready = false;
 }
@@ -798,6 +805,11 @@ int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
     c->mls_pum_fpc = value;
     return PTYK_OK;
   }
+  if (key == PTYK_TUNE_MLS_GRAPHS) {
+    if (value != 0 && value != 1) return fail(c, PTYK_EINVAL, "MLs group graphs: 0 or 1");
+    c->mls_graphs = value;
+    return PTYK_OK;
+  }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
 }
 
@@ -807,8 +819,8 @@ int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
     *value = dm_wave_frames(c);
     return PTYK_OK;
   }
-  if (key == PTYK_TUNE_MLS_MOD_CB || key == PTYK_TUNE_MLS_PUM_FPC) {
-    *value = key == PTYK_TUNE_MLS_MOD_CB ? c->mls_mod_cb : c->mls_pum_fpc;
+  if (key == PTYK_TUNE_MLS_MOD_CB || key == PTYK_TUNE_MLS_PUM_FPC || key == PTYK_TUNE_MLS_GRAPHS) {
+    *value = key == PTYK_TUNE_MLS_MOD_CB ? c->mls_mod_cb : key == PTYK_TUNE_MLS_PUM_FPC ? c->mls_pum_fpc : c->mls_graphs;
     return PTYK_OK;
   }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
@@ -823,15 +835,19 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
   if (!d.side) {
     HIPCHK(c, hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
-    HIPCHK(c, hipEventCreateWithFlags(&d.ev_fork, hipEventDisableTiming));
-    HIPCHK(c, hipEventCreateWithFlags(&d.ev_join, hipEventDisableTiming));
+    for (hipEvent_t& ev : d.ev_side) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(c, hipStreamCreateWithFlags(&d.side2, hipStreamNonBlocking));
+    for (hipEvent_t& ev : d.ev_join) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&d.ev_vpdone, hipEventDisableTiming));
   }
   if (g > d.gmax) {
-    void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,
+    mls_drop_graphs(d);  // they hold the addresses of the scratch about to move
+    void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,  d.gsel,
                     d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot};
     for (void* p : ptrs)  // tmp and tmp2 live inside psi
       if (p) hipFree(p);
     d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = d.dxb = d.pum_part = nullptr;
+    d.gsel = nullptr;
     d.gscal = d.ftot = d.pos_part = d.pos_ftot = d.vp_part = d.vp_ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
@@ -852,11 +868,13 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.pum_part, (size_t)mls_pum_part_len(g, d.k, mm)));
     HIPCHK(c, dalloc(&d.vp_part, (size_t)g * ch + 2 * nb));
     HIPCHK(c, dalloc(&d.vp_ftot, (size_t)8 * g));
+    HIPCHK(c, dalloc(&d.gsel, (size_t)1 + 2 * g));
     d.gmax = g;
   }
   const long need = std::max<long>(
       {6L * g * ch, (long)g * ch + 2 * nb, 2L * std::max(d.rows, d.cols), 1024L, mls_part_min(d.k)});
   if (need > d.part_cap) {
+    mls_drop_graphs(d);
     if (d.part) hipFree(d.part);
     d.part = nullptr;
     d.part_cap = 0;
@@ -942,6 +960,7 @@ int ptyk_mls_begin(ptyk_ctx* c, const ptyk_mls_config* cfg, const float* positio
   HIPCHK(c, dalloc(&d.acc, c->mm));
   HIPCHK(c, dalloc(&d.precond, canvas));
   HIPCHK(c, dalloc(&d.gframes, n));
+  HIPCHK(c, dalloc(&d.goffs_dev, (size_t)n + 1));
   int r = mls_reserve(c, 1);
   if (r) return r;
   std::vector<double> evo(2 * (size_t)n, 1.0);
@@ -990,8 +1009,11 @@ static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const
   if (r) return r;
   if (total > 0)
     HIPCHK(c, hipMemcpyAsync(d.gframes, group_frames, total * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.goffs_dev, group_offsets, (n_groups + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
   d.mod_cb = c->mls_mod_cb;
   d.pum_fpc = c->mls_pum_fpc;
+  d.use_graphs = c->mls_graphs != 0;
   MlsTimer tm;
   if (kernel_ms) {
     r = timer_reserve(c, (size_t)16 * n_groups + 64);

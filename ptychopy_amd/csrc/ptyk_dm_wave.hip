@@ -592,6 +592,93 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pu_rows(float2* __restrict__ pu
   }
 }
 
+// The inverse row pass of mode 0 (k_dmw_pass_c's arithmetic, chi_0 written back for the rest of
+// the group) followed, while the rows are still in LDS, by the row shifts of the probe update
+// (k_mls_pu_rows's arithmetic on chi_0 conj(op)): one pass instead of two, and chi_0 is not read
+// back.  Planes 0 .. G - 1 (mode 0) only; a workgroup's rows belong to one plane.
+template <int M>
+__global__ __launch_bounds__(PTYK_NT) void k_mls_pass_c_pu(float2* __restrict__ psi, long n_groups,
+                                                           float2* __restrict__ pu, const float2* __restrict__ op,
+                                                           const float2* __restrict__ sub, const int* __restrict__ gf,
+                                                           float sign, const float2* __restrict__ tw) {
+  using C = WaveCfg<M>;
+  using IO = RowIO<M>;
+  using L = SwzRowLayout<M, M>;
+  static_assert(C::L % C::GPW == 0, "a workgroup's row groups must share a plane");
+  static_assert((2 * PTYK_NT) % M == 0, "a thread's columns must repeat across its rows");
+  __shared__ float2 lds[C::ROWS * M];
+  const int tid = threadIdx.x;
+  const long mm = (long)M * M;
+  const int j = (int)(((long)blockIdx.x * C::GPW) / C::L);  // plane = frame of the group
+  {
+    float4 v[IO::PAIRS];
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      long f;
+      int r, c;
+      v[i] = IO::locate(e, 0, n_groups, &f, &r, &c) ? *reinterpret_cast<const float4*>(psi + (f * M + r) * M + c)
+                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = v[i];
+    }
+  }
+  // the object patch rows of the probe update, in flight during the inverse transform
+  float4 bv[IO::PAIRS];
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    long f;
+    int r, c;
+    IO::locate(2 * (tid + i * PTYK_NT), 0, n_groups, &f, &r, &c);
+    bv[i] = *reinterpret_cast<const float4*>(op + (long)j * mm + r * M + c);
+  }
+  __syncthreads();
+  cross_rows<M, true>(lds, tw, tid);
+  __syncthreads();
+  fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c;
+    IO::locate(e, 0, n_groups, &f, &r, &c);
+    float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]);
+    const float4 a = *p, b = bv[i];
+    *reinterpret_cast<float4*>(psi + (f * M + r) * M + c) = a;
+    *p = make_float4(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y, a.z * b.z + a.w * b.w, a.w * b.z - a.z * b.w);
+  }
+  const float sx = sign * sub[gf[j]].x;
+  const int c = (2 * tid) % M;
+  float2 w0, w1;
+  sincosf(sx * shift_freq(c, M) * -6.283185307179586f, &w0.y, &w0.x);
+  sincosf(sx * shift_freq(c + 1, M) * -6.283185307179586f, &w1.y, &w1.x);
+  __syncthreads();
+  fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+  const float inv_m = 1.f / (float)M;
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, c)]);
+    const float4 t = *p;
+    const float2 u0 = cscale(cmul(make_float2(t.x, t.y), w0), inv_m);
+    const float2 u1 = cscale(cmul(make_float2(t.z, t.w), w1), inv_m);
+    *p = make_float4(u0.x, u0.y, u1.x, u1.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+  for (int i = 0; i < IO::PAIRS; ++i) {
+    const int e = 2 * (tid + i * PTYK_NT);
+    long f;
+    int r, c2;
+    IO::locate(e, 0, n_groups, &f, &r, &c2);
+    *reinterpret_cast<float4*>(pu + (f * M + r) * M + c2) = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+  }
+}
+
 // Column shifts in place: a workgroup takes NC columns (whole 16-column strips) of plane j.
 template <int M>
 __global__ __launch_bounds__(PTYK_NT) void k_mls_shift_cols(float2* __restrict__ x, const float2* __restrict__ sub,
@@ -723,7 +810,7 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
 template <int M, int K, int CB>
 static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
                                 double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
-                                hipStream_t s, MlsTimer* tm) {
+                                const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   using C = WaveCfg<M>;
   constexpr int BPF = C::R0 * (M / CB);
   const long groups = (long)K * G * C::L, groups_c = (long)c_modes * G * C::L;
@@ -747,18 +834,22 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
   hipLaunchKernelGGL((k_mls_pass_b<M, K, CB>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp,
                      amp_scale, gf, part, tw);
   if (tm) tm->mark(s, MLS_T_PASS_C);
-  hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
+  if (shift && c_modes == 1)
+    hipLaunchKernelGGL((k_mls_pass_c_pu<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, groups_c, shift->pu, shift->op,
+                       shift->sub, gf, shift->sign, tw);
+  else
+    hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
   return hipGetLastError();
 }
 
 template <int M, int K>
 static hipError_t mls_modulus_kc(float2* psi, int G, int cb, const __half* amp, const float* amp_scale, const int* gf,
-                                 double* part, const float2* tw, const MlsExitArgs* exit, int cm, hipStream_t s,
-                                 MlsTimer* tm) {
+                                 double* part, const float2* tw, const MlsExitArgs* exit, int cm,
+                                 const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   switch (cb) {
-    case 8: return mls_modulus_k<M, K, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 16: return mls_modulus_k<M, K, 16>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 32: return mls_modulus_k<M, K, 32>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 8: return mls_modulus_k<M, K, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 16: return mls_modulus_k<M, K, 16>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 32: return mls_modulus_k<M, K, 32>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -766,16 +857,16 @@ static hipError_t mls_modulus_kc(float2* psi, int G, int cb, const __half* amp, 
 template <int M>
 static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, const __half* amp, const float* amp_scale,
                                 const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int cm,
-                                hipStream_t s, MlsTimer* tm) {
+                                const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   switch (K) {
-    case 1: return mls_modulus_kc<M, 1>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 2: return mls_modulus_kc<M, 2>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 3: return mls_modulus_kc<M, 3>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 4: return mls_modulus_kc<M, 4>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 5: return mls_modulus_kc<M, 5>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 6: return mls_modulus_kc<M, 6>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 7: return mls_modulus_kc<M, 7>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
-    case 8: return mls_modulus_kc<M, 8>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, s, tm);
+    case 1: return mls_modulus_kc<M, 1>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 2: return mls_modulus_kc<M, 2>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 3: return mls_modulus_kc<M, 3>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 4: return mls_modulus_kc<M, 4>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 5: return mls_modulus_kc<M, 5>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 6: return mls_modulus_kc<M, 6>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 7: return mls_modulus_kc<M, 7>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 8: return mls_modulus_kc<M, 8>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -787,12 +878,12 @@ int mls_modulus_blocks(int m, int k, int cb) {
 
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
-                              hipStream_t s, MlsTimer* tm) {
+                              const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   if (c_modes < 1 || c_modes > K) return hipErrorInvalidValue;
   switch (mls_modulus_blocks(m, K, cb) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
-    case 256: return mls_modulus_m<256>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
-    case 512: return mls_modulus_m<512>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, s, tm);
+    case 128: return mls_modulus_m<128>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
+    case 256: return mls_modulus_m<256>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
+    case 512: return mls_modulus_m<512>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -921,6 +1012,20 @@ static hipError_t mls_shift_m(float2* pu, const float2* chi, const float2* op, i
                      sign, tw);
   hipLaunchKernelGGL((k_mls_shift_cols<M>), dim3(M / NC, G), dim3(PTYK_NT), 0, s, pu, sub, gf, sign, tw);
   return hipGetLastError();
+}
+
+hipError_t mls_shift_cols(float2* pu, int G, int m, const float2* sub, const int* gf, float sign, const float2* tw,
+                          hipStream_t s) {
+  auto go = [&](auto kern, int nc) {
+    hipLaunchKernelGGL(kern, dim3(m / nc, G), dim3(PTYK_NT), 0, s, pu, sub, gf, sign, tw);
+    return hipGetLastError();
+  };
+  switch (m) {
+    case 128: return go(k_mls_shift_cols<128>, (plan_min_batch<128, PTYK_NT>() + 15) / 16 * 16);
+    case 256: return go(k_mls_shift_cols<256>, (plan_min_batch<256, PTYK_NT>() + 15) / 16 * 16);
+    case 512: return go(k_mls_shift_cols<512>, (plan_min_batch<512, PTYK_NT>() + 15) / 16 * 16);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int G, int m, const float2* sub,

@@ -4,6 +4,8 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include <map>
+
 namespace ptyk {
 
 // Supported transform lengths (square frames M x M).
@@ -117,9 +119,22 @@ hipError_t mls_shift_passes(float2* pu, const float2* chi, const float2* op, int
                             const int* gf, float sign, const float2* tw, hipStream_t s);
 // c_modes: mode planes the inverse row pass covers (K, or 1 when modes 2..K are left in their
 // column-transformed state for mls_mode_updates).
+// shift: when given with c_modes == 1, the inverse row pass also forms the probe update's row
+// shifts of chi_0 conj(op) into pu (the first of mls_shift_passes' two passes; mls_shift_cols
+// finishes them).
+struct MlsShiftArgs {
+  float2* pu;
+  const float2* op;
+  const float2* sub;
+  float sign;
+};
 hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
-                              int c_modes, hipStream_t s, MlsTimer* tm = nullptr);
+                              int c_modes, const MlsShiftArgs* shift /* nullable */, hipStream_t s,
+                              MlsTimer* tm = nullptr);
+// The column shifts of mls_shift_passes alone (after a fused row pass).
+hipError_t mls_shift_cols(float2* pu, int G, int m, const float2* sub, const int* gf, float sign, const float2* tw,
+                          hipStream_t s);
 // Probe updates of modes 2..K (P_k += mean_j w_j chi_k,j conj(op_j) * beta) straight from the
 // column-transformed planes: the inverse row pass and the frame sum in one kernel over chunks of
 // fpc frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame order
@@ -200,9 +215,21 @@ struct MlsDev {
   float2* pum_part = nullptr;   // chunk partials of the probe-mode updates (mls_pum_part_len)
   double* vp_part = nullptr;    // variable-probe update partials on the second stream
   double* vp_ftot = nullptr;    // its per-frame totals, gmax
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t side = nullptr, side2 = nullptr;  // second-stream work of a group (streams A and B)
+  hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};  // fork points of the second stream's work
+  hipEvent_t ev_vpdone = nullptr;                                 // end of the variable-mode update
+  hipEvent_t ev_join[2] = {nullptr, nullptr};
+  // Group graphs: the work of one group (both streams) captured once per group shape and flag set
+  // and replayed for every group of that shape.  A replay takes its frame lists from gsel, which
+  // the graph's first kernel fills from the iteration's table (gframes, goffs_dev) at the group
+  // index gsel[0] and then advances.
+  int* goffs_dev = nullptr;  // group offsets of the iteration (n + 1)
+  int* gsel = nullptr;       // [0] next group index, [1, 1 + gmax) this group's frames, then the next group's
+  std::map<uint64_t, hipGraphExec_t> graphs;
+  uint64_t alloc_gen = 0;    // bumped when the group scratch moves (captured graphs hold its addresses)
+  bool use_graphs = true;
 };
+constexpr int MLS_SIDE_FORKS = 4;
 int mls_chunks(int m);  // frame chunks of the per-frame reductions
 // Probe::initEvo: normalisations and the initial illumination (host-synchronous).
 hipError_t mls_init(MlsDev& d, const double* frame_power /* host, n */, hipStream_t s);
@@ -212,6 +239,8 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
 // Partial-sum capacity (doubles) the MLs kernels need beyond the per-group reductions.
 long mls_part_min(int k);
 hipError_t mls_roi(MlsDev& d, hipStream_t s);
+// Destroys the captured group graphs (before the scratch they address moves or is freed).
+void mls_drop_graphs(MlsDev& d);
 
 // ---- learned warm start: weighted average of (amplitude, phase) patches at the ROIs of the
 // selected frames sel[0..n) into obj; pixels whose weight sum is <= rel_thr * max_q

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "ptyk_device.h"
@@ -1228,6 +1229,23 @@ __global__ __launch_bounds__(NT) void k_ortho_step(float2* __restrict__ probe, i
 
 inline dim3 g1(long count) { return dim3((unsigned)((count + NT - 1) / NT)); }
 
+// First kernel of a group graph: this group's frames (sel[1 ..]) and the next group's
+// (sel[1 + gmax ..]) from the iteration's table at the group index sel[0], which then advances.
+__global__ void k_select_group(const int* __restrict__ gframes, const int* __restrict__ goffs, int has_next,
+                               int gmax, int* __restrict__ sel) {
+  const int g = sel[0];
+  const int a = goffs[g], b = goffs[g + 1];
+  for (int i = threadIdx.x; i < b - a; i += blockDim.x) sel[1 + i] = gframes[a + i];
+  if (has_next) {
+    const int c = goffs[g + 2];
+    for (int i = threadIdx.x; i < c - b; i += blockDim.x) sel[1 + gmax + i] = gframes[b + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sel[0] = g + 1;
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }
+
 }  // namespace
 
 int mls_chunks(int m) { return m * m / ppb_of(m); }
@@ -1264,6 +1282,11 @@ hipError_t mls_ortho(MlsDev& d, hipStream_t s) {
     std::swap(ping, pong);
   }
   return hipSuccess;
+}
+
+void mls_drop_graphs(MlsDev& d) {
+  for (auto& kv : d.graphs) hipGraphExecDestroy(kv.second);
+  d.graphs.clear();
 }
 
 hipError_t mls_roi(MlsDev& d, hipStream_t s) {
@@ -1416,10 +1439,31 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
   // rest of group g writes, and they write only spec and vp, which nothing after that point in
   // group g reads
   const bool prefetch = subpix && mod_blocks && d.side;
-  bool vp_ready = false;
-  for (int g = 0; g < n_groups; ++g) {
+  // Second-stream work of a group, on two streams: A takes the position step and then the next
+  // group's illumination probes (which overwrite vp, read by the position step), B the
+  // variable-mode update and then the evolution dots (which read the updated V).  Each piece waits
+  // on its own fork event, recorded on s where the piece's inputs are final, and the illumination
+  // probes also on the end of B's variable-mode update (they read V).  The pieces are enqueued
+  // after the group's main-stream launches, so the host submits those first.
+  struct SideItem {
+    int lane;               // 0: stream A, 1: stream B
+    bool after_vp;          // also waits for the variable-mode update of this group
+    bool signal_vp;         // this is the variable-mode update: its end is recorded
+    std::function<hipError_t(hipStream_t)> work;
+  };
+  SideItem side_work[MLS_SIDE_FORKS];
+  int n_side = 0;
+  auto fork = [&](int lane, bool after_vp, bool signal_vp, std::function<hipError_t(hipStream_t)> work) -> hipError_t {
+    if (n_side >= MLS_SIDE_FORKS) return hipErrorInvalidValue;
+    TRY(hipEventRecord(d.ev_side[n_side], s));
+    side_work[n_side++] = SideItem{lane, after_vp, signal_vp, std::move(work)};
+    return hipSuccess;
+  };
+  // One group's work on both streams; gf and gfn are the frame lists of this group and the next.
+  auto group_body = [&](int g, const int* gf, const int* gfn) -> hipError_t {
     const int G = goffs[g + 1] - goffs[g];
-    const int* gf = d.gframes + goffs[g];
+    const bool vp_ready = prefetch && g > 0;  // formed by the previous group's second-stream work
+    bool vp_on_side = false;                  // the variable-mode update runs on stream B this group
     const long gm = (long)G * mm;
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
@@ -1431,6 +1475,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     const bool pos_1d_step = pos_step && mod_blocks && has_pos_1d(m) && d.side;
     // modes 2..K only feed the fused probe-mode update: their inverse row pass is left to it
     const bool defer_modes = probe_rec && !d.apply_multimodal_update && K > 1 && mod_blocks && d.pum_part;
+    const bool pu_fused = defer_modes && subpix;
     if (subpix && mod_blocks) {
       // the shifted illumination probes' spectra formed in the inverse row pass itself
       if (!vp_ready) {
@@ -1447,20 +1492,21 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       // from O, P and vp; the position step's patch copy takes its own forward transform
       const bool spec2d = pos_step && !pos_1d_step;
       const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
+      // with mode 0 alone in the inverse row pass, that pass also forms the probe update's row shifts
+      const MlsShiftArgs shf{d.pu, d.op, d.sub, -1.f};
       TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
-                             defer_modes ? 1 : K, s, tm));
+                             defer_modes ? 1 : K, pu_fused ? &shf : nullptr, s, tm));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
       if (pos_1d_step) {
         // the position step reads only chi0, vp and op, which stay fixed until the group ends,
         // and writes only its own buffers and the positions: it runs beside the rest of the
         // group (forked here rather than after the step application, which measured 0.8 ms per
         // 51x51 / 256^2 / 5-mode iteration slower)
-        TRY(hipEventRecord(d.ev_fork, s));
-        TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
-        const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, d.side);
-        hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, d.side, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
-        TRY(hipGetLastError());
-        TRY(hipEventRecord(d.ev_join, d.side));
+        TRY(fork(0, false, false, [&d, G, m, gf](hipStream_t st) -> hipError_t {
+          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
+          hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, st, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
+          return hipGetLastError();
+        }));
       }
     } else {
       LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
@@ -1497,8 +1543,12 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       if (probe_rec) {
         if (ll == 1) {
           if (subpix && mod_blocks) {
-            // chi0 conj(op) with its sub-pixel shift in two passes (row shifts, column shifts)
-            TRY(mls_shift_passes(d.pu, chi0, d.op, G, m, d.sub, gf, -1.f, d.tw, s));
+            // chi0 conj(op) with its sub-pixel shift in two passes (row shifts, column shifts); the
+            // row shifts already ran inside the inverse row pass when pu_fused
+            if (pu_fused)
+              TRY(mls_shift_cols(d.pu, G, m, d.sub, gf, -1.f, d.tw, s));
+            else
+              TRY(mls_shift_passes(d.pu, chi0, d.op, G, m, d.sub, gf, -1.f, d.tw, s));
           } else {
             LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
             CHK();
@@ -1515,21 +1565,21 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
             // only V, its own scratch and the coefficients: it runs on the second stream beside
             // the object direction, the LSQ steps and their application
             CHK();
-            TRY(hipEventRecord(d.ev_fork, s));
-            TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
-            const int nb = (mm + NT - 1) / NT;
-            double* p2 = d.vp_part + (long)G * ch;  // |acc|^2 partials
-            double* p3 = p2 + nb;                   // |V|^2 partials
-            hipLaunchKernelGGL(k_vp_resid, gch, dim3(NT), 0, d.side, d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m,
-                               d.cols, d.vmode, d.gscal, d.vp_part);
-            hipLaunchKernelGGL(k_vp_coef, dim3(1), dim3(1024), 0, d.side, d.vp_part, ch, gf, G, mm, d.evo, d.n,
-                               d.vp_ftot, d.gscal);
-            hipLaunchKernelGGL(k_vp_acc, dim3(nb), dim3(NT), 0, d.side, d.acc, d.tmp, G, mm, d.gscal, p2);
-            hipLaunchKernelGGL(k_vp_apply, dim3(nb), dim3(NT), 0, d.side, d.vmode, d.acc, mm, p2, nb,
-                               (double)G / d.n, p3);
-            hipLaunchKernelGGL(k_vp_norm, dim3(nb), dim3(NT), 0, d.side, d.vmode, mm, p3, nb);
-            TRY(hipGetLastError());
-            TRY(hipEventRecord(d.ev_join, d.side));
+            TRY(fork(1, false, true, [&d, G, m, mm, ch, gch, gf](hipStream_t st) -> hipError_t {
+              const int nb = (mm + NT - 1) / NT;
+              double* p2 = d.vp_part + (long)G * ch;  // |acc|^2 partials
+              double* p3 = p2 + nb;                   // |V|^2 partials
+              hipLaunchKernelGGL(k_vp_resid, gch, dim3(NT), 0, st, d.tmp, d.pu, d.pum, d.illum, d.roi, gf, m, d.cols,
+                                 d.vmode, d.gscal, d.vp_part);
+              hipLaunchKernelGGL(k_vp_coef, dim3(1), dim3(1024), 0, st, d.vp_part, ch, gf, G, mm, d.evo, d.n,
+                                 d.vp_ftot, d.gscal);
+              hipLaunchKernelGGL(k_vp_acc, dim3(nb), dim3(NT), 0, st, d.acc, d.tmp, G, mm, d.gscal, p2);
+              hipLaunchKernelGGL(k_vp_apply, dim3(nb), dim3(NT), 0, st, d.vmode, d.acc, mm, p2, nb, (double)G / d.n,
+                                 p3);
+              hipLaunchKernelGGL(k_vp_norm, dim3(nb), dim3(NT), 0, st, d.vmode, mm, p3, nb);
+              return hipGetLastError();
+            }));
+            vp_on_side = true;
           }
         } else {
           LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, chi, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 1);
@@ -1582,14 +1632,13 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
           // the evolution dots need the updated V (second stream, in order) and, with variable
           // intensity, the updated probe (this stream): they follow on the second stream, beside
           // the probe-mode update, which touches neither evo, V nor probe mode 1
-          TRY(hipEventRecord(d.ev_fork, s));
-          TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
-          const float2* page2 = d.variable_intensity ? d.probe : nullptr;
-          hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, d.side, d.op, d.vmode, page2, chi0, m, d.part);
-          hipLaunchKernelGGL(k_evo_update, dim3(1), dim3(1024), 0, d.side, d.part, ch, gf, G, mm, d.evo, d.n,
-                             page2 ? 2 : 1, d.ftot);
-          TRY(hipGetLastError());
-          TRY(hipEventRecord(d.ev_join, d.side));
+          TRY(fork(1, false, false, [&d, G, m, mm, ch, gch, gf, chi0](hipStream_t st) -> hipError_t {
+            const float2* page2 = d.variable_intensity ? d.probe : nullptr;
+            hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, st, d.op, d.vmode, page2, chi0, m, d.part);
+            hipLaunchKernelGGL(k_evo_update, dim3(1), dim3(1024), 0, st, d.part, ch, gf, G, mm, d.evo, d.n,
+                               page2 ? 2 : 1, d.ftot);
+            return hipGetLastError();
+          }));
         } else if (d.variable_probe) {
           {
             const int nb = (mm + NT - 1) / NT;
@@ -1616,17 +1665,13 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         }
       }
     }
-    vp_ready = false;
     if (prefetch && g + 1 < n_groups) {
       const int Gn = goffs[g + 2] - goffs[g + 1];
-      const int* gfn = d.gframes + goffs[g + 1];
-      TRY(hipEventRecord(d.ev_fork, s));
-      TRY(hipStreamWaitEvent(d.side, d.ev_fork, 0));
-      TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, d.side));
-      TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gfn, d.n, var_any ? 1 : 0, Gn, m, d.tw, d.side));
-      TRY(fft_cols_batch(d.vp, Gn, m, true, d.tw, d.side));
-      TRY(hipEventRecord(d.ev_join, d.side));
-      vp_ready = true;
+      TRY(fork(0, vp_on_side, false, [&d, m, Gn, gfn, var_any](hipStream_t st) -> hipError_t {
+        TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, st));
+        TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gfn, d.n, var_any ? 1 : 0, Gn, m, d.tw, st));
+        return fft_cols_batch(d.vp, Gn, m, true, d.tw, st);
+      }));
     }
     MARK(MLS_T_MODES);
     if (defer_modes) {
@@ -1637,8 +1682,60 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
              d.cols, fw, d.gscal);
       CHK();
     }
+    // the group's second-stream work in fork order, then the joins before the next group's writes
+    bool used[2] = {false, false};
+    for (int i = 0; i < n_side; ++i) {
+      SideItem& w = side_work[i];
+      hipStream_t st = w.lane ? d.side2 : d.side;
+      TRY(hipStreamWaitEvent(st, d.ev_side[i], 0));
+      if (w.after_vp) TRY(hipStreamWaitEvent(st, d.ev_vpdone, 0));
+      TRY(w.work(st));
+      if (w.signal_vp) TRY(hipEventRecord(d.ev_vpdone, st));
+      used[w.lane] = true;
+    }
     MARK(MLS_T_JOIN);
-    if (pos_1d_step || vp_side || vp_ready) TRY(hipStreamWaitEvent(s, d.ev_join, 0));  // before the next group's writes
+    for (int l = 0; l < 2; ++l)
+      if (used[l]) {
+        TRY(hipEventRecord(d.ev_join[l], l ? d.side2 : d.side));
+        TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
+      }
+    n_side = 0;
+    return hipSuccess;
+  };
+  // Group graphs (not under the launch timer): a group's shape and the iteration's flags key
+  // one captured graph, replayed by one launch per group.
+  const bool graphs = d.use_graphs && !tm && d.side && d.gsel && d.goffs_dev;
+  if (graphs) LAUNCH(k_set_int, dim3(1), dim3(1), d.gsel, 0);
+  for (int g = 0; g < n_groups; ++g) {
+    if (!graphs) {
+      TRY(group_body(g, d.gframes + goffs[g], g + 1 < n_groups ? d.gframes + goffs[g + 1] : nullptr));
+      continue;
+    }
+    const int G = goffs[g + 1] - goffs[g], Gn = g + 1 < n_groups ? goffs[g + 2] - goffs[g + 1] : 0;
+    if (G >= 4096 || Gn >= 4096) return hipErrorInvalidValue;
+    const uint64_t key = (uint64_t)G | (uint64_t)Gn << 12 | (uint64_t)(g > 0) << 24 |
+                         (uint64_t)(iter >= d.pps) << 25 | (uint64_t)var_update << 26 | (uint64_t)probe_rec << 27 |
+                         (uint64_t)obj_rec_iter << 28 | (uint64_t)(iter < 10) << 29 | (uint64_t)d.mod_cb << 30 |
+                         (uint64_t)d.pum_fpc << 36;
+    auto it = d.graphs.find(key);
+    if (it == d.graphs.end()) {
+      if (d.graphs.size() >= 64) mls_drop_graphs(d);
+      hipGraph_t graph = nullptr;
+      TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      hipError_t e = hipSuccess;
+      hipLaunchKernelGGL(k_select_group, dim3(1), dim3(64), 0, s, d.gframes, d.goffs_dev, Gn > 0 ? 1 : 0, d.gmax,
+                         d.gsel);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = group_body(g, d.gsel + 1, d.gsel + 1 + d.gmax);
+      const hipError_t e2 = hipStreamEndCapture(s, &graph);
+      if (e == hipSuccess) e = e2;
+      hipGraphExec_t exec = nullptr;
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) hipGraphDestroy(graph);
+      if (e != hipSuccess) return e;
+      it = d.graphs.emplace(key, exec).first;
+    }
+    TRY(hipGraphLaunch(it->second, s));
   }
   MARK(MLS_T_ITER);
   if (K > d.n_rec) TRY(mls_ortho(d, s));
@@ -1648,10 +1745,13 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
 
 hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hipStream_t s, MlsTimer* tm) {
   const hipError_t e = mls_iteration_body(d, iter, goffs, n_groups, s, tm);
-  // an error between a fork and its join leaves work on the second stream that still reads the
+  // an error between a fork and its join leaves work on the side streams that still reads the
   // group buffers and writes the positions, V and the evolution: it finishes before the error is
   // reported, so that no later call overwrites what it reads
-  if (e != hipSuccess && d.side) (void)hipStreamSynchronize(d.side);
+  if (e != hipSuccess) {
+    if (d.side) (void)hipStreamSynchronize(d.side);
+    if (d.side2) (void)hipStreamSynchronize(d.side2);
+  }
   return e;
 }
 

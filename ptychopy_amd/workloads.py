@@ -129,10 +129,12 @@ def mls_kind_bytes(kind: str, m: int, k: int, n: int, groups: int) -> float:
     storage precision, for the three-pass form with probe modes 2..K deferred to their update:
     exit_row_pass reads the object patch and mode 0's illumination probe (8 + 8 B/px) and writes
     K exit-wave planes and the patch (8 K + 8); modulus_column_pass reads and writes the K planes
-    and reads FP16 A (16 K + 2); inverse_row_pass reads and writes mode 0's plane (16); probe_modes
-    reads planes 2..K, the patch and the illumination at the ROI (8 (K - 1) + 8 + 4)."""
+    and reads FP16 A (16 K + 2); inverse_row_pass reads mode 0's plane and the patch and writes
+    chi_0 and the row-shifted probe update (32; it forms the probe update's row shifts while the rows
+    are in LDS); probe_modes reads planes 2..K, the patch and the illumination at the ROI
+    (8 (K - 1) + 8 + 4)."""
     px = float(n) * m * m
-    per_px = {"exit_row_pass": 8.0 * k + 24.0, "modulus_column_pass": 16.0 * k + 2.0, "inverse_row_pass": 16.0,
+    per_px = {"exit_row_pass": 8.0 * k + 24.0, "modulus_column_pass": 16.0 * k + 2.0, "inverse_row_pass": 32.0,
               "probe_modes": 8.0 * (k - 1) + 12.0}
     if kind not in per_px:
         return 0.0
