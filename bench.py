@@ -113,19 +113,29 @@ def cpu_baseline(wl: dict, frames: int, min_seconds: float = 10.0) -> dict:
                       f"frames, 512x512, same 1027x1027 canvas; {dt:.1f} s on {cpu_model()}", **host_info()}
 
 
-def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: int) -> dict:
-    """FP64 oracle MLs iteration `it` restricted to the first two groups of the same workload."""
+def mls_cpu_baseline(W: dict, amp_deq, probe0, obj0, vmode, evo1, groups, it: int,
+                     min_seconds: float = 10.0) -> dict:
+    """FP64 oracle MLs iteration `it` over the first groups of the same workload: two groups time
+    the rate, then a fresh state runs as many groups as take about min_seconds."""
     import oracle
-    st = oracle.MlsState(obj0, probe0, vmode, evo1, W["pos"], amp_deq)
-    st.init((amp_deq.astype(np.float64) ** 2).sum(axis=(1, 2)))
+    a64 = amp_deq.astype(np.float64)
+    fp = (a64 ** 2).sum(axis=(1, 2))
     prm = oracle.mls_params(W["box"], iter=it)
-    sample = groups[:2]
+
+    def run(sample):
+        st = oracle.MlsState(obj0, probe0, vmode, evo1, W["pos"], a64)
+        st.init(fp)
+        t0 = time.perf_counter()
+        st.iteration(prm, sample)
+        return time.perf_counter() - t0
+
+    dt2 = run(groups[:2])
+    ng = int(min(len(groups), max(2, round(min_seconds / max(dt2 / 2, 1e-3)))))
+    sample = groups[:ng]
     frames = int(sum(len(g) for g in sample))
-    t0 = time.perf_counter()
-    st.iteration(prm, sample)
-    dt = time.perf_counter() - t0
+    dt = run(sample)
     return {"value": frames / dt, "unit": "frames/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"FP64 oracle MLs iteration {it} over its first 2 groups ({frames} of {W['n']} frames; "
+            "sample": f"FP64 oracle MLs iteration {it} over its first {ng} groups ({frames} of {W['n']} frames; "
                       f"whole-canvas steps included), 256x256, 5 modes; {dt:.1f} s on {cpu_model()}",
             **host_info()}
 

@@ -140,8 +140,8 @@ int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
  * PTYK_TUNE_MLS_MOD_CB: column width (8, 16 or 32; default 16) of the 64-row blocks of the MLs
  * modulus pass for M in {128, 256, 512}.
  * PTYK_TUNE_MLS_PUM_FPC: frames per chunk (2..64; default 4) of the MLs probe-mode update.
- * PTYK_TUNE_MLS_GRAPHS: 1 (default) replays each MLs group from a HIP graph captured once per
- * group shape and flag set (one launch per group); 0 launches every kernel from the host. */
+ * PTYK_TUNE_MLS_GRAPHS: 1 replays each MLs group from a HIP graph captured once per group shape
+ * and flag set (one launch per group); 0 (default) launches every kernel from the host. */
 enum { PTYK_TUNE_DM_WAVE = 1, PTYK_TUNE_MLS_MOD_CB = 2, PTYK_TUNE_MLS_PUM_FPC = 3, PTYK_TUNE_MLS_GRAPHS = 4 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
 int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);

@@ -52,7 +52,7 @@ struct ptyk_ctx {
   // wave-scheduled DM
   int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
   int mls_mod_cb = 16, mls_pum_fpc = 4;  // tuning: MLs modulus block width, probe-mode chunk frames
-  int mls_graphs = 1;                    // tuning: MLs groups replayed from captured graphs
+  int mls_graphs = 0;                    // tuning: MLs groups replayed from captured graphs (off: slower here)
   int dm_wave = 0, dm_waves = 0;
   double2* d_obj = nullptr;
   double2* d_probe = nullptr;

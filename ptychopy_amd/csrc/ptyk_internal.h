@@ -227,7 +227,7 @@ struct MlsDev {
   int* gsel = nullptr;       // [0] next group index, [1, 1 + gmax) this group's frames, then the next group's
   std::map<uint64_t, hipGraphExec_t> graphs;
   uint64_t alloc_gen = 0;    // bumped when the group scratch moves (captured graphs hold its addresses)
-  bool use_graphs = true;
+  bool use_graphs = false;
 };
 constexpr int MLS_SIDE_FORKS = 4;
 int mls_chunks(int m);  // frame chunks of the per-frame reductions
