@@ -407,6 +407,7 @@ def main() -> int:
     ap.add_argument("--mls-shape", default=None,
                     help="MLs launch shape 'CB,FPC,GRAPHS' (modulus block width, probe-mode chunk frames, "
                          "group graphs 0/1); default: the autotuner's choice")
+    ap.add_argument("--dm-graph", type=int, default=0, help="dm_c1: replay iterations from captured HIP graphs")
     ap.add_argument("--dm-wave", type=int, default=None,
                     help="frames per DM wave (default: the autotuner's choice; 0 automatic, -1 one pass)")
     args = ap.parse_args()
@@ -475,6 +476,9 @@ def main() -> int:
         args.dm_wave, tune_ms = autotune.tune_dm_wave(wl["pos"], m, (rows, cols))
     e = Engine(n, m, (rows, cols))
     e.set_tuning(TUNE_DM_WAVE, args.dm_wave)
+    if args.dm_graph:
+        from ptychopy_amd._engine import TUNE_DM_GRAPH
+        e.set_tuning(TUNE_DM_GRAPH, 1)
     e.set_frames(wl["amp"])
     e.set_positions(wl["pos"])
     e.clear_object()
@@ -539,7 +543,7 @@ def main() -> int:
                                "frac": it_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: float(v) for k, v in zip(KERNELS, kt)},
         "dm_wave_frames": e.get_tuning(1),
-        "tuning": {"dm_wave": args.dm_wave, "table_ms": tune_ms},
+        "tuning": {"dm_wave": args.dm_wave, "dm_graph": args.dm_graph, "table_ms": tune_ms},
         "setup_s": t_gen,
     }
     if not args.no_cpu:

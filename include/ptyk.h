@@ -141,8 +141,12 @@ int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
  * modulus pass for M in {128, 256, 512}.
  * PTYK_TUNE_MLS_PUM_FPC: frames per chunk (2..64; default 4) of the MLs probe-mode update.
  * PTYK_TUNE_MLS_GRAPHS: 1 replays each MLs group from a HIP graph captured once per group shape
- * and flag set (one launch per group); 0 (default) launches every kernel from the host. */
-enum { PTYK_TUNE_DM_WAVE = 1, PTYK_TUNE_MLS_MOD_CB = 2, PTYK_TUNE_MLS_PUM_FPC = 3, PTYK_TUNE_MLS_GRAPHS = 4 };
+ * and flag set (one launch per group); 0 (default) launches every kernel from the host.
+ * PTYK_TUNE_DM_GRAPH: 1 replays every DM iteration after the first from a HIP graph captured once
+ * per flag set on the handle's stream (re-captured when the schedule changes); 0 (default)
+ * launches the iteration's kernels from the host. */
+enum { PTYK_TUNE_DM_WAVE = 1, PTYK_TUNE_MLS_MOD_CB = 2, PTYK_TUNE_MLS_PUM_FPC = 3, PTYK_TUNE_MLS_GRAPHS = 4,
+       PTYK_TUNE_DM_GRAPH = 5 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
 int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);
 

@@ -53,6 +53,9 @@ struct ptyk_ctx {
   int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
   int mls_mod_cb = 16, mls_pum_fpc = 4;  // tuning: MLs modulus block width, probe-mode chunk frames
   int mls_graphs = 0;                    // tuning: MLs groups replayed from captured graphs (off: slower here)
+  int dm_graph = 0;                      // tuning: DM iterations replayed from captured graphs
+  std::vector<std::pair<uint32_t, hipGraphExec_t>> dm_graphs;  // one per flag set (dm_graph)
+  std::vector<uintptr_t> dm_graph_sig;   // the schedule they were captured with
   int dm_wave = 0, dm_waves = 0;
   double2* d_obj = nullptr;
   double2* d_probe = nullptr;
@@ -131,7 +134,13 @@ static void free_mls(ptyk_ctx* c) {
   c->mls_ready = false;
 }
 
+static void drop_dm_graphs(ptyk_ctx* c) {
+  for (auto& g : c->dm_graphs) hipGraphExecDestroy(g.second);
+  c->dm_graphs.clear();
+}
+
 static void free_all(ptyk_ctx* c) {
+  drop_dm_graphs(c);
   free_mls(c);
   for (void* p : {(void*)c->truth, (void*)c->cov, (void*)c->pint, (void*)c->epart})
     if (p) hipFree(p);
@@ -770,14 +779,44 @@ static int dm_launch(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   return PTYK_OK;
 }
 
+static std::vector<uintptr_t> dm_signature(const ptyk_ctx* c);
+
+// With the DM-graph knob, an iteration after the first replays a graph captured once per flag set
+// on the handle's stream (re-captured when the schedule's buffers or frame lists change: the
+// signature holds their addresses and the table generation counter).
 static int dm_run(ptyk_ctx* c, uint32_t flags, float* kernel_ms) {
   if (!c) return PTYK_EINVAL;
   if (!c->have_frames || !c->have_pos || !c->have_probe)
     return fail(c, PTYK_ESTATE, "frames, positions and probe must be set");
   hipSetDevice(c->cfg.device);
+  const bool started = c->dm_started && c->psi;  // the first call sets host-side state eagerly
   int r = dm_prepare(c);
   if (r) return r;
-  return dm_launch(c, flags, kernel_ms);
+  if (!c->dm_graph || kernel_ms || !started) return dm_launch(c, flags, kernel_ms);
+  const std::vector<uintptr_t> sig = dm_signature(c);
+  if (sig != c->dm_graph_sig) {
+    drop_dm_graphs(c);
+    c->dm_graph_sig = sig;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : c->dm_graphs)
+    if (g.first == flags) exec = g.second;
+  if (!exec) {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    r = dm_launch(c, flags, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (r != PTYK_OK || ec != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return r != PTYK_OK ? r : fail(c, PTYK_EDEVICE, "DM graph capture failed");
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, PTYK_EDEVICE, std::string("DM graph instantiate: ") + hipGetErrorString(ei));
+    c->dm_graphs.emplace_back(flags, exec);
+  }
+  HIPCHK(c, hipGraphLaunch(exec, c->stream));
+  return PTYK_OK;
 }
 
 int ptyk_dm_iteration(ptyk_ctx* c, uint32_t flags) { return dm_run(c, flags, nullptr); }
@@ -810,6 +849,12 @@ int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
     c->mls_graphs = value;
     return PTYK_OK;
   }
+  if (key == PTYK_TUNE_DM_GRAPH) {
+    if (value != 0 && value != 1) return fail(c, PTYK_EINVAL, "DM graphs: 0 or 1");
+    c->dm_graph = value;
+    if (!value) drop_dm_graphs(c);
+    return PTYK_OK;
+  }
   return fail(c, PTYK_EINVAL, "unknown tuning key");
 }
 
@@ -817,6 +862,10 @@ int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
   if (!c || !value) return PTYK_EINVAL;
   if (key == PTYK_TUNE_DM_WAVE) {
     *value = dm_wave_frames(c);
+    return PTYK_OK;
+  }
+  if (key == PTYK_TUNE_DM_GRAPH) {
+    *value = c->dm_graph;
     return PTYK_OK;
   }
   if (key == PTYK_TUNE_MLS_MOD_CB || key == PTYK_TUNE_MLS_PUM_FPC || key == PTYK_TUNE_MLS_GRAPHS) {

@@ -125,3 +125,32 @@ def test_batch_replays_after_positions_move(gpu):
         assert np.array_equal(e2.get_object(), ref_moved[0]) and np.array_equal(e2.get_psi(), ref_moved[1])
     e1.close()
     e2.close()
+
+
+@pytest.mark.parametrize("m,wave", [(128, 0), (128, -1), (32, 0)])
+def test_standalone_dm_graph_replays_equal_direct_launches(gpu, m, wave):
+    """Standalone DM iterations replayed from graphs captured on the handle's stream (one per flag
+    set: phase constraint, plain, probe update; re-captured after the positions move, which
+    rebuilds the frame lists) equal direct launches bit for bit, with host reads of the state
+    between the replays."""
+    from ptychopy_amd._engine import TUNE_DM_GRAPH, TUNE_DM_WAVE, FLAG_PHASE_CONSTRAINT, FLAG_UPDATE_PROBE
+    P = make_problem(m=m, n_side=4, seed=41)
+    moved = P["pos"] + np.float32(1.0)
+    assert (np.ceil(moved.max(axis=0)) + m <= np.array(P["shape"])).all()
+    runs = []
+    for graph in (1, 0):
+        e = _engine(P)
+        e.set_tuning(TUNE_DM_WAVE, wave)
+        e.set_tuning(TUNE_DM_GRAPH, graph)
+        assert e.get_tuning(TUNE_DM_GRAPH) == graph
+        snaps = []
+        for i in range(1, 10):
+            if i == 6:
+                e.set_positions(moved)
+            e.dm_iteration((FLAG_PHASE_CONSTRAINT if i <= 1 else 0) | (FLAG_UPDATE_PROBE if i >= 4 else 0))
+            snaps.append(e.get_object())
+        runs.append((snaps, e.get_probe(), e.get_psi()))
+        e.close()
+    for i, (a, b) in enumerate(zip(runs[0][0], runs[1][0])):
+        assert np.array_equal(a, b), f"object after iteration {i + 1}"
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
