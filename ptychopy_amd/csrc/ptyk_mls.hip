@@ -1445,18 +1445,24 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
   // on its own fork event, recorded on s where the piece's inputs are final, and the illumination
   // probes also on the end of B's variable-mode update (they read V).  The pieces are enqueued
   // after the group's main-stream launches, so the host submits those first.
+  // Pieces whose inputs become final at the same point share one fork event (each event record is
+  // a marker between two main-stream kernels, a few microseconds of the main stream each).
   struct SideItem {
     int lane;               // 0: stream A, 1: stream B
+    int point;              // its fork event, ev_side[point]
     bool after_vp;          // also waits for the variable-mode update of this group
     bool signal_vp;         // this is the variable-mode update: its end is recorded
     std::function<hipError_t(hipStream_t)> work;
   };
   SideItem side_work[MLS_SIDE_FORKS];
-  int n_side = 0;
+  int n_side = 0, n_points = 0;
+  auto fork_point = [&]() -> hipError_t {
+    if (n_points >= MLS_SIDE_FORKS) return hipErrorInvalidValue;
+    return hipEventRecord(d.ev_side[n_points++], s);
+  };
   auto fork = [&](int lane, bool after_vp, bool signal_vp, std::function<hipError_t(hipStream_t)> work) -> hipError_t {
-    if (n_side >= MLS_SIDE_FORKS) return hipErrorInvalidValue;
-    TRY(hipEventRecord(d.ev_side[n_side], s));
-    side_work[n_side++] = SideItem{lane, after_vp, signal_vp, std::move(work)};
+    if (n_side >= MLS_SIDE_FORKS || n_points == 0) return hipErrorInvalidValue;
+    side_work[n_side++] = SideItem{lane, n_points - 1, after_vp, signal_vp, std::move(work)};
     return hipSuccess;
   };
   // One group's work on both streams; gf and gfn are the frame lists of this group and the next.
@@ -1464,6 +1470,8 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     const int G = goffs[g + 1] - goffs[g];
     const bool vp_ready = prefetch && g > 0;  // formed by the previous group's second-stream work
     bool vp_on_side = false;                  // the variable-mode update runs on stream B this group
+    // side pieces waiting for their fork point
+    std::function<hipError_t(hipStream_t)> pos_work, vp_work, evo_work;
     const long gm = (long)G * mm;
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
@@ -1502,11 +1510,11 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         // and writes only its own buffers and the positions: it runs beside the rest of the
         // group (forked here rather than after the step application, which measured 0.8 ms per
         // 51x51 / 256^2 / 5-mode iteration slower)
-        TRY(fork(0, false, false, [&d, G, m, gf](hipStream_t st) -> hipError_t {
+        pos_work = [&d, G, m, gf](hipStream_t st) -> hipError_t {
           const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
           hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, st, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
           return hipGetLastError();
-        }));
+        };
       }
     } else {
       LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
@@ -1565,7 +1573,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
             // only V, its own scratch and the coefficients: it runs on the second stream beside
             // the object direction, the LSQ steps and their application
             CHK();
-            TRY(fork(1, false, true, [&d, G, m, mm, ch, gch, gf](hipStream_t st) -> hipError_t {
+            vp_work = [&d, G, m, mm, ch, gch, gf](hipStream_t st) -> hipError_t {
               const int nb = (mm + NT - 1) / NT;
               double* p2 = d.vp_part + (long)G * ch;  // |acc|^2 partials
               double* p3 = p2 + nb;                   // |V|^2 partials
@@ -1578,7 +1586,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
                                  p3);
               hipLaunchKernelGGL(k_vp_norm, dim3(nb), dim3(NT), 0, st, d.vmode, mm, p3, nb);
               return hipGetLastError();
-            }));
+            };
             vp_on_side = true;
           }
         } else {
@@ -1587,6 +1595,13 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         CHK();
       }
       const float2* pl = llp == 1 ? nullptr : d.probe + (long)(llp - 1) * mm;
+      if (ll == 1 && (pos_work || vp_work)) {
+        // the position step (its inputs final since the inverse row pass) and the variable-mode
+        // update (since the probe-update average) fork here, at one point
+        TRY(fork_point());
+        if (pos_work) TRY(fork(0, false, false, std::move(pos_work)));
+        if (vp_work) TRY(fork(1, false, true, std::move(vp_work)));
+      }
       MARK(MLS_T_PRECOND);
       if (obj_rec) {
         const int scatter = d.delta_p != 0 ? 1 : 0;
@@ -1632,13 +1647,13 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
           // the evolution dots need the updated V (second stream, in order) and, with variable
           // intensity, the updated probe (this stream): they follow on the second stream, beside
           // the probe-mode update, which touches neither evo, V nor probe mode 1
-          TRY(fork(1, false, false, [&d, G, m, mm, ch, gch, gf, chi0](hipStream_t st) -> hipError_t {
+          evo_work = [&d, G, m, mm, ch, gch, gf, chi0](hipStream_t st) -> hipError_t {
             const float2* page2 = d.variable_intensity ? d.probe : nullptr;
             hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, st, d.op, d.vmode, page2, chi0, m, d.part);
             hipLaunchKernelGGL(k_evo_update, dim3(1), dim3(1024), 0, st, d.part, ch, gf, G, mm, d.evo, d.n,
                                page2 ? 2 : 1, d.ftot);
             return hipGetLastError();
-          }));
+          };
         } else if (d.variable_probe) {
           {
             const int nb = (mm + NT - 1) / NT;
@@ -1665,7 +1680,12 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         }
       }
     }
-    if (prefetch && g + 1 < n_groups) {
+    // the evolution dots (inputs final since the step application; nothing runs between it and
+    // here on this stream) and the next group's illumination probes fork at one point
+    const bool pre_side = prefetch && g + 1 < n_groups;
+    if (evo_work || pre_side) TRY(fork_point());
+    if (evo_work) TRY(fork(1, false, false, std::move(evo_work)));
+    if (pre_side) {
       const int Gn = goffs[g + 2] - goffs[g + 1];
       TRY(fork(0, vp_on_side, false, [&d, m, Gn, gfn, var_any](hipStream_t st) -> hipError_t {
         TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, st));
@@ -1687,19 +1707,24 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     for (int i = 0; i < n_side; ++i) {
       SideItem& w = side_work[i];
       hipStream_t st = w.lane ? d.side2 : d.side;
-      TRY(hipStreamWaitEvent(st, d.ev_side[i], 0));
+      TRY(hipStreamWaitEvent(st, d.ev_side[w.point], 0));
       if (w.after_vp) TRY(hipStreamWaitEvent(st, d.ev_vpdone, 0));
       TRY(w.work(st));
       if (w.signal_vp) TRY(hipEventRecord(d.ev_vpdone, st));
       used[w.lane] = true;
     }
     MARK(MLS_T_JOIN);
-    for (int l = 0; l < 2; ++l)
-      if (used[l]) {
-        TRY(hipEventRecord(d.ev_join[l], l ? d.side2 : d.side));
-        TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
-      }
-    n_side = 0;
+    // one join on this stream: with both side streams in use, stream A first waits for B's end
+    if (used[0] && used[1]) {
+      TRY(hipEventRecord(d.ev_join[1], d.side2));
+      TRY(hipStreamWaitEvent(d.side, d.ev_join[1], 0));
+    }
+    if (used[0] || used[1]) {
+      const int l = used[0] ? 0 : 1;
+      TRY(hipEventRecord(d.ev_join[l], l ? d.side2 : d.side));
+      TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
+    }
+    n_side = n_points = 0;
     return hipSuccess;
   };
   // Group graphs (not under the launch timer): a group's shape and the iteration's flags key
