@@ -451,6 +451,11 @@ static hipError_t wave_impl(const DmWaveArgs& a, hipStream_t s, DmTimer* tm) {
 // over 64 x 16 blocks that holds every mode of a block, so the modulus, which needs
 // sum_k |Psi_k|^2 at each pixel, runs between the forward and inverse 64-point column transforms.
 
+// Frequency of FFT index i in cycles per sample, in [-0.5, 0.5) (the MLs spectral shift's).
+__device__ __forceinline__ float shift_freq(int i, int m) {
+  return (float)((i + (m >> 1)) & (m - 1)) / (float)m - 0.5f;
+}
+
 // Row pass of the MLs exit waves formed in place of their load (MLS.cpp:119-126 with
 // d_readFromArray_c, utilitiesKernels.cu:1064-1084): plane k G + j, row r holds
 // psi_k = P_k O_r, or O_r vp_j for mode 0 with the sub-pixel shift; mode 0's workgroups also
@@ -511,11 +516,45 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_a(float2* __restrict__ psi
     }
   }
   row_pass_forward<M>(lds, psi, 0, n_groups, tw, tid);
-}
-
-// Frequency of FFT index i in cycles per sample, in [-0.5, 0.5) (the MLs spectral shift's).
-__device__ __forceinline__ float shift_freq(int i, int m) {
-  return (float)((i + (m >> 1)) & (m - 1)) / (float)m - 0.5f;
+  if (k == 0 && x.dx) {
+    // the column derivative of the patch rows this workgroup wrote (read back by the threads
+    // that wrote them): row transform, 2 pi i f_col / M, inverse row transform
+    __syncthreads();  // the exit-wave stores have read the tile
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      long f;
+      int r, c;
+      IO::locate(e, 0, n_groups, &f, &r, &c);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) =
+          *reinterpret_cast<const float4*>(x.op + (long)j * mm + r * M + c);
+    }
+    __syncthreads();
+    fft_lds<M, false, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+    const float inv = 1.f / (float)M;
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      const int c = e % M;
+      float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, c)]);
+      const float4 t = *p;
+      const float s0 = shift_freq(c, M) * inv, s1 = shift_freq(c + 1, M) * inv;
+      const float2 a = cscale(make_float2(-t.y * 6.283185307179586f, t.x * 6.283185307179586f), s0);
+      const float2 b = cscale(make_float2(-t.w * 6.283185307179586f, t.z * 6.283185307179586f), s1);
+      *p = make_float4(a.x, a.y, b.x, b.y);
+    }
+    __syncthreads();
+    fft_lds<M, true, C::ROWS, PTYK_NT, L, false>(lds, tw, tid);
+#pragma unroll
+    for (int i = 0; i < IO::PAIRS; ++i) {
+      const int e = 2 * (tid + i * PTYK_NT);
+      long f;
+      int r, c;
+      IO::locate(e, 0, n_groups, &f, &r, &c);
+      *reinterpret_cast<float4*>(x.dx + (long)j * mm + r * M + c) =
+          *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+    }
+  }
 }
 
 // The MLs sub-pixel shift x <- IFFT2(FFT2(x) exp(-2 pi i sign (sub.x f[col] + sub.y f[row]))) / M^2

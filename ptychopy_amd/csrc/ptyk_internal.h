@@ -106,6 +106,9 @@ struct MlsExitArgs {
   int subpix;
   float2* op;
   float2* op2;  // nullable
+  // nullable: mode 0's workgroups also write the column derivative of each object-patch row,
+  // dX = IFFT_row(FFT_row(O_r) 2 pi i f_col) / M (the position step's first pass)
+  float2* dx;
 };
 int mls_modulus_blocks(int m, int k, int cb);
 // MLs launch categories of ptyk_mls_iteration_timed (main stream only; the second stream's

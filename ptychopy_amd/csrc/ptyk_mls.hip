@@ -933,10 +933,11 @@ __global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__
 // Position-step partials of G frames from the object patches op; dX goes to dx.  Returns the
 // chunks per frame, or 0 when m has no one-dimensional form (the caller then takes the 2D one).
 static int pos_1d(const float2* op, float2* dx, const float2* vp, const float2* chi, int G, int m, const float2* tw,
-                  double* part, hipStream_t s) {
+                  double* part, hipStream_t s, bool dx_ready = false) {
   auto go = [&](auto rows_kern, auto cols_kern, int nb, int nc) {
     const long rows = (long)G * m;
-    hipLaunchKernelGGL(rows_kern, dim3((unsigned)((rows + nb - 1) / nb)), dim3(PTYK_NT), 0, s, op, dx, rows, tw);
+    if (!dx_ready)
+      hipLaunchKernelGGL(rows_kern, dim3((unsigned)((rows + nb - 1) / nb)), dim3(PTYK_NT), 0, s, op, dx, rows, tw);
     hipLaunchKernelGGL(cols_kern, dim3(m / nc, G), dim3(PTYK_NT), 0, s, op, dx, vp, chi, tw, part);
     return m / nc;
   };
@@ -1499,7 +1500,9 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       // three passes for the exit waves, the first of which forms them (and the object patches)
       // from O, P and vp; the position step's patch copy takes its own forward transform
       const bool spec2d = pos_step && !pos_1d_step;
-      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr};
+      // with the one-dimensional position step, mode 0's exit-wave workgroups also form its dX rows
+      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr,
+                           pos_1d_step ? d.dxb : nullptr};
       // with mode 0 alone in the inverse row pass, that pass also forms the probe update's row shifts
       const MlsShiftArgs shf{d.pu, d.op, d.sub, -1.f};
       TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
@@ -1511,7 +1514,8 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         // group (forked here rather than after the step application, which measured 0.8 ms per
         // 51x51 / 256^2 / 5-mode iteration slower)
         pos_work = [&d, G, m, gf](hipStream_t st) -> hipError_t {
-          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
+          // dX came with the exit waves (k_mls_pass_a)
+          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st, true);
           hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, st, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
           return hipGetLastError();
         };
