@@ -799,60 +799,13 @@ __global__ __launch_bounds__(NT) void k_pos_partial(const float2* __restrict__ d
 // (Probe.cpp:956-1118) have multipliers that depend on one frequency index only, so the
 // transforms along the other axis cancel (IFFT o FFT = M along an axis):
 // dX = IFFT_rows(FFT_rows(O_r) 2 pi i f_col) / M and dY = IFFT_cols(FFT_cols(O_r) 2 pi i f_row) / M.
-// k_dx_rows writes dX; k_pos_cols forms dY of whole 16-column strips in LDS and sums the four
-// products with vp and chi over them, so dY is never stored.
+// The exit-wave row pass (k_mls_pass_a) writes dX while the patch rows are in its LDS; k_pos_cols
+// forms dY of whole 16-column strips in LDS and sums the four products with vp and chi over
+// them, so dY is never stored.
 
 // t 2 pi i s for a spectrum value t and a real scale s
 __device__ __forceinline__ float2 deriv_of(float2 t, float sc) {
   return cscale(make_float2(-t.y * kTwoPi, t.x * kTwoPi), sc);
-}
-
-template <int M>
-__global__ __launch_bounds__(PTYK_NT) void k_dx_rows(const float2* __restrict__ op, float2* __restrict__ dx, long rows,
-                                                     const float2* __restrict__ tw) {
-  constexpr int NB = plan_min_batch<M, PTYK_NT>();
-  constexpr int PAIRS = NB * M / 2 / PTYK_NT;
-  static_assert((NB * M / 2) % PTYK_NT == 0, "row tile must divide among the threads");
-  using L = SwzRowLayout<M, M>;
-  __shared__ float2 lds[NB * M];
-  const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * NB;
-  {
-    float4 v[PAIRS];
-#pragma unroll
-    for (int i = 0; i < PAIRS; ++i) {
-      const int e = 2 * (tid + i * PTYK_NT);
-      const long g = row0 + e / M;
-      v[i] = g < rows ? *reinterpret_cast<const float4*>(op + g * M + (e % M)) : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int i = 0; i < PAIRS; ++i) {
-      const int e = 2 * (tid + i * PTYK_NT);
-      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = v[i];
-    }
-  }
-  __syncthreads();
-  fft_lds<M, false, NB, PTYK_NT, L, false>(lds, tw, tid);
-  const float inv = 1.f / (float)M;
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const int c = e % M;
-    float4* p = reinterpret_cast<float4*>(&lds[L::addr(e / M, c)]);
-    const float4 t = *p;
-    const float2 a = deriv_of(make_float2(t.x, t.y), freq(c, M) * inv);
-    const float2 b = deriv_of(make_float2(t.z, t.w), freq(c + 1, M) * inv);
-    *p = make_float4(a.x, a.y, b.x, b.y);
-  }
-  __syncthreads();
-  fft_lds<M, true, NB, PTYK_NT, L, false>(lds, tw, tid);
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * PTYK_NT);
-    const long g = row0 + e / M;
-    if (g < rows)
-      *reinterpret_cast<float4*>(dx + g * M + (e % M)) = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
-  }
 }
 
 // Partials part[(j * gridDim.x + blockIdx.x) * 4 + v] of [sum |dX vp|^2, sum Re(chi conj(dX vp)),
@@ -930,21 +883,17 @@ __global__ __launch_bounds__(PTYK_NT) void k_pos_cols(const float2* __restrict__
   }
 }
 
-// Position-step partials of G frames from the object patches op; dX goes to dx.  Returns the
-// chunks per frame, or 0 when m has no one-dimensional form (the caller then takes the 2D one).
-static int pos_1d(const float2* op, float2* dx, const float2* vp, const float2* chi, int G, int m, const float2* tw,
-                  double* part, hipStream_t s, bool dx_ready = false) {
-  auto go = [&](auto rows_kern, auto cols_kern, int nb, int nc) {
-    const long rows = (long)G * m;
-    if (!dx_ready)
-      hipLaunchKernelGGL(rows_kern, dim3((unsigned)((rows + nb - 1) / nb)), dim3(PTYK_NT), 0, s, op, dx, rows, tw);
+// Position-step partials of G frames from the object patches op and their row derivatives dx
+// (written by the exit-wave row pass).  Returns the chunks per frame, or 0 when m has no
+// one-dimensional form (the caller then takes the 2D one).
+static int pos_1d(const float2* op, const float2* dx, const float2* vp, const float2* chi, int G, int m,
+                  const float2* tw, double* part, hipStream_t s) {
+  auto go = [&](auto cols_kern, int nc) {
     hipLaunchKernelGGL(cols_kern, dim3(m / nc, G), dim3(PTYK_NT), 0, s, op, dx, vp, chi, tw, part);
     return m / nc;
   };
-#define PTYK_POS_CASE(MM)                                                                         \
-  case MM:                                                                                        \
-    return go(k_dx_rows<MM>, k_pos_cols<MM>, plan_min_batch<MM, PTYK_NT>(),                       \
-              (plan_min_batch<MM, PTYK_NT>() + 15) / 16 * 16);
+#define PTYK_POS_CASE(MM) \
+  case MM: return go(k_pos_cols<MM>, (plan_min_batch<MM, PTYK_NT>() + 15) / 16 * 16);
   switch (m) {
     PTYK_POS_CASE(128)
     PTYK_POS_CASE(256)
@@ -1515,7 +1464,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         // 51x51 / 256^2 / 5-mode iteration slower)
         pos_work = [&d, G, m, gf](hipStream_t st) -> hipError_t {
           // dX came with the exit waves (k_mls_pass_a)
-          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st, true);
+          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
           hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, st, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
           return hipGetLastError();
         };
