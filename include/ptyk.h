@@ -144,9 +144,13 @@ int ptyk_dm_iteration_timed(ptyk_ctx* ctx, uint32_t flags, float* kernel_ms);
  * and flag set (one launch per group); 0 (default) launches every kernel from the host.
  * PTYK_TUNE_DM_GRAPH: 1 replays every DM iteration after the first from a HIP graph captured once
  * per flag set on the handle's stream (re-captured when the schedule changes); 0 (default)
- * launches the iteration's kernels from the host. */
+ * launches the iteration's kernels from the host.
+ * PTYK_TUNE_MLS_ROW_WAVES: 1 (default) runs the MLs exit-wave and inverse row passes and the
+ * probe-mode update for M in {128, 256} as wave tiles (one wave per tile of whole row groups,
+ * no workgroup barrier between its transform stages); 0 as workgroup tiles.  The row passes give
+ * bitwise the same values either way; the probe-mode sums add their frames in another order. */
 enum { PTYK_TUNE_DM_WAVE = 1, PTYK_TUNE_MLS_MOD_CB = 2, PTYK_TUNE_MLS_PUM_FPC = 3, PTYK_TUNE_MLS_GRAPHS = 4,
-       PTYK_TUNE_DM_GRAPH = 5 };
+       PTYK_TUNE_DM_GRAPH = 5, PTYK_TUNE_MLS_ROW_WAVES = 6 };
 int ptyk_set_tuning(ptyk_ctx* ctx, int32_t key, int32_t value);
 int ptyk_get_tuning(ptyk_ctx* ctx, int32_t key, int32_t* value);
 

@@ -49,6 +49,7 @@ TUNE_MLS_MOD_CB = 2
 TUNE_MLS_PUM_FPC = 3
 TUNE_MLS_GRAPHS = 4
 TUNE_DM_GRAPH = 5
+TUNE_MLS_ROW_WAVES = 6
 DM_KERNELS = ("row_pass", "column_pass", "inverse_row_pass", "probe_gather", "object_gather", "probe_apply",
               "object_apply", "finish")
 

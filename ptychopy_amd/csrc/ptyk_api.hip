@@ -52,6 +52,7 @@ struct ptyk_ctx {
   // wave-scheduled DM
   int dm_wave_cfg = 0;  // tuning: 0 auto, -1 all frames per pass, k > 0 frames per wave
   int mls_mod_cb = 16, mls_pum_fpc = 4;  // tuning: MLs modulus block width, probe-mode chunk frames
+  int mls_row_waves = 1;                   // tuning: wave-tile MLs row passes (M <= 256)
   int mls_graphs = 0;                    // tuning: MLs groups replayed from captured graphs (off: slower here)
   int dm_graph = 0;                      // tuning: DM iterations replayed from captured graphs
   std::vector<std::pair<uint32_t, hipGraphExec_t>> dm_graphs;  // one per flag set (dm_graph)
@@ -849,6 +850,11 @@ int ptyk_set_tuning(ptyk_ctx* c, int32_t key, int32_t value) {
     c->mls_graphs = value;
     return PTYK_OK;
   }
+  if (key == PTYK_TUNE_MLS_ROW_WAVES) {
+    if (value != 0 && value != 1) return fail(c, PTYK_EINVAL, "MLs row waves: 0 or 1");
+    c->mls_row_waves = value;
+    return PTYK_OK;
+  }
   if (key == PTYK_TUNE_DM_GRAPH) {
     if (value != 0 && value != 1) return fail(c, PTYK_EINVAL, "DM graphs: 0 or 1");
     c->dm_graph = value;
@@ -866,6 +872,10 @@ int ptyk_get_tuning(ptyk_ctx* c, int32_t key, int32_t* value) {
   }
   if (key == PTYK_TUNE_DM_GRAPH) {
     *value = c->dm_graph;
+    return PTYK_OK;
+  }
+  if (key == PTYK_TUNE_MLS_ROW_WAVES) {
+    *value = c->mls_row_waves;
     return PTYK_OK;
   }
   if (key == PTYK_TUNE_MLS_MOD_CB || key == PTYK_TUNE_MLS_PUM_FPC || key == PTYK_TUNE_MLS_GRAPHS) {
@@ -1062,6 +1072,7 @@ static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const
                            c->stream));
   d.mod_cb = c->mls_mod_cb;
   d.pum_fpc = c->mls_pum_fpc;
+  d.row_waves = c->mls_row_waves != 0;
   d.use_graphs = c->mls_graphs != 0;
   MlsTimer tm;
   if (kernel_ms) {

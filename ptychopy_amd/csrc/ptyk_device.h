@@ -209,14 +209,34 @@ struct ColLayout {
   __device__ __forceinline__ static int addr(int b, int e) { return e * PITCH + b; }
 };
 
+// LDS exchange point.  Workgroup form: a barrier.  Wave form: the wave's LDS operations issued
+// so far complete (lgkmcnt(0); the LDS serves one wave's operations in issue order) and the
+// compiler may move no LDS access across the point; no other wave is waited for.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // gfx9 encoding: lgkmcnt(0), vmcnt and expcnt not waited on
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One radix-R Stockham stage over NB sequences, NT threads, in-place through registers.
 // BATCH_FAST: consecutive threads take consecutive sequences (strip layout) rather than
 // consecutive butterflies of one sequence (row layout).
 // PREFETCH: issue every twiddle load of the thread before the first use (interleaved with the
 // LDS reads and the products, the scheduler waited on each load in turn); off where the extra
 // registers would spill (1024-thread workgroups).
-template <int N, int R, int NS, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true>
+// WAVE: the NT = 64 threads of one wave own the sequences, so the exchanges between the stages
+// need only the wave's own LDS operations to complete (lds_sync<true>), not a workgroup barrier:
+// the other waves of the workgroup run their own transforms independently.
+template <int N, int R, int NS, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true,
+          bool WAVE = false>
 __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __restrict__ tw, int tid) {
+  static_assert(!WAVE || NT == 64, "a wave-local transform runs on one wave");
   constexpr int NBF = NB * (N / R);
   static_assert(NBF % NT == 0, "butterflies must divide evenly among threads");
   constexpr int BPT = NBF / NT;
@@ -253,7 +273,7 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
     }
     Dft<R, INV>::run(v[q]);
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
     const int j = jj[q];
@@ -261,17 +281,17 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
 #pragma unroll
     for (int r = 0; r < R; ++r) lds[L::addr(bb[q], idx + r * NS)] = v[q][r];
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 }
 
 // Full transform of NB sequences already in LDS (caller syncs before the call).
-template <int N, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true>
+template <int N, bool INV, int NB, int NT, class L, bool BATCH_FAST, bool PREFETCH = true, bool WAVE = false>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ tw, int tid) {
-  stockham_stage<N, plan_radix<N>(0), plan_ns<N>(0), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
+  stockham_stage<N, plan_radix<N>(0), plan_ns<N>(0), INV, NB, NT, L, BATCH_FAST, PREFETCH, WAVE>(lds, tw, tid);
   if constexpr (Plan<N>::S > 1)
-    stockham_stage<N, plan_radix<N>(1), plan_ns<N>(1), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
+    stockham_stage<N, plan_radix<N>(1), plan_ns<N>(1), INV, NB, NT, L, BATCH_FAST, PREFETCH, WAVE>(lds, tw, tid);
   if constexpr (Plan<N>::S > 2)
-    stockham_stage<N, plan_radix<N>(2), plan_ns<N>(2), INV, NB, NT, L, BATCH_FAST, PREFETCH>(lds, tw, tid);
+    stockham_stage<N, plan_radix<N>(2), plan_ns<N>(2), INV, NB, NT, L, BATCH_FAST, PREFETCH, WAVE>(lds, tw, tid);
 }
 
 // ---------------------------------------------------------------- reductions

@@ -83,6 +83,69 @@ __device__ __forceinline__ void cross_rows(float2* lds, const float2* __restrict
   }
 }
 
+// One wave's tile of a row pass (M <= 256): GPV whole row groups of one plane, ROWS = GPV R0
+// rows {n1 + 64 q}, n1 = t GPV + g for tile t of the plane's T = 64 / GPV tiles.  Pair i of a
+// lane is tile element 2 (lane + 64 i): local row lrow = g R0 + q, column lcol; a lane holds the
+// same columns in every row of its row groups (pair (g R0 + q) HALVES + h), so the radix-R0 step
+// across the rows runs in its registers with no exchange.
+template <int M>
+struct WaveTile {
+  static constexpr int R0 = M / 64;
+  static constexpr int GPV = plan_min_batch<M, 64>() / R0;
+  static constexpr int ROWS = GPV * R0;
+  static constexpr int PAIRS = ROWS * M / 128;  // float4 per lane
+  static constexpr int HALVES = M / 128;        // column pairs per row of a lane
+  static constexpr int T = 64 / GPV;
+  static_assert(M == 128 || M == 256, "a wave tile of whole row groups: M = 128 or 256");
+  static_assert(PAIRS == 8 && 64 % GPV == 0, "tile shape");
+  __device__ __forceinline__ static int lrow(int lane, int i) { return (2 * (lane + 64 * i)) / M; }
+  __device__ __forceinline__ static int lcol(int lane, int i) { return (2 * (lane + 64 * i)) % M; }
+  __device__ __forceinline__ static int grow(int t, int lane, int i) {
+    const int lr = lrow(lane, i);
+    return t * GPV + lr / R0 + 64 * (lr % R0);
+  }
+  // cross_rows' arithmetic on the lane's pairs: forward DFT then W_M^(n1 k2), or the conjugated
+  // twiddle then the inverse DFT
+  template <bool INV>
+  __device__ __forceinline__ static void cross(float4* v, int t, const float2* __restrict__ tw) {
+#pragma unroll
+    for (int g = 0; g < GPV; ++g) {
+      const int n1 = t * GPV + g;
+      float2 w[R0];
+#pragma unroll
+      for (int k2 = 1; k2 < R0; ++k2) {
+        w[k2] = tw[(n1 * k2 * (PTYK_TWN / M)) & (PTYK_TWN - 1)];
+        if (INV) w[k2] = cconj(w[k2]);
+      }
+#pragma unroll
+      for (int h = 0; h < HALVES; ++h) {
+        float2 a[R0], b[R0];
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+          const float4 x = v[(g * R0 + q) * HALVES + h];
+          a[q] = make_float2(x.x, x.y);
+          b[q] = make_float2(x.z, x.w);
+        }
+        if (!INV) {
+          Dft<R0, false>::run(a);
+          Dft<R0, false>::run(b);
+        }
+#pragma unroll
+        for (int k2 = 1; k2 < R0; ++k2) {
+          a[k2] = cmul(a[k2], w[k2]);
+          b[k2] = cmul(b[k2], w[k2]);
+        }
+        if (INV) {
+          Dft<R0, true>::run(a);
+          Dft<R0, true>::run(b);
+        }
+#pragma unroll
+        for (int q = 0; q < R0; ++q) v[(g * R0 + q) * HALVES + h] = make_float4(a[q].x, a[q].y, b[q].x, b[q].y);
+      }
+    }
+  }
+};
+
 // Forward half of a row pass once its input rows are in LDS (the caller has not synchronised):
 // row FFTs, the radix-R0 step across rows, and the store of the tile.
 template <int M>
@@ -718,6 +781,180 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_c_pu(float2* __restrict__ 
   }
 }
 
+// Wave-tile forms of the two MLs row passes (M <= 256): the same arithmetic per row (the same
+// Stockham stages and the same radix-R0 step across the rows, so the results are bitwise those
+// of k_mls_pass_a / k_mls_pass_c_pu), but each wave owns one tile of whole row groups
+// (WaveTile) and exchanges through LDS with wave-local synchronisation only: the waves of a
+// workgroup never wait for one another, so one wave's loads overlap another's transforms.
+// Wave index blockIdx.x W + w covers planes x tiles; waves past the end return at once (no
+// workgroup barrier follows).
+template <int M, int W>
+__global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ psi, MlsExitArgs x, int G,
+                                                         int planes, const float2* __restrict__ tw) {
+  using WT = WaveTile<M>;
+  using L = SwzRowLayout<M, M>;
+  constexpr int PAIRS = WT::PAIRS;
+  __shared__ float2 lds[W][WT::ROWS * M];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wid = (int)blockIdx.x * W + w;
+  if (wid >= planes * WT::T) return;
+  const int plane = wid / WT::T, t = wid % WT::T;
+  const int k = plane / G, j = plane % G;
+  float2* tile = lds[w];
+  const int2 o = x.roi[x.gf[j]];
+  const long mm = (long)M * M;
+  const float2* pk = x.probe + (long)k * mm;
+  const bool shifted = k == 0 && x.subpix;
+  constexpr int HALF = PAIRS / 2;
+#pragma unroll
+  for (int h0 = 0; h0 < PAIRS; h0 += HALF) {
+    float4 pv[HALF];
+    float2 o0[HALF], o1[HALF];
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int r = WT::grow(t, lane, h0 + u), c = WT::lcol(lane, h0 + u);
+      const float2* orow = x.obj + (long)(o.x + r) * x.cols + (o.y + c);
+      o0[u] = orow[0];
+      o1[u] = orow[1];
+      pv[u] = shifted ? *reinterpret_cast<const float4*>(x.vp + (long)j * mm + r * M + c)
+                      : *reinterpret_cast<const float4*>(pk + r * M + c);
+    }
+#pragma unroll
+    for (int u = 0; u < HALF; ++u) {
+      const int r = WT::grow(t, lane, h0 + u), c = WT::lcol(lane, h0 + u);
+      const float2 p0 = make_float2(pv[u].x, pv[u].y), p1 = make_float2(pv[u].z, pv[u].w);
+      // k_psi's operand order: O * vp for the shifted mode 0, P * O otherwise
+      const float2 a = shifted ? cmul(o0[u], p0) : cmul(p0, o0[u]);
+      const float2 b = shifted ? cmul(o1[u], p1) : cmul(p1, o1[u]);
+      *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, h0 + u), c)]) = make_float4(a.x, a.y, b.x, b.y);
+      if (k == 0) {
+        const float4 ov = make_float4(o0[u].x, o0[u].y, o1[u].x, o1[u].y);
+        *reinterpret_cast<float4*>(x.op + (long)j * mm + r * M + c) = ov;
+        if (x.op2) *reinterpret_cast<float4*>(x.op2 + (long)j * mm + r * M + c) = ov;
+      }
+    }
+  }
+  lds_sync<true>();
+  fft_lds<M, false, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      v[i] = *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]);
+    WT::template cross<false>(v, t, tw);
+    float2* dst = psi + (long)plane * mm;
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      *reinterpret_cast<float4*>(dst + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i)) = v[i];
+  }
+  if (k == 0 && x.dx) {
+    // the column derivative of the patch rows this wave wrote (read back by the lanes that wrote
+    // them): row transform, 2 pi i f_col / M, inverse row transform
+    lds_sync<true>();
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int c = WT::lcol(lane, i);
+      *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), c)]) =
+          *reinterpret_cast<const float4*>(x.op + (long)j * mm + (long)WT::grow(t, lane, i) * M + c);
+    }
+    lds_sync<true>();
+    fft_lds<M, false, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+    const float inv = 1.f / (float)M;
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int c = WT::lcol(lane, i);
+      float4* p = reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), c)]);
+      const float4 v = *p;
+      const float s0 = shift_freq(c, M) * inv, s1 = shift_freq(c + 1, M) * inv;
+      const float2 a = cscale(make_float2(-v.y * 6.283185307179586f, v.x * 6.283185307179586f), s0);
+      const float2 b = cscale(make_float2(-v.w * 6.283185307179586f, v.z * 6.283185307179586f), s1);
+      *p = make_float4(a.x, a.y, b.x, b.y);
+    }
+    lds_sync<true>();
+    fft_lds<M, true, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int c = WT::lcol(lane, i);
+      *reinterpret_cast<float4*>(x.dx + (long)j * mm + (long)WT::grow(t, lane, i) * M + c) =
+          *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), c)]);
+    }
+  }
+}
+
+template <int M, int W>
+__global__ __launch_bounds__(64 * W) void k_mls_pass_c_pu_w(float2* __restrict__ psi, int G, float2* __restrict__ pu,
+                                                            const float2* __restrict__ op,
+                                                            const float2* __restrict__ sub, const int* __restrict__ gf,
+                                                            float sign, const float2* __restrict__ tw) {
+  using WT = WaveTile<M>;
+  using L = SwzRowLayout<M, M>;
+  constexpr int PAIRS = WT::PAIRS;
+  __shared__ float2 lds[W][WT::ROWS * M];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wid = (int)blockIdx.x * W + w;
+  if (wid >= G * WT::T) return;
+  const int j = wid / WT::T, t = wid % WT::T;
+  float2* tile = lds[w];
+  const long mm = (long)M * M;
+  float2* pl = psi + (long)j * mm;
+  const float2* ol = op + (long)j * mm;
+  {
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      v[i] = *reinterpret_cast<const float4*>(pl + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i));
+    WT::template cross<true>(v, t, tw);
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]) = v[i];
+  }
+  // the object patch rows of the probe update, in flight during the inverse transform
+  float4 bv[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i)
+    bv[i] = *reinterpret_cast<const float4*>(ol + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i));
+  lds_sync<true>();
+  fft_lds<M, true, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int c = WT::lcol(lane, i);
+    float4* p = reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), c)]);
+    const float4 a = *p, b = bv[i];
+    *reinterpret_cast<float4*>(pl + (long)WT::grow(t, lane, i) * M + c) = a;
+    *p = make_float4(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y, a.z * b.z + a.w * b.w, a.w * b.z - a.z * b.w);
+  }
+  const float sx = sign * sub[gf[j]].x;
+  float2 w0[WT::HALVES], w1[WT::HALVES];
+#pragma unroll
+  for (int h = 0; h < WT::HALVES; ++h) {
+    const int c = WT::lcol(lane, h);
+    sincosf(sx * shift_freq(c, M) * -6.283185307179586f, &w0[h].y, &w0[h].x);
+    sincosf(sx * shift_freq(c + 1, M) * -6.283185307179586f, &w1[h].y, &w1[h].x);
+  }
+  lds_sync<true>();
+  fft_lds<M, false, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+  const float inv_m = 1.f / (float)M;
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int h = i % WT::HALVES;
+    float4* p = reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]);
+    const float4 v = *p;
+    const float2 u0 = cscale(cmul(make_float2(v.x, v.y), w0[h]), inv_m);
+    const float2 u1 = cscale(cmul(make_float2(v.z, v.w), w1[h]), inv_m);
+    *p = make_float4(u0.x, u0.y, u1.x, u1.y);
+  }
+  lds_sync<true>();
+  fft_lds<M, true, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+  float2* dst = pu + (long)j * mm;
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const int c = WT::lcol(lane, i);
+    *reinterpret_cast<float4*>(dst + (long)WT::grow(t, lane, i) * M + c) =
+        *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), c)]);
+  }
+}
+constexpr int ROW_WAVES = 4;
+
 // Column shifts in place: a workgroup takes NC columns (whole 16-column strips) of plane j.
 template <int M>
 __global__ __launch_bounds__(PTYK_NT) void k_mls_shift_cols(float2* __restrict__ x, const float2* __restrict__ sub,
@@ -847,8 +1084,8 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
 }
 
 template <int M, int K, int CB>
-static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const float* amp_scale, const int* gf,
-                                double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
+static hipError_t mls_modulus_k(float2* psi, int G, bool row_waves, const __half* amp, const float* amp_scale,
+                                const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
                                 const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   using C = WaveCfg<M>;
   constexpr int BPF = C::R0 * (M / CB);
@@ -863,8 +1100,16 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
     if (e != hipSuccess) return e;
     lds_set = true;
   }
+  // wave-tile row passes (M <= 256): one wave per tile of whole row groups
+  constexpr bool WAVE_OK = M <= 256;
+  const bool waves = WAVE_OK && row_waves;
+  const int wt = WaveTile<WAVE_OK ? M : 256>::T;
   if (tm) tm->mark(s, MLS_T_PASS_A);
-  if (exit)
+  if (exit && waves) {
+    if constexpr (WAVE_OK)
+      hipLaunchKernelGGL((k_mls_pass_a_w<M, ROW_WAVES>), dim3((unsigned)((K * G * wt + ROW_WAVES - 1) / ROW_WAVES)),
+                         dim3(64 * ROW_WAVES), 0, s, psi, *exit, G, K * G, tw);
+  } else if (exit)
     hipLaunchKernelGGL((k_mls_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
   else
     hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
@@ -873,7 +1118,11 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
   hipLaunchKernelGGL((k_mls_pass_b<M, K, CB>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp,
                      amp_scale, gf, part, tw);
   if (tm) tm->mark(s, MLS_T_PASS_C);
-  if (shift && c_modes == 1)
+  if (shift && c_modes == 1 && waves) {
+    if constexpr (WAVE_OK)
+      hipLaunchKernelGGL((k_mls_pass_c_pu_w<M, ROW_WAVES>), dim3((unsigned)((G * wt + ROW_WAVES - 1) / ROW_WAVES)),
+                         dim3(64 * ROW_WAVES), 0, s, psi, G, shift->pu, shift->op, shift->sub, gf, shift->sign, tw);
+  } else if (shift && c_modes == 1)
     hipLaunchKernelGGL((k_mls_pass_c_pu<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, groups_c, shift->pu, shift->op,
                        shift->sub, gf, shift->sign, tw);
   else
@@ -882,30 +1131,30 @@ static hipError_t mls_modulus_k(float2* psi, int G, const __half* amp, const flo
 }
 
 template <int M, int K>
-static hipError_t mls_modulus_kc(float2* psi, int G, int cb, const __half* amp, const float* amp_scale, const int* gf,
-                                 double* part, const float2* tw, const MlsExitArgs* exit, int cm,
+static hipError_t mls_modulus_kc(float2* psi, int G, int cb, bool rw, const __half* amp, const float* amp_scale,
+                                 const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int cm,
                                  const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   switch (cb) {
-    case 8: return mls_modulus_k<M, K, 8>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 16: return mls_modulus_k<M, K, 16>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 32: return mls_modulus_k<M, K, 32>(psi, G, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 8: return mls_modulus_k<M, K, 8>(psi, G, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 16: return mls_modulus_k<M, K, 16>(psi, G, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 32: return mls_modulus_k<M, K, 32>(psi, G, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int M>
-static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, const __half* amp, const float* amp_scale,
+static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, bool rw, const __half* amp, const float* amp_scale,
                                 const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int cm,
                                 const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   switch (K) {
-    case 1: return mls_modulus_kc<M, 1>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 2: return mls_modulus_kc<M, 2>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 3: return mls_modulus_kc<M, 3>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 4: return mls_modulus_kc<M, 4>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 5: return mls_modulus_kc<M, 5>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 6: return mls_modulus_kc<M, 6>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 7: return mls_modulus_kc<M, 7>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
-    case 8: return mls_modulus_kc<M, 8>(psi, G, cb, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 1: return mls_modulus_kc<M, 1>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 2: return mls_modulus_kc<M, 2>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 3: return mls_modulus_kc<M, 3>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 4: return mls_modulus_kc<M, 4>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 5: return mls_modulus_kc<M, 5>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 6: return mls_modulus_kc<M, 6>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 7: return mls_modulus_kc<M, 7>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    case 8: return mls_modulus_kc<M, 8>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -915,14 +1164,18 @@ int mls_modulus_blocks(int m, int k, int cb) {
   return (m / 64) * (m / cb);
 }
 
-hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, bool row_waves, const __half* amp,
+                              const float* amp_scale,
                               const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
                               const MlsShiftArgs* shift, hipStream_t s, MlsTimer* tm) {
   if (c_modes < 1 || c_modes > K) return hipErrorInvalidValue;
   switch (mls_modulus_blocks(m, K, cb) ? m : 0) {
-    case 128: return mls_modulus_m<128>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
-    case 256: return mls_modulus_m<256>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
-    case 512: return mls_modulus_m<512>(psi, K, G, cb, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
+    case 128:
+      return mls_modulus_m<128>(psi, K, G, cb, row_waves, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
+    case 256:
+      return mls_modulus_m<256>(psi, K, G, cb, row_waves, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
+    case 512:
+      return mls_modulus_m<512>(psi, K, G, cb, row_waves, amp, amp_scale, gf, part, tw, exit, c_modes, shift, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1003,6 +1256,84 @@ __global__ __launch_bounds__(PTYK_NT) void k_pum_rows(const float2* __restrict__
   }
 }
 
+// The same sums with one wave per frame tile (M <= 256): workgroup (tile t, mode k, chunk z) of
+// W waves; wave w takes the frames j0 + w, j0 + w + W, ... of the chunk, each through its own
+// LDS tile of GPV row groups (ROWS = GPV R0 rows {n1 + 64 q}, n1 = t GPV + g): the radix-R0 step
+// across the rows in registers (each lane loads whole columns of its row groups), the inverse
+// row FFT with wave-local exchanges, then the weighted products into per-lane sums.  No
+// workgroup barrier runs until the chunk is done, so the waves' loads and transforms overlap
+// one another; the W per-wave sums then add in wave order (a fixed order: deterministic).
+template <int M, int W>
+__global__ __launch_bounds__(64 * W) void k_pum_waves(const float2* __restrict__ psi, const float2* __restrict__ op,
+                                                      const float* __restrict__ illum, const int2* __restrict__ roi,
+                                                      const int* __restrict__ gf, int K, int G, int cols, float fw,
+                                                      float2* __restrict__ part, int fpc,
+                                                      const float2* __restrict__ tw) {
+  using WT = WaveTile<M>;
+  using L = SwzRowLayout<M, M>;
+  constexpr int PAIRS = WT::PAIRS;
+  __shared__ float2 lds[W][WT::ROWS * M];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = blockIdx.x, k = blockIdx.y + 1;
+  const int j0 = blockIdx.z * fpc, j1 = min(G, j0 + fpc);
+  const long mm = (long)M * M;
+  float2* tile = lds[w];
+  float4 acc[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int j = j0 + w; j < j1; j += W) {
+    const float2* src = psi + ((long)k * G + j) * mm;
+    float4 v[PAIRS];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      v[i] = *reinterpret_cast<const float4*>(src + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i));
+    WT::template cross<true>(v, t, tw);
+    lds_sync<true>();  // the previous frame's products have read the tile
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]) = v[i];
+    lds_sync<true>();
+    fft_lds<M, true, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+    const int2 o = roi[gf[j]];
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+      const int r = WT::grow(t, lane, i), c = WT::lcol(lane, i);
+      const float4 bq = *reinterpret_cast<const float4*>(op + (long)j * mm + (long)r * M + c);
+      const float* ip = illum + (long)(o.x + r) * cols + o.y + c;
+      const float p0 = ip[0], p1 = ip[1];
+      const float4 u = *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), c)]);
+      const float w0 = p0 / (p0 + fw), w1 = p1 / (p1 + fw);
+      const float2 d0 = make_float2(u.x * bq.x + u.y * bq.y, u.y * bq.x - u.x * bq.y);
+      const float2 d1 = make_float2(u.z * bq.z + u.w * bq.w, u.w * bq.z - u.z * bq.w);
+      acc[i].x += w0 * d0.x;
+      acc[i].y += w0 * d0.y;
+      acc[i].z += w1 * d1.x;
+      acc[i].w += w1 * d1.y;
+    }
+  }
+  // the waves' sums in wave order
+  __syncthreads();
+  float4* slots = reinterpret_cast<float4*>(tile);  // slot lane + 64 i of this wave's tile
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) slots[lane + 64 * i] = acc[i];
+  __syncthreads();
+  float2* out = part + ((long)blockIdx.z * (K - 1) + (k - 1)) * mm;
+  for (int sl = threadIdx.x; sl < 64 * PAIRS; sl += 64 * W) {
+    float4 a = reinterpret_cast<const float4*>(lds[0])[sl];
+#pragma unroll
+    for (int u = 1; u < W; ++u) {
+      const float4 x = reinterpret_cast<const float4*>(lds[u])[sl];
+      a.x += x.x;
+      a.y += x.y;
+      a.z += x.z;
+      a.w += x.w;
+    }
+    const int i = sl >> 6, ln = sl & 63;
+    *reinterpret_cast<float4*>(out + (long)WT::grow(t, ln, i) * M + WT::lcol(ln, i)) = a;
+  }
+}
+constexpr int PUM_WAVES = 4;
+
 // P_k += (sum over chunks in order) / G * beta for modes 2..K.
 __global__ void k_pum_finish(float2* __restrict__ probe, const float2* __restrict__ part, int K, int chunks, int G,
                              long mm, const double* __restrict__ beta) {
@@ -1020,11 +1351,21 @@ __global__ void k_pum_finish(float2* __restrict__ probe, const float2* __restric
 template <int M>
 static hipError_t mode_updates_m(float2* probe, const float2* psi, const float2* op, const float* illum,
                                  const int2* roi, const int* gf, int K, int G, int cols, float fw,
-                                 const double* beta, float2* part, int fpc, const float2* tw, hipStream_t s) {
+                                 const double* beta, float2* part, int fpc, bool row_waves, const float2* tw,
+                                 hipStream_t s) {
   using C = WaveCfg<M>;
   const int chunks = (G + fpc - 1) / fpc;
-  hipLaunchKernelGGL((k_pum_rows<M>), dim3(C::L / C::GPW, K - 1, chunks), dim3(PTYK_NT), 0, s, psi, op, illum, roi,
-                     gf, K, G, cols, fw, part, fpc, tw);
+  bool launched = false;
+  if constexpr (M <= 256) {
+    if (row_waves) {
+      hipLaunchKernelGGL((k_pum_waves<M, PUM_WAVES>), dim3(WaveTile<M>::T, K - 1, chunks), dim3(64 * PUM_WAVES), 0,
+                         s, psi, op, illum, roi, gf, K, G, cols, fw, part, fpc, tw);
+      launched = true;
+    }
+  }
+  if (!launched)
+    hipLaunchKernelGGL((k_pum_rows<M>), dim3(C::L / C::GPW, K - 1, chunks), dim3(PTYK_NT), 0, s, psi, op, illum, roi,
+                       gf, K, G, cols, fw, part, fpc, tw);
   const long mm = (long)M * M;
   hipLaunchKernelGGL(k_pum_finish, dim3((unsigned)((mm + PTYK_NT - 1) / PTYK_NT), K - 1), dim3(PTYK_NT), 0, s, probe,
                      part, K, chunks, G, mm, beta);
@@ -1033,12 +1374,15 @@ static hipError_t mode_updates_m(float2* probe, const float2* psi, const float2*
 
 hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
                             const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
-                            int fpc, const float2* tw, hipStream_t s) {
+                            int fpc, bool row_waves, const float2* tw, hipStream_t s) {
   if (K < 2 || G < 1 || fpc < MLS_PUM_FPC_MIN) return hipErrorInvalidValue;
   switch (m) {
-    case 128: return mode_updates_m<128>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
-    case 256: return mode_updates_m<256>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
-    case 512: return mode_updates_m<512>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, tw, s);
+    case 128:
+      return mode_updates_m<128>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, row_waves, tw, s);
+    case 256:
+      return mode_updates_m<256>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, row_waves, tw, s);
+    case 512:
+      return mode_updates_m<512>(probe, psi, op, illum, roi, gf, K, G, cols, fw, beta, part, fpc, row_waves, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

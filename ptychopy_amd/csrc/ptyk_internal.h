@@ -131,8 +131,11 @@ struct MlsShiftArgs {
   const float2* sub;
   float sign;
 };
-hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, const __half* amp, const float* amp_scale,
-                              const int* gf, double* part, const float2* tw, const MlsExitArgs* exit /* nullable */,
+// row_waves: the exit-wave and inverse row passes run as wave tiles for m <= 256 (bitwise the
+// same results as the workgroup-tile passes).
+hipError_t mls_modulus_passes(float2* psi, int K, int G, int m, int cb, bool row_waves, const __half* amp,
+                              const float* amp_scale, const int* gf, double* part, const float2* tw,
+                              const MlsExitArgs* exit /* nullable */,
                               int c_modes, const MlsShiftArgs* shift /* nullable */, hipStream_t s,
                               MlsTimer* tm = nullptr);
 // The column shifts of mls_shift_passes alone (after a fused row pass).
@@ -142,14 +145,15 @@ hipError_t mls_shift_cols(float2* pu, int G, int m, const float2* sub, const int
 // column-transformed planes: the inverse row pass and the frame sum in one kernel over chunks of
 // fpc frames (partials in part, mls_pum_part_len elements), then the chunk sum in frame order
 // (m in {128, 256, 512}, fpc >= MLS_PUM_FPC_MIN; ptyk_dm_wave.hip).  beta: the mean probe step
-// (re, im).
+// (re, im).  row_waves (m <= 256): the chunk's frames spread over the waves of a workgroup, whose
+// sums add in wave order.
 constexpr int MLS_PUM_FPC_MIN = 2;
 inline long mls_pum_part_len(int g, int k, long mm) {  // room for any fpc >= MLS_PUM_FPC_MIN
   return (long)((g + MLS_PUM_FPC_MIN - 1) / MLS_PUM_FPC_MIN) * (k > 1 ? k - 1 : 1) * mm;
 }
 hipError_t mls_mode_updates(float2* probe, const float2* psi, const float2* op, const float* illum, const int2* roi,
                             const int* gf, int K, int G, int m, int cols, float fw, const double* beta, float2* part,
-                            int fpc, const float2* tw, hipStream_t s);
+                            int fpc, bool row_waves, const float2* tw, hipStream_t s);
 
 // ---- ePIE: one persistent workgroup runs all positions of one iteration in order.
 struct EpieArgs {
@@ -184,6 +188,7 @@ struct MlsDev {
   // launch shapes (tuning knobs, same results): column width of the modulus pass's blocks, frames
   // per chunk of the probe-mode update
   int mod_cb = 16, pum_fpc = 4;
+  bool row_waves = true;  // wave-tile row passes and probe-mode update (m <= 256)
   // context arrays (not owned)
   float2* obj = nullptr;
   float2* probe = nullptr;

@@ -1454,7 +1454,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
                            pos_1d_step ? d.dxb : nullptr};
       // with mode 0 alone in the inverse row pass, that pass also forms the probe update's row shifts
       const MlsShiftArgs shf{d.pu, d.op, d.sub, -1.f};
-      TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
+      TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.row_waves, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
                              defer_modes ? 1 : K, pu_fused ? &shf : nullptr, s, tm));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
       if (pos_1d_step) {
@@ -1649,7 +1649,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     MARK(MLS_T_MODES);
     if (defer_modes) {
       TRY(mls_mode_updates(d.probe, d.psi, d.op, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
-                           d.pum_part, d.pum_fpc, d.tw, s));
+                           d.pum_part, d.pum_fpc, d.row_waves, d.tw, s));
     } else if (fuse_modes) {
       LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, m,
              d.cols, fw, d.gscal);
@@ -1694,7 +1694,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     const uint64_t key = (uint64_t)G | (uint64_t)Gn << 12 | (uint64_t)(g > 0) << 24 |
                          (uint64_t)(iter >= d.pps) << 25 | (uint64_t)var_update << 26 | (uint64_t)probe_rec << 27 |
                          (uint64_t)obj_rec_iter << 28 | (uint64_t)(iter < 10) << 29 | (uint64_t)d.mod_cb << 30 |
-                         (uint64_t)d.pum_fpc << 36;
+                         (uint64_t)d.pum_fpc << 36 | (uint64_t)d.row_waves << 44;
     auto it = d.graphs.find(key);
     if (it == d.graphs.end()) {
       if (d.graphs.size() >= 64) mls_drop_graphs(d);

@@ -202,10 +202,46 @@ def test_mls_launch_shapes_give_the_same_iteration(gpu, orc, m):
         assert rel_err(o, base[0]) < 1e-6 and rel_err(p, base[1]) < 1e-6, fpc
     from ptychopy_amd._engine import PtykError
     e = _engine_for(P, P["state"].amp)
-    for key, bad in ((E.TUNE_MLS_MOD_CB, 12), (E.TUNE_MLS_PUM_FPC, 1)):
+    for key, bad in ((E.TUNE_MLS_MOD_CB, 12), (E.TUNE_MLS_PUM_FPC, 1), (E.TUNE_MLS_ROW_WAVES, 2)):
         with pytest.raises(PtykError):
             e.set_tuning(key, bad)
     e.close()
+
+
+@pytest.mark.parametrize("m,k", [(128, 1), (256, 1), (128, 3), (256, 3)])
+def test_mls_row_waves_match_workgroup_tiles(gpu, orc, m, k):
+    """The wave-tile row passes (one wave per tile of whole row groups, wave-local LDS exchanges)
+    compute every row with the same arithmetic as the workgroup-tile passes: with one mode (no
+    probe-mode update) five iterations through position refinement (the exit-wave pass also
+    forms the patches' row derivatives), the variable probe and sub-pixel shifts give the same
+    state bit for bit.  With more modes the probe-mode sums add their frames in another order:
+    after six iterations (the rounding differences carried through every later group) within
+    1e-5 of each other, and the sixth iteration within 1e-5 of the oracle either way."""
+    from ptychopy_amd import _engine as E
+    P = _problem(k=k, m=m, n_side=4, seed=23)
+    rng0 = np.random.default_rng(9)
+    groups = [mls_groups(P["n"], 5, rng0) for _ in range(6)]
+    runs = []
+    for waves in (1, 0):
+        e = _engine_for(P, P["state"].amp, probe_pos_search=2, variable_probe=1, variable_intensity=1)
+        e.set_tuning(E.TUNE_MLS_ROW_WAVES, waves)
+        assert e.get_tuning(E.TUNE_MLS_ROW_WAVES) == waves
+        for it in range(1, 6):
+            e.mls_iteration(it, groups[it - 1])
+        amp = e.get_frames().astype(np.float64)
+        o = _oracle_from_engine(e, orc, amp)
+        e.mls_iteration(6, groups[5])
+        o.iteration(orc.mls_params(P["box"], iter=6, pps=2, variable_probe=1, variable_intensity=1), groups[5])
+        assert rel_err(e.get_object(), o.obj) < 1e-5 and rel_err(e.get_probe(), o.probe) < 1e-5, waves
+        g = e.mls_get_state()
+        runs.append((e.get_object(), e.get_probe(), g["variable_mode"], g["positions"], g["evolution"]))
+        e.close()
+    names = ("object", "probe", "variable mode", "positions", "evolution")
+    for name, a, b in zip(names, *runs):
+        if k == 1:
+            assert np.array_equal(a, b), name
+        else:
+            assert rel_err(a, b) < 1e-5, name
 
 
 def test_mls_autotuner_persists_a_timing_table(gpu, tmp_path):
