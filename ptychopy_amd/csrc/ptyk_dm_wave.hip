@@ -1049,9 +1049,11 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
     av[p] = idx < BLK ? __half2float(amp[(long)f * mm + (long)(C::R0 * (idx / CB) + k2) * M + c0 + idx % CB]) * sc
                       : 0.f;
   }
+  // each wave transforms its own mode's columns with wave-local exchanges; the modulus, which
+  // reads every mode at a pixel, waits for all of them
+  lds_sync<true>();
+  fft_lds<64, false, CB, 64, ModeColLayout<CB>, true, true, true>(lds + wk * BLK, tw, ln);
   __syncthreads();
-  // all modes in one batch: each stage's barrier serves every mode
-  fft_lds<64, false, K * CB, BT, ModeColLayout<CB>, true>(lds, tw, tid);
   const float inv_mm = 1.f / (float)mm;
   double err = 0.0;
 #pragma unroll
@@ -1069,7 +1071,7 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
     for (int k = 0; k < K; ++k) lds[k * BLK + idx] = cscale(lds[k * BLK + idx], R);
   }
   __syncthreads();
-  fft_lds<64, true, K * CB, BT, ModeColLayout<CB>, true>(lds, tw, tid);
+  fft_lds<64, true, CB, 64, ModeColLayout<CB>, true, true, true>(lds + wk * BLK, tw, ln);
   {
     float2* base = psi + ((long)wk * G + j) * mm + (long)(64 * k2) * M + c0;
 #pragma unroll

@@ -489,18 +489,38 @@ __global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, 
   if (q >= mm) return;
   const int r = q / m, c = q % m;
   float2 acc = make_float2(0.f, 0.f);
+  auto add = [&](float p, float2 u) {
+    const float w = p / (p + fw);
+    acc.x += w * u.x;
+    acc.y += w * u.y;
+  };
+  int j = 0;
+  if (!on_the_fly) {
+    // eight frames' loads in flight before their sums (which stay in frame order)
+    constexpr int B = 8;
+    for (; j + B <= G; j += B) {
+      float p[B];
+      float2 u[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int2 o = roi[gf[j + b]];
+        p[b] = illum[(long)(o.x + r) * cols + o.y + c];
+        u[b] = src[(long)(j + b) * mm + q];
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) add(p[b], u[b]);
+    }
+  }
 #pragma unroll 4
-  for (int j = 0; j < G; ++j) {
+  for (; j < G; ++j) {
     const int2 o = roi[gf[j]];
     const float p = illum[(long)(o.x + r) * cols + o.y + c];
-    const float w = p / (p + fw);
     float2 u = src[(long)j * mm + q];
     if (on_the_fly) {
       const float2 b = op[(long)j * mm + q];
       u = make_float2(u.x * b.x + u.y * b.y, u.y * b.x - u.x * b.y);
     }
-    acc.x += w * u.x;
-    acc.y += w * u.y;
+    add(p, u);
   }
   pum[q] = cscale(acc, (float)(1.0 / G));
 }
@@ -969,7 +989,16 @@ __global__ __launch_bounds__(NT) void k_vp_acc(float2* __restrict__ acc, const f
   double s = 0.0;
   if (q < mm) {
     float2 a = make_float2(0.f, 0.f);
-    for (int j = 0; j < G; ++j) a = cadd(a, cscale(img[(long)j * mm + q], (float)gscal[MLS_S_COEF + j]));
+    int j = 0;
+    constexpr int B = 8;  // eight frames' loads in flight before their sums (in frame order)
+    for (; j + B <= G; j += B) {
+      float2 u[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) u[b] = img[(long)(j + b) * mm + q];
+#pragma unroll
+      for (int b = 0; b < B; ++b) a = cadd(a, cscale(u[b], (float)gscal[MLS_S_COEF + j + b]));
+    }
+    for (; j < G; ++j) a = cadd(a, cscale(img[(long)j * mm + q], (float)gscal[MLS_S_COEF + j]));
     a = cscale(a, (float)(1.0 / mm));
     acc[q] = a;
     s = (double)a.x * a.x + (double)a.y * a.y;
