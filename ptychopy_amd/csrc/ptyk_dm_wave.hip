@@ -789,7 +789,7 @@ __global__ __launch_bounds__(PTYK_NT) void k_mls_pass_c_pu(float2* __restrict__ 
 // Wave index blockIdx.x W + w covers planes x tiles; waves past the end return at once (no
 // workgroup barrier follows).
 template <int M, int W>
-__global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ psi, MlsExitArgs x, int G,
+__global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ psi, MlsExitArgs x, int G, int p0,
                                                          int planes, const float2* __restrict__ tw) {
   using WT = WaveTile<M>;
   using L = SwzRowLayout<M, M>;
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ ps
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wid = (int)blockIdx.x * W + w;
   if (wid >= planes * WT::T) return;
-  const int plane = wid / WT::T, t = wid % WT::T;
+  const int plane = p0 + wid / WT::T, t = wid % WT::T;
   const int k = plane / G, j = plane % G;
   float2* tile = lds[w];
   const int2 o = x.roi[x.gf[j]];
@@ -1106,11 +1106,13 @@ static hipError_t mls_modulus_k(float2* psi, int G, bool row_waves, const __half
   constexpr bool WAVE_OK = M <= 256;
   const bool waves = WAVE_OK && row_waves;
   const int wt = WaveTile<WAVE_OK ? M : 256>::T;
+  if (exit && (exit->p0 != 0 || exit->np != 0) && !waves) return hipErrorInvalidValue;
   if (tm) tm->mark(s, MLS_T_PASS_A);
   if (exit && waves) {
+    const int np = exit->np ? exit->np : K * G;
     if constexpr (WAVE_OK)
-      hipLaunchKernelGGL((k_mls_pass_a_w<M, ROW_WAVES>), dim3((unsigned)((K * G * wt + ROW_WAVES - 1) / ROW_WAVES)),
-                         dim3(64 * ROW_WAVES), 0, s, psi, *exit, G, K * G, tw);
+      hipLaunchKernelGGL((k_mls_pass_a_w<M, ROW_WAVES>), dim3((unsigned)((np * wt + ROW_WAVES - 1) / ROW_WAVES)),
+                         dim3(64 * ROW_WAVES), 0, s, psi, *exit, G, exit->p0, np, tw);
   } else if (exit)
     hipLaunchKernelGGL((k_mls_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
   else
@@ -1157,6 +1159,21 @@ static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, bool rw, cons
     case 6: return mls_modulus_kc<M, 6>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     case 7: return mls_modulus_kc<M, 7>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
     case 8: return mls_modulus_kc<M, 8>(psi, G, cb, rw, amp, amp_scale, gf, part, tw, exit, cm, shift, s, tm);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t mls_exit_rows(float2* psi, int G, int m, const MlsExitArgs* exit, int p0, int np, const float2* tw,
+                         hipStream_t s) {
+  if (!exit || p0 < 0 || np < 1) return hipErrorInvalidValue;
+  auto go = [&](auto kern, int T) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)((np * T + ROW_WAVES - 1) / ROW_WAVES)), dim3(64 * ROW_WAVES), 0, s, psi,
+                       *exit, G, p0, np, tw);
+    return hipGetLastError();
+  };
+  switch (m) {
+    case 128: return go(k_mls_pass_a_w<128, ROW_WAVES>, WaveTile<128>::T);
+    case 256: return go(k_mls_pass_a_w<256, ROW_WAVES>, WaveTile<256>::T);
     default: return hipErrorInvalidValue;
   }
 }

@@ -109,8 +109,16 @@ struct MlsExitArgs {
   // nullable: mode 0's workgroups also write the column derivative of each object-patch row,
   // dX = IFFT_row(FFT_row(O_r) 2 pi i f_col) / M (the position step's first pass)
   float2* dx;
+  // the exit-wave row pass covers planes [p0, p0 + np) of the K G (np = 0: all of them; a
+  // partial range needs the wave-tile pass, m in {128, 256} with row_waves)
+  int p0 = 0, np = 0;
 };
 int mls_modulus_blocks(int m, int k, int cb);
+// The exit-wave row pass alone over planes [p0, p0 + np) (wave tiles, m in {128, 256}): the
+// probe modes 1..K-1 of a group can be formed before the group's mode 0, whose inputs (the
+// illumination probes, the patches the position step still reads) become final later.
+hipError_t mls_exit_rows(float2* psi, int G, int m, const MlsExitArgs* exit, int p0, int np, const float2* tw,
+                         hipStream_t s);
 // MLs launch categories of ptyk_mls_iteration_timed (main stream only; the second stream's
 // launches overlap them and are not charged).
 enum { MLS_T_ITER, MLS_T_PROBES, MLS_T_PASS_A, MLS_T_PASS_B, MLS_T_PASS_C, MLS_T_PROBE_UPDATE, MLS_T_PRECOND,

@@ -1444,6 +1444,11 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     side_work[n_side++] = SideItem{lane, n_points - 1, after_vp, signal_vp, std::move(work)};
     return hipSuccess;
   };
+  // The join of a group's side streams can be left to the start of the next group (not under
+  // graph capture, where every group's capture must end joined): join_lane is the stream whose
+  // end event the main stream still has to wait on, join_G the size of that group.
+  bool defer_join = false;
+  int join_lane = -1, join_G = 0;
   // One group's work on both streams; gf and gfn are the frame lists of this group and the next.
   auto group_body = [&](int g, const int* gf, const int* gfn) -> hipError_t {
     const int G = goffs[g + 1] - goffs[g];
@@ -1455,11 +1460,28 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     const dim3 gpix((mm + NT - 1) / NT, G), gch(ch, G);
     // illumination probe, exit waves, Fourier transforms
     const bool pos_step = iter >= d.pps;
-    MARK(MLS_T_PROBES);
-    if (subpix && !vp_ready) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     // with the three-pass exit waves the object patches stay in op, and the position step's
     // derivative images come from one-dimensional transforms of them
     const bool pos_1d_step = pos_step && mod_blocks && has_pos_1d(m) && d.side;
+    const bool spec2d = pos_step && !pos_1d_step;
+    // the exit waves of modes 1..K-1 read only O and P_1..K-1 (final on this stream) and write
+    // only their planes G .. K G - 1, past every plane the previous group's side work reads (its
+    // chi_0 planes 0 .. join_G - 1) when G >= join_G: they go ahead of the join, so the main
+    // stream has work while the previous group's side streams finish
+    bool modes_early = false;
+    const MlsExitArgs ex0{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr,
+                          pos_1d_step ? d.dxb : nullptr};
+    if (join_lane >= 0) {
+      if (vp_ready && mod_blocks && d.row_waves && (m == 128 || m == 256) && K > 1 && G >= join_G) {
+        MARK(MLS_T_PASS_A);
+        TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s));
+        modes_early = true;
+      }
+      TRY(hipStreamWaitEvent(s, d.ev_join[join_lane], 0));
+      join_lane = -1;
+    }
+    MARK(MLS_T_PROBES);
+    if (subpix && !vp_ready) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     // modes 2..K only feed the fused probe-mode update: their inverse row pass is left to it
     const bool defer_modes = probe_rec && !d.apply_multimodal_update && K > 1 && mod_blocks && d.pum_part;
     const bool pu_fused = defer_modes && subpix;
@@ -1476,11 +1498,10 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     }
     if (mod_blocks) {
       // three passes for the exit waves, the first of which forms them (and the object patches)
-      // from O, P and vp; the position step's patch copy takes its own forward transform
-      const bool spec2d = pos_step && !pos_1d_step;
-      // with the one-dimensional position step, mode 0's exit-wave workgroups also form its dX rows
-      const MlsExitArgs ex{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr,
-                           pos_1d_step ? d.dxb : nullptr};
+      // from O, P and vp; the position step's patch copy takes its own forward transform; with
+      // the one-dimensional position step, mode 0's exit-wave workgroups also form its dX rows
+      MlsExitArgs ex = ex0;
+      if (modes_early) ex.np = G;  // mode 0's planes only
       // with mode 0 alone in the inverse row pass, that pass also forms the probe update's row shifts
       const MlsShiftArgs shf{d.pu, d.op, d.sub, -1.f};
       TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.row_waves, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
@@ -1704,7 +1725,12 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     if (used[0] || used[1]) {
       const int l = used[0] ? 0 : 1;
       TRY(hipEventRecord(d.ev_join[l], l ? d.side2 : d.side));
-      TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
+      if (defer_join) {
+        join_lane = l;
+        join_G = G;
+      } else {
+        TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
+      }
     }
     n_side = n_points = 0;
     return hipSuccess;
@@ -1712,6 +1738,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
   // Group graphs (not under the launch timer): a group's shape and the iteration's flags key
   // one captured graph, replayed by one launch per group.
   const bool graphs = d.use_graphs && !tm && d.side && d.gsel && d.goffs_dev;
+  defer_join = !graphs;
   if (graphs) LAUNCH(k_set_int, dim3(1), dim3(1), d.gsel, 0);
   for (int g = 0; g < n_groups; ++g) {
     if (!graphs) {
@@ -1744,6 +1771,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     }
     TRY(hipGraphLaunch(it->second, s));
   }
+  if (join_lane >= 0) TRY(hipStreamWaitEvent(s, d.ev_join[join_lane], 0));  // the last group's
   MARK(MLS_T_ITER);
   if (K > d.n_rec) TRY(mls_ortho(d, s));
   MARK(-1);
