@@ -117,6 +117,12 @@ static hipError_t dalloc(T** p, size_t count) {
 
 static void free_mls(ptyk_ctx* c) {
   MlsDev& d = c->mls;
+  if (d.enq) {  // no task may be left to touch the streams and events released below
+    (void)d.enq->drain();
+    d.enq.reset();
+  }
+  if (d.side) (void)hipStreamSynchronize(d.side);
+  if (d.side2) (void)hipStreamSynchronize(d.side2);
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
                   d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot,
@@ -899,6 +905,7 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     for (hipEvent_t& ev : d.ev_join) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&d.ev_vpdone, hipEventDisableTiming));
   }
+  if (!d.enq) d.enq.reset(new SideEnqueuer(c->cfg.device));
   if (g > d.gmax) {
     mls_drop_graphs(d);  // they hold the addresses of the scratch about to move
     void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,  d.gsel,

@@ -4,7 +4,13 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 namespace ptyk {
 
@@ -180,6 +186,31 @@ struct EpieArgs {
 hipError_t epie_iteration(const EpieArgs& a, hipStream_t s);
 
 // ---- MLs (maximum likelihood with least-squares steps), ptyk_mls.hip.
+// A host thread that enqueues a group's side-stream work (event waits, launches, event records)
+// while the calling thread goes on enqueueing the main stream's: the HIP calls of one iteration
+// are shared between two host threads.  Tasks run in the order they were posted; drain() returns
+// once every posted task has run, with the first error any of them returned (then cleared).
+class SideEnqueuer {
+ public:
+  explicit SideEnqueuer(int device);
+  ~SideEnqueuer();
+  SideEnqueuer(const SideEnqueuer&) = delete;
+  SideEnqueuer& operator=(const SideEnqueuer&) = delete;
+  void post(std::function<hipError_t()> task);
+  hipError_t drain();
+
+ private:
+  void run();
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::deque<std::function<hipError_t()>> q_;
+  long posted_ = 0, done_ = 0;
+  hipError_t err_ = hipSuccess;
+  bool stop_ = false;
+  int device_;
+  std::thread th_;
+};
+
 // Slots of MlsDev::gscal (device FP64 scalars written and read by the MLs kernels).
 enum { MLS_S_ONORM = 0, MLS_S_TMP = 1, MLS_S_BO = 2 /* re, im */, MLS_S_BP = 4 /* re, im */, MLS_S_PMAX = 6,
        MLS_S_COEF = 16 /* one per frame of the group */ };
@@ -244,6 +275,8 @@ struct MlsDev {
   std::map<uint64_t, hipGraphExec_t> graphs;
   uint64_t alloc_gen = 0;    // bumped when the group scratch moves (captured graphs hold its addresses)
   bool use_graphs = false;
+  // side-stream work enqueued from a second host thread (not under graph capture)
+  std::unique_ptr<SideEnqueuer> enq;
 };
 constexpr int MLS_SIDE_FORKS = 4;
 int mls_chunks(int m);  // frame chunks of the per-frame reductions

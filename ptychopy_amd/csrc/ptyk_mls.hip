@@ -1364,6 +1364,59 @@ hipError_t mls_init(MlsDev& d, const double* frame_power, hipStream_t s) {
 
 long mls_part_min(int k) { return (6L + std::max(k, ORTHO_KMAX)) * ORTHO_NB; }
 
+SideEnqueuer::SideEnqueuer(int device) : device_(device), th_([this] { run(); }) {}
+
+SideEnqueuer::~SideEnqueuer() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_one();
+  th_.join();
+}
+
+void SideEnqueuer::post(std::function<hipError_t()> task) {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    q_.push_back(std::move(task));
+    ++posted_;
+  }
+  cv_.notify_one();
+}
+
+hipError_t SideEnqueuer::drain() {
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [this] { return done_ == posted_; });
+  const hipError_t e = err_;
+  err_ = hipSuccess;
+  return e;
+}
+
+void SideEnqueuer::run() {
+  const hipError_t set = hipSetDevice(device_);  // the current device is per host thread
+  for (;;) {
+    std::function<hipError_t()> task;
+    bool skip;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+      if (q_.empty()) return;  // stop_ with nothing left
+      task = std::move(q_.front());
+      q_.pop_front();
+      // after an error the remaining tasks are dropped (their waits and launches would build on
+      // a schedule that is already broken); drain() reports the first error
+      skip = err_ != hipSuccess;
+    }
+    const hipError_t e = set != hipSuccess ? set : (skip ? hipSuccess : task());
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (e != hipSuccess && err_ == hipSuccess) err_ = e;
+      ++done_;
+    }
+    cv_done_.notify_all();
+  }
+}
+
 #define MARK(c) \
   do {          \
     if (tm) tm->mark(s, (c)); \
@@ -1439,16 +1492,45 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     if (n_points >= MLS_SIDE_FORKS) return hipErrorInvalidValue;
     return hipEventRecord(d.ev_side[n_points++], s);
   };
-  auto fork = [&](int lane, bool after_vp, bool signal_vp, std::function<hipError_t(hipStream_t)> work) -> hipError_t {
-    if (n_side >= MLS_SIDE_FORKS || n_points == 0) return hipErrorInvalidValue;
-    side_work[n_side++] = SideItem{lane, n_points - 1, after_vp, signal_vp, std::move(work)};
-    return hipSuccess;
-  };
   // The join of a group's side streams can be left to the start of the next group (not under
   // graph capture, where every group's capture must end joined): join_lane is the stream whose
-  // end event the main stream still has to wait on, join_G the size of that group.
+  // end event the main stream still has to wait on, join_G the size of that group.  With the
+  // join deferred, a second host thread (d.enq) enqueues each side piece as soon as it forks,
+  // while this thread goes on with the main stream; the main stream's wait on the join event
+  // follows the drain of that thread (so the event has been recorded), and so does every fork
+  // event's next record (the side pieces' waits on it have been enqueued).
   bool defer_join = false;
   int join_lane = -1, join_G = 0;
+  bool side_used[2] = {false, false};  // streams A and B took work this group
+  auto side_task = [&d](const SideItem& w) {
+    hipStream_t st = w.lane ? d.side2 : d.side;
+    return [&d, st, point = w.point, after_vp = w.after_vp, signal_vp = w.signal_vp,
+            work = w.work]() -> hipError_t {
+      TRY(hipStreamWaitEvent(st, d.ev_side[point], 0));
+      if (after_vp) TRY(hipStreamWaitEvent(st, d.ev_vpdone, 0));
+      TRY(work(st));
+      if (signal_vp) TRY(hipEventRecord(d.ev_vpdone, st));
+      return hipSuccess;
+    };
+  };
+  auto fork = [&](int lane, bool after_vp, bool signal_vp, std::function<hipError_t(hipStream_t)> work) -> hipError_t {
+    if (n_side >= MLS_SIDE_FORKS || n_points == 0) return hipErrorInvalidValue;
+    SideItem w{lane, n_points - 1, after_vp, signal_vp, std::move(work)};
+    side_used[lane] = true;
+    if (defer_join && d.enq) {
+      d.enq->post(side_task(w));  // enqueued now, by the second host thread
+    } else {
+      side_work[n_side++] = std::move(w);  // enqueued after the group's main-stream launches
+    }
+    return hipSuccess;
+  };
+  // the deferred join: the second host thread's tasks have all run (the join event is recorded)
+  auto join_wait = [&]() -> hipError_t {
+    if (d.enq) TRY(d.enq->drain());
+    TRY(hipStreamWaitEvent(s, d.ev_join[join_lane], 0));
+    join_lane = -1;
+    return hipSuccess;
+  };
   // One group's work on both streams; gf and gfn are the frame lists of this group and the next.
   auto group_body = [&](int g, const int* gf, const int* gfn) -> hipError_t {
     const int G = goffs[g + 1] - goffs[g];
@@ -1477,8 +1559,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s));
         modes_early = true;
       }
-      TRY(hipStreamWaitEvent(s, d.ev_join[join_lane], 0));
-      join_lane = -1;
+      TRY(join_wait());
     }
     MARK(MLS_T_PROBES);
     if (subpix && !vp_ready) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
@@ -1705,34 +1786,32 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
              d.cols, fw, d.gscal);
       CHK();
     }
-    // the group's second-stream work in fork order, then the joins before the next group's writes
-    bool used[2] = {false, false};
-    for (int i = 0; i < n_side; ++i) {
-      SideItem& w = side_work[i];
-      hipStream_t st = w.lane ? d.side2 : d.side;
-      TRY(hipStreamWaitEvent(st, d.ev_side[w.point], 0));
-      if (w.after_vp) TRY(hipStreamWaitEvent(st, d.ev_vpdone, 0));
-      TRY(w.work(st));
-      if (w.signal_vp) TRY(hipEventRecord(d.ev_vpdone, st));
-      used[w.lane] = true;
-    }
+    // the group's second-stream work in fork order (when not already enqueued by the second host
+    // thread), then the joins before the next group's writes
+    for (int i = 0; i < n_side; ++i) TRY(side_task(side_work[i])());
     MARK(MLS_T_JOIN);
     // one join on this stream: with both side streams in use, stream A first waits for B's end
-    if (used[0] && used[1]) {
-      TRY(hipEventRecord(d.ev_join[1], d.side2));
-      TRY(hipStreamWaitEvent(d.side, d.ev_join[1], 0));
-    }
-    if (used[0] || used[1]) {
-      const int l = used[0] ? 0 : 1;
-      TRY(hipEventRecord(d.ev_join[l], l ? d.side2 : d.side));
-      if (defer_join) {
-        join_lane = l;
-        join_G = G;
-      } else {
-        TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
-      }
-    }
+    const bool u0 = side_used[0], u1 = side_used[1];
+    side_used[0] = side_used[1] = false;
     n_side = n_points = 0;
+    if (!u0 && !u1) return hipSuccess;
+    const int l = u0 ? 0 : 1;
+    auto record_join = [&d, u0, u1, l]() -> hipError_t {
+      if (u0 && u1) {
+        TRY(hipEventRecord(d.ev_join[1], d.side2));
+        TRY(hipStreamWaitEvent(d.side, d.ev_join[1], 0));
+      }
+      return hipEventRecord(d.ev_join[l], l ? d.side2 : d.side);
+    };
+    if (defer_join) {
+      if (d.enq) d.enq->post(record_join);
+      else TRY(record_join());
+      join_lane = l;
+      join_G = G;
+    } else {
+      TRY(record_join());
+      TRY(hipStreamWaitEvent(s, d.ev_join[l], 0));
+    }
     return hipSuccess;
   };
   // Group graphs (not under the launch timer): a group's shape and the iteration's flags key
@@ -1771,7 +1850,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     }
     TRY(hipGraphLaunch(it->second, s));
   }
-  if (join_lane >= 0) TRY(hipStreamWaitEvent(s, d.ev_join[join_lane], 0));  // the last group's
+  if (join_lane >= 0) TRY(join_wait());  // the last group's
   MARK(MLS_T_ITER);
   if (K > d.n_rec) TRY(mls_ortho(d, s));
   MARK(-1);
@@ -1784,6 +1863,7 @@ hipError_t mls_iteration(MlsDev& d, int iter, const int* goffs, int n_groups, hi
   // group buffers and writes the positions, V and the evolution: it finishes before the error is
   // reported, so that no later call overwrites what it reads
   if (e != hipSuccess) {
+    if (d.enq) (void)d.enq->drain();  // nothing more is enqueued on the side streams
     if (d.side) (void)hipStreamSynchronize(d.side);
     if (d.side2) (void)hipStreamSynchronize(d.side2);
   }
