@@ -124,18 +124,19 @@ def dm_bytes_per_frame(m: int, k: int, obj_px: int, n: int) -> float:
     return m * m * (2 + 16 * k) + (16 * obj_px + 16 * k * m * m) / n
 
 
-def mls_kind_bytes(kind: str, m: int, k: int, n: int, groups: int) -> float:
+def mls_kind_bytes(kind: str, m: int, k: int, n: int, groups: int, refine: bool = True) -> float:
     """Algorithmic bytes of one MLs iteration's launches of a main-stream kind (all groups), at
     storage precision, for the three-pass form with probe modes 2..K deferred to their update:
     exit_row_pass reads the object patch and mode 0's illumination probe (8 + 8 B/px) and writes
-    K exit-wave planes and the patch (8 K + 8); modulus_column_pass reads and writes the K planes
-    and reads FP16 A (16 K + 2); inverse_row_pass reads mode 0's plane and the patch and writes
-    chi_0 and the row-shifted probe update (32; it forms the probe update's row shifts while the rows
-    are in LDS); probe_modes reads planes 2..K, the patch and the illumination at the ROI
+    K exit-wave planes and the patch (8 K + 8), and with position refinement (refine) the patch's
+    row derivative dX (8); modulus_column_pass reads and writes the K planes and reads FP16 A
+    (16 K + 2); inverse_row_pass reads mode 0's plane and the patch and writes chi_0 and the
+    row-shifted probe update (32; it forms the probe update's row shifts while the rows are in
+    LDS); probe_modes reads planes 2..K, the patch and the illumination at the ROI
     (8 (K - 1) + 8 + 4)."""
     px = float(n) * m * m
-    per_px = {"exit_row_pass": 8.0 * k + 24.0, "modulus_column_pass": 16.0 * k + 2.0, "inverse_row_pass": 32.0,
-              "probe_modes": 8.0 * (k - 1) + 12.0}
+    per_px = {"exit_row_pass": 8.0 * k + 24.0 + (8.0 if refine else 0.0), "modulus_column_pass": 16.0 * k + 2.0,
+              "inverse_row_pass": 32.0, "probe_modes": 8.0 * (k - 1) + 12.0}
     if kind not in per_px:
         return 0.0
     return per_px[kind] * px + (8.0 * k * m * m * groups if kind == "exit_row_pass" else 0.0)
