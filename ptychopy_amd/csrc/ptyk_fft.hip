@@ -131,6 +131,23 @@ static hipError_t fft2_pair_impl(float2* dst, const float2* a, const float2* b, 
   return hipGetLastError();
 }
 
+template <int M>
+static hipError_t fft_rows_pair_impl(float2* dst, const float2* a, const float2* b, const float2* tw, hipStream_t s) {
+  const long rows = 2L * M;
+  const long gr = (rows + FftCfg<M>::RNB - 1) / FftCfg<M>::RNB;
+  hipLaunchKernelGGL((k_fft_rows<M, false, true>), dim3((unsigned)gr), dim3(PTYK_NT), 0, s, dst, rows, tw, a, b);
+  return hipGetLastError();
+}
+
+hipError_t fft_rows_pair(float2* dst, const float2* a, const float2* b, int m, const float2* tw, hipStream_t s) {
+  switch (m) {
+    case 128: return fft_rows_pair_impl<128>(dst, a, b, tw, s);
+    case 256: return fft_rows_pair_impl<256>(dst, a, b, tw, s);
+    case 512: return fft_rows_pair_impl<512>(dst, a, b, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t fft2_pair(float2* dst, const float2* a, const float2* b, int m, const float2* tw, hipStream_t s) {
   switch (m) {
     case 16: return fft2_pair_impl<16>(dst, a, b, tw, s);

@@ -27,6 +27,8 @@ hipError_t fft_cols_batch(float2* data, long n_frames, int m, bool inverse, cons
 // Forward 2D FFT of two frames a and b (any sizes fft2_batch takes) into dst[0..2 m^2), out of
 // place: the row pass reads a and b, so they need not be copied next to each other first.
 hipError_t fft2_pair(float2* dst, const float2* a, const float2* b, int m, const float2* tw, hipStream_t s);
+// The row half of fft2_pair alone (m in {128, 256, 512}): dst holds the row spectra of a and b.
+hipError_t fft_rows_pair(float2* dst, const float2* a, const float2* b, int m, const float2* tw, hipStream_t s);
 
 // ---- DM (difference map) per-iteration launches; see ptyk_dm.hip for the equations.
 struct DmArgs {

@@ -325,72 +325,128 @@ __global__ void k_vp(float2* __restrict__ vp, const float2* __restrict__ spec, c
   vp[(long)j * mm + q] = v;
 }
 
-// The sub-pixel form of k_vp fused into the inverse row transform: the spectrum
-// ramp_j (e0 F0 + e1 FV) / M^2 is formed (k_vp's arithmetic) in the load of each row, the rows are
-// inverse-transformed in LDS and stored; the caller then runs the inverse column pass.
+// The sub-pixel form of k_vp in three launches, column pass first: rspec holds the row spectra
+// R_P = FFT_rows(P_0) and R_V = FFT_rows(V) (fft_rows_pair); per frame a column pass transforms
+// e0 R_P + e1 R_V along the columns, multiplies by the row-frequency ramp exp(-2 pi i sy f_row) / M
+// and transforms back, then a row pass multiplies by the column-frequency ramp
+// exp(-2 pi i sx f_col) / M and inverse-transforms the rows.  k_vp's 2D ramp is the product of the
+// two and transforms along rows and columns commute, so this is
+// IFFT2(ramp_j (e0 FFT2(P_0) + e1 FFT2(V))) / M^2 (Probe::shift_probe of the frame's probe), in
+// another rounding order than 2D spectra, a ramped inverse row pass and an inverse column pass (one
+// launch more, which measured 0.5 ms per 51x51 / 256^2 / 5-mode iteration slower).
 template <int M>
-__global__ __launch_bounds__(NT) void k_vp_rows(float2* __restrict__ vp, const float2* __restrict__ spec,
+__global__ __launch_bounds__(NT) void k_vp_cols(float2* __restrict__ vp, const float2* __restrict__ rspec,
                                                 const double* __restrict__ evo, const float2* __restrict__ sub,
                                                 const int* __restrict__ gf, int n, int var_any,
                                                 const float2* __restrict__ tw) {
+  constexpr int NC = (plan_min_batch<M, NT>() + 15) / 16 * 16;
+  constexpr int PAIRS = NC * M / 2 / NT;
+  constexpr int MM = M * M;
+  static_assert((NC * M / 2) % NT == 0, "column tile must divide among the threads");
+  __shared__ float2 lds[NC * M];
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y, f = gf[j];
+  const float e0 = var_any ? (float)evo[f] : 1.f, e1 = var_any ? (float)evo[n + f] : 0.f;
+  const float sy = sub[f].y;
+  auto off = [&](int e) {  // e: strip-major element index of the tile (even)
+    return (long)((e >> 4) % M) * M + (e / (M * 16)) * 16 + (e & 15);
+  };
+  const float2* ra = rspec + blockIdx.x * NC;
+  {
+    float4 av[PAIRS], bv[PAIRS];
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const long q = off(2 * (tid + k * NT));
+      av[k] = *reinterpret_cast<const float4*>(ra + q);
+      bv[k] = *reinterpret_cast<const float4*>(ra + MM + q);
+    }
+#pragma unroll
+    for (int k = 0; k < PAIRS; ++k) {
+      const float4 a = av[k], b = bv[k];
+      *reinterpret_cast<float4*>(&lds[2 * (tid + k * NT)]) =
+          make_float4(e0 * a.x + e1 * b.x, e0 * a.y + e1 * b.y, e0 * a.z + e1 * b.z, e0 * a.w + e1 * b.w);
+    }
+  }
+  __syncthreads();
+  fft_lds<M, false, NC, NT, StripLayout<M>, true>(lds, tw, tid);
+  const float inv_m = 1.f / (float)M;
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * NT);
+    float sn, cs;
+    sincosf(sy * freq((e >> 4) % M, M) * -kTwoPi, &sn, &cs);  // both values of the pair share the row
+    float4* p = reinterpret_cast<float4*>(&lds[e]);
+    const float4 t = *p;
+    const float2 u0 = cscale(cmul(make_float2(t.x, t.y), make_float2(cs, sn)), inv_m);
+    const float2 u1 = cscale(cmul(make_float2(t.z, t.w), make_float2(cs, sn)), inv_m);
+    *p = make_float4(u0.x, u0.y, u1.x, u1.y);
+  }
+  __syncthreads();
+  fft_lds<M, true, NC, NT, StripLayout<M>, true>(lds, tw, tid);
+  float2* out = vp + (long)j * MM + blockIdx.x * NC;
+#pragma unroll
+  for (int k = 0; k < PAIRS; ++k) {
+    const int e = 2 * (tid + k * NT);
+    *reinterpret_cast<float4*>(out + off(e)) = *reinterpret_cast<const float4*>(&lds[e]);
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(NT) void k_vp_rows_ramp(float2* __restrict__ vp, const float2* __restrict__ sub,
+                                                     const int* __restrict__ gf, const float2* __restrict__ tw) {
   constexpr int NB = plan_min_batch<M, NT>();
   constexpr int PAIRS = NB * M / 2 / NT;
   static_assert(M % NB == 0 && (NB * M / 2) % NT == 0, "a workgroup's rows must share a plane");
+  static_assert((2 * NT) % M == 0, "a thread's columns must repeat across its rows");
   using L = SwzRowLayout<M, M>;
   __shared__ float2 lds[NB * M];
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * NB;
-  const int j = (int)(row0 / M), f = gf[j];
-  const float e0 = var_any ? (float)evo[f] : 1.f, e1 = var_any ? (float)evo[n + f] : 0.f;
-  const float2 sv = sub[f];
-  constexpr int MM = M * M;
-  float4 av[PAIRS], bv[PAIRS];
+  const float sx = sub[gf[(int)(row0 / M)]].x;
+  const int c = (2 * tid) % M;  // the thread's two columns, the same in every row it holds
+  const float inv_m = 1.f / (float)M;
+  float2 w0, w1;
+  sincosf(sx * freq(c, M) * -kTwoPi, &w0.y, &w0.x);
+  sincosf(sx * freq(c + 1, M) * -kTwoPi, &w1.y, &w1.x);
+  {
+    float4 v[PAIRS];
 #pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * NT);
-    const int q = (int)(row0 % M + e / M) * M + e % M;
-    av[i] = *reinterpret_cast<const float4*>(spec + q);
-    bv[i] = *reinterpret_cast<const float4*>(spec + MM + q);
-  }
+    for (int i = 0; i < PAIRS; ++i) v[i] = *reinterpret_cast<const float4*>(vp + row0 * M + 2 * (tid + i * NT));
 #pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const int e = 2 * (tid + i * NT);
-    const int r = (int)(row0 % M) + e / M, c = e % M;
-    float2 v[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const float2 a = h ? make_float2(av[i].z, av[i].w) : make_float2(av[i].x, av[i].y);
-      const float2 b = h ? make_float2(bv[i].z, bv[i].w) : make_float2(bv[i].x, bv[i].y);
-      const float2 w = make_float2(e0 * a.x + e1 * b.x, e0 * a.y + e1 * b.y);
-      const float nom = sv.x * freq(c + h, M) + sv.y * freq(r, M);
-      float sn, cs;
-      sincosf(nom * -kTwoPi, &sn, &cs);
-      v[h] = cscale(cmul(w, make_float2(cs, sn)), 1.f / (float)MM);
+    for (int i = 0; i < PAIRS; ++i) {
+      const int e = 2 * (tid + i * NT);
+      const float2 u0 = cscale(cmul(make_float2(v[i].x, v[i].y), w0), inv_m);
+      const float2 u1 = cscale(cmul(make_float2(v[i].z, v[i].w), w1), inv_m);
+      *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = make_float4(u0.x, u0.y, u1.x, u1.y);
     }
-    *reinterpret_cast<float4*>(&lds[L::addr(e / M, e % M)]) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
   }
   __syncthreads();
   fft_lds<M, true, NB, NT, L, false>(lds, tw, tid);
 #pragma unroll
   for (int i = 0; i < PAIRS; ++i) {
     const int e = 2 * (tid + i * NT);
-    *reinterpret_cast<float4*>(vp + (row0 + e / M) * M + e % M) =
-        *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
+    *reinterpret_cast<float4*>(vp + row0 * M + e) = *reinterpret_cast<const float4*>(&lds[L::addr(e / M, e % M)]);
   }
 }
 
-static hipError_t vp_rows(float2* vp, const float2* spec, const double* evo, const float2* sub, const int* gf, int n,
-                          int var_any, int G, int m, const float2* tw, hipStream_t s) {
+// The illumination probes of G frames from the row spectra rspec (see k_vp_cols).
+static hipError_t vp_from_row_spectra(float2* vp, const float2* rspec, const double* evo, const float2* sub,
+                                      const int* gf, int n, int var_any, int G, int m, const float2* tw,
+                                      hipStream_t s) {
   switch (m) {
-#define PTYK_VP_CASE(MM)                                                                                        \
-  case MM:                                                                                                      \
-    hipLaunchKernelGGL((k_vp_rows<MM>), dim3((unsigned)((long)G * MM / plan_min_batch<MM, NT>())), dim3(NT), 0, \
-                       s, vp, spec, evo, sub, gf, n, var_any, tw);                                              \
-    return hipGetLastError();
-    PTYK_VP_CASE(128)
-    PTYK_VP_CASE(256)
-    PTYK_VP_CASE(512)
-#undef PTYK_VP_CASE
+#define PTYK_VPC_CASE(MM)                                                                                         \
+  case MM: {                                                                                                      \
+    constexpr int NC = (plan_min_batch<MM, NT>() + 15) / 16 * 16;                                                \
+    hipLaunchKernelGGL((k_vp_cols<MM>), dim3(MM / NC, G), dim3(NT), 0, s, vp, rspec, evo, sub, gf, n, var_any,   \
+                       tw);                                                                                       \
+    hipLaunchKernelGGL((k_vp_rows_ramp<MM>), dim3((unsigned)((long)G * MM / plan_min_batch<MM, NT>())), dim3(NT), \
+                       0, s, vp, sub, gf, tw);                                                                    \
+    return hipGetLastError();                                                                                     \
+  }
+    PTYK_VPC_CASE(128)
+    PTYK_VPC_CASE(256)
+    PTYK_VPC_CASE(512)
+#undef PTYK_VPC_CASE
     default: return hipErrorInvalidValue;
   }
 }
@@ -1559,18 +1615,19 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s));
         modes_early = true;
       }
+      MARK(MLS_T_JOIN);  // the wait is charged to the join, not to the launch before it
       TRY(join_wait());
     }
     MARK(MLS_T_PROBES);
-    if (subpix && !vp_ready) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
+    if (subpix && !vp_ready && !mod_blocks) TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, s));  // spectra of P_0 and V
     // modes 2..K only feed the fused probe-mode update: their inverse row pass is left to it
     const bool defer_modes = probe_rec && !d.apply_multimodal_update && K > 1 && mod_blocks && d.pum_part;
     const bool pu_fused = defer_modes && subpix;
     if (subpix && mod_blocks) {
-      // the shifted illumination probes' spectra formed in the inverse row pass itself
+      // the shifted illumination probes from the row spectra of P_0 and V, column pass first
       if (!vp_ready) {
-        TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
-        TRY(fft_cols_batch(d.vp, G, m, true, d.tw, s));
+        TRY(fft_rows_pair(d.spec, d.probe, d.vmode, m, d.tw, s));
+        TRY(vp_from_row_spectra(d.vp, d.spec, d.evo, d.sub, gf, d.n, var_any ? 1 : 0, G, m, d.tw, s));
       }
     } else {
       LAUNCH(k_vp, gpix, dim3(NT), d.vp, d.spec, d.probe, d.vmode, d.evo, d.sub, gf, d.n, m, subpix, var_any ? 1 : 0);
@@ -1772,9 +1829,8 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     if (pre_side) {
       const int Gn = goffs[g + 2] - goffs[g + 1];
       TRY(fork(0, vp_on_side, false, [&d, m, Gn, gfn, var_any](hipStream_t st) -> hipError_t {
-        TRY(fft2_pair(d.spec, d.probe, d.vmode, m, d.tw, st));
-        TRY(vp_rows(d.vp, d.spec, d.evo, d.sub, gfn, d.n, var_any ? 1 : 0, Gn, m, d.tw, st));
-        return fft_cols_batch(d.vp, Gn, m, true, d.tw, st);
+        TRY(fft_rows_pair(d.spec, d.probe, d.vmode, m, d.tw, st));
+        return vp_from_row_spectra(d.vp, d.spec, d.evo, d.sub, gfn, d.n, var_any ? 1 : 0, Gn, m, d.tw, st);
       }));
     }
     MARK(MLS_T_MODES);
