@@ -198,11 +198,14 @@ int ptyk_mls_begin(ptyk_ctx* ctx, const ptyk_mls_config* cfg, const float* posit
 int ptyk_mls_iteration(ptyk_ctx* ctx, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
                        int32_t n_groups, float* fourier_errors);
 /* Same, timing the main-stream launches with HIP events, summed per kind over the iteration:
- * kernel_ms[12] = {per-iteration steps (ROIs, object norm, illumination, mode
+ * kernel_ms[15] = {per-iteration steps (ROIs, object norm, illumination, mode
  * re-orthogonalisation), illumination probes formed in line (0 when the previous group formed
  * them on the second stream), exit-wave row pass, modulus column pass, inverse row pass, probe
  * update and its shift, preconditioned object direction, LSQ sums and solve, step application,
- * probe modes 2..K, waits for the second stream; then the host time to enqueue the iteration}. */
+ * probe modes 2..K, waits for the second stream; then the own kernel time of the exit-wave row
+ * pass, the modulus column pass and the inverse row pass (start/stop events recorded by their
+ * dispatches: the intervals above also carry the events' own processing on the stream); then
+ * the host time to enqueue the iteration}. */
 int ptyk_mls_iteration_timed(ptyk_ctx* ctx, int32_t iter, const int32_t* group_frames, const int32_t* group_offsets,
                              int32_t n_groups, float* fourier_errors, float* kernel_ms);
 /* MLs state; every non-null buffer is copied out (get) or in (set). */

@@ -197,7 +197,8 @@ def _group_tables(groups):
 
 # launch kinds of ptyk_mls_iteration_timed, in order (include/ptyk.h)
 MLS_KINDS = ("iteration_steps", "illumination_probes", "exit_row_pass", "modulus_column_pass", "inverse_row_pass",
-             "probe_update", "object_direction", "lsq", "step_application", "probe_modes", "side_stream_wait")
+             "probe_update", "object_direction", "lsq", "step_application", "probe_modes", "side_stream_wait",
+             "exit_row_pass_own", "modulus_column_pass_own", "inverse_row_pass_own")
 
 
 class Engine:

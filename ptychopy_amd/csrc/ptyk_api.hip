@@ -1083,11 +1083,16 @@ static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const
   d.use_graphs = c->mls_graphs != 0;
   MlsTimer tm;
   if (kernel_ms) {
-    r = timer_reserve(c, (size_t)16 * n_groups + 64);
+    // marks, then start/stop pairs for the three streamed passes (at most four launches a group)
+    const size_t marks = (size_t)24 * n_groups + 64, pairs = (size_t)4 * n_groups + 8;
+    r = timer_reserve(c, marks + 2 * pairs);
     if (r) return r;
     tm.ev = c->evpool.data();
     tm.cat = c->evcat.data();
-    tm.cap = (int)c->evpool.size();
+    tm.cap = (int)marks;
+    tm.kev = c->evpool.data() + marks;
+    tm.kcat = c->evcat.data() + marks;
+    tm.kcap = (int)pairs;
   }
   const auto t0 = std::chrono::steady_clock::now();
   HIPCHK(c, mls_iteration(d, iter, group_offsets, n_groups, c->stream, kernel_ms ? &tm : nullptr));
@@ -1103,6 +1108,11 @@ static int mls_run(ptyk_ctx* c, int32_t iter, const int32_t* group_frames, const
       float ms = 0.f;
       HIPCHK(c, hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
       if (tm.cat[i] >= 0 && tm.cat[i] < MLS_T_COUNT) kernel_ms[tm.cat[i]] += ms;
+    }
+    for (int i = 0; i < tm.kn; ++i) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, tm.kev[2 * i], tm.kev[2 * i + 1]));
+      if (tm.kcat[i] >= 0 && tm.kcat[i] < MLS_T_COUNT) kernel_ms[tm.kcat[i]] += ms;
     }
     kernel_ms[MLS_T_COUNT] = (float)host_ms;
   }

@@ -12,6 +12,8 @@
 // on-package cache between passes.  Object and probe increments are accumulated per wave into
 // FP64 buffers in frame order (deterministic), and applied once after the last wave, so every
 // pass reads the object and probe as they were at the start of the iteration.
+#include <hip/hip_ext.h>
+
 #include "ptyk_device.h"
 #include "ptyk_internal.h"
 
@@ -1085,6 +1087,18 @@ __global__ __launch_bounds__(64 * K) void k_mls_pass_b(float2* __restrict__ psi,
   if (tid == 0) part[(long)j * BPF + blk] = err;
 }
 
+// A launch timed by its own dispatch's start/stop events when the timer has a pair free (timed
+// iterations), a plain launch otherwise.
+template <typename F, typename... A>
+static void launch_own(LaunchTimer* tm, int category, F kernel, dim3 grid, dim3 block, size_t shm, hipStream_t s,
+                       A... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tm && tm->own(category, &e0, &e1))
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)shm, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+}
+
 template <int M, int K, int CB>
 static hipError_t mls_modulus_k(float2* psi, int G, bool row_waves, const __half* amp, const float* amp_scale,
                                 const int* gf, double* part, const float2* tw, const MlsExitArgs* exit, int c_modes,
@@ -1111,26 +1125,28 @@ static hipError_t mls_modulus_k(float2* psi, int G, bool row_waves, const __half
   if (exit && waves) {
     const int np = exit->np ? exit->np : K * G;
     if constexpr (WAVE_OK)
-      hipLaunchKernelGGL((k_mls_pass_a_w<M, ROW_WAVES>), dim3((unsigned)((np * wt + ROW_WAVES - 1) / ROW_WAVES)),
-                         dim3(64 * ROW_WAVES), 0, s, psi, *exit, G, exit->p0, np, tw);
+      launch_own(tm, MLS_T_OWN_A, k_mls_pass_a_w<M, ROW_WAVES>,
+                 dim3((unsigned)((np * wt + ROW_WAVES - 1) / ROW_WAVES)), dim3(64 * ROW_WAVES), 0, s, psi, *exit, G,
+                 exit->p0, np, tw);
   } else if (exit)
-    hipLaunchKernelGGL((k_mls_pass_a<M>), dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
+    launch_own(tm, MLS_T_OWN_A, k_mls_pass_a<M>, dim3(ga), dim3(PTYK_NT), 0, s, psi, *exit, G, groups, tw);
   else
     hipLaunchKernelGGL((k_dmw_pass_a<M, false>), dim3(ga), dim3(PTYK_NT), 0, s, psi, nullptr, 0, nullptr, nullptr, 0L,
                        groups, tw);
   if (tm) tm->mark(s, MLS_T_PASS_B);
-  hipLaunchKernelGGL((k_mls_pass_b<M, K, CB>), dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp,
-                     amp_scale, gf, part, tw);
+  launch_own(tm, MLS_T_OWN_B, k_mls_pass_b<M, K, CB>, dim3((unsigned)(G * BPF)), dim3(64 * K), lds, s, psi, G, amp,
+             amp_scale, gf, part, tw);
   if (tm) tm->mark(s, MLS_T_PASS_C);
   if (shift && c_modes == 1 && waves) {
     if constexpr (WAVE_OK)
-      hipLaunchKernelGGL((k_mls_pass_c_pu_w<M, ROW_WAVES>), dim3((unsigned)((G * wt + ROW_WAVES - 1) / ROW_WAVES)),
-                         dim3(64 * ROW_WAVES), 0, s, psi, G, shift->pu, shift->op, shift->sub, gf, shift->sign, tw);
+      launch_own(tm, MLS_T_OWN_C, k_mls_pass_c_pu_w<M, ROW_WAVES>,
+                 dim3((unsigned)((G * wt + ROW_WAVES - 1) / ROW_WAVES)), dim3(64 * ROW_WAVES), 0, s, psi, G,
+                 shift->pu, shift->op, shift->sub, gf, shift->sign, tw);
   } else if (shift && c_modes == 1)
-    hipLaunchKernelGGL((k_mls_pass_c_pu<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, groups_c, shift->pu, shift->op,
-                       shift->sub, gf, shift->sign, tw);
+    launch_own(tm, MLS_T_OWN_C, k_mls_pass_c_pu<M>, dim3(gc), dim3(PTYK_NT), 0, s, psi, groups_c, shift->pu,
+               shift->op, shift->sub, gf, shift->sign, tw);
   else
-    hipLaunchKernelGGL((k_dmw_pass_c<M>), dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
+    launch_own(tm, MLS_T_OWN_C, k_dmw_pass_c<M>, dim3(gc), dim3(PTYK_NT), 0, s, psi, 0L, groups_c, tw);
   return hipGetLastError();
 }
 
@@ -1164,11 +1180,11 @@ static hipError_t mls_modulus_m(float2* psi, int K, int G, int cb, bool rw, cons
 }
 
 hipError_t mls_exit_rows(float2* psi, int G, int m, const MlsExitArgs* exit, int p0, int np, const float2* tw,
-                         hipStream_t s) {
+                         hipStream_t s, LaunchTimer* tm) {
   if (!exit || p0 < 0 || np < 1) return hipErrorInvalidValue;
   auto go = [&](auto kern, int T) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)((np * T + ROW_WAVES - 1) / ROW_WAVES)), dim3(64 * ROW_WAVES), 0, s, psi,
-                       *exit, G, p0, np, tw);
+    launch_own(tm, MLS_T_OWN_A, kern, dim3((unsigned)((np * T + ROW_WAVES - 1) / ROW_WAVES)), dim3(64 * ROW_WAVES), 0,
+               s, psi, *exit, G, p0, np, tw);
     return hipGetLastError();
   };
   switch (m) {

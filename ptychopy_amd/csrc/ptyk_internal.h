@@ -65,6 +65,18 @@ struct LaunchTimer {
       cat[n++] = category;
     }
   }
+  // Start/stop pairs recorded by a kernel's own dispatch (hipExtLaunchKernelGGL): the kernel's
+  // own duration, without the stream's event processing that the marks' intervals carry.
+  hipEvent_t* kev = nullptr;  // 2 kcap entries
+  int* kcat = nullptr;        // kcap entries
+  int kcap = 0, kn = 0;
+  bool own(int category, hipEvent_t* start, hipEvent_t* stop) {
+    if (kn >= kcap) return false;
+    *start = kev[2 * kn];
+    *stop = kev[2 * kn + 1];
+    kcat[kn++] = category;
+    return true;
+  }
 };
 typedef LaunchTimer DmTimer;
 
@@ -126,11 +138,13 @@ int mls_modulus_blocks(int m, int k, int cb);
 // probe modes 1..K-1 of a group can be formed before the group's mode 0, whose inputs (the
 // illumination probes, the patches the position step still reads) become final later.
 hipError_t mls_exit_rows(float2* psi, int G, int m, const MlsExitArgs* exit, int p0, int np, const float2* tw,
-                         hipStream_t s);
+                         hipStream_t s, LaunchTimer* tm = nullptr);
 // MLs launch categories of ptyk_mls_iteration_timed (main stream only; the second stream's
 // launches overlap them and are not charged).
+// MLS_T_OWN_A/B/C: the own kernel time of the exit-wave row pass, the modulus column pass and the
+// inverse row pass (start/stop events of their dispatches), beside the marks' intervals.
 enum { MLS_T_ITER, MLS_T_PROBES, MLS_T_PASS_A, MLS_T_PASS_B, MLS_T_PASS_C, MLS_T_PROBE_UPDATE, MLS_T_PRECOND,
-       MLS_T_LSQ, MLS_T_STEP, MLS_T_MODES, MLS_T_JOIN, MLS_T_COUNT };
+       MLS_T_LSQ, MLS_T_STEP, MLS_T_MODES, MLS_T_JOIN, MLS_T_OWN_A, MLS_T_OWN_B, MLS_T_OWN_C, MLS_T_COUNT };
 typedef LaunchTimer MlsTimer;
 // pu_j = IFFT(exp(-2 pi i sign (sub f)) / M^2 FFT(chi_j conj(op_j))) for G planes in two passes,
 // the separable ramp applied as row shifts then column shifts (m in {128, 256, 512}; ptyk_dm_wave.hip).

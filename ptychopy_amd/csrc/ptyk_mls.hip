@@ -1612,7 +1612,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     if (join_lane >= 0) {
       if (vp_ready && mod_blocks && d.row_waves && (m == 128 || m == 256) && K > 1 && G >= join_G) {
         MARK(MLS_T_PASS_A);
-        TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s));
+        TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s, tm));
         modes_early = true;
       }
       MARK(MLS_T_JOIN);  // the wait is charged to the join, not to the launch before it
