@@ -125,7 +125,7 @@ static void free_mls(ptyk_ctx* c) {
   if (d.side2) (void)hipStreamSynchronize(d.side2);
   void* ptrs[] = {d.vmode, d.evo, d.illum, d.pos, d.pos_o, d.roi, d.sub, d.step_obj, d.step_probe, d.ferr, d.flag,
                   d.op, d.vp, d.psi, d.pu, d.spec, d.pum, d.acc, d.precond, d.part, d.gscal, d.gframes,
-                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot,
+                  d.ftot, d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot, d.op_alt, d.dxb_alt,
                   d.goffs_dev, d.gsel};  // tmp, tmp2 in psi
   mls_drop_graphs(d);
   for (void* p : ptrs)
@@ -908,17 +908,18 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   if (!d.enq) d.enq.reset(new SideEnqueuer(c->cfg.device));
   if (g > d.gmax) {
     mls_drop_graphs(d);  // they hold the addresses of the scratch about to move
-    void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,  d.gsel,
-                    d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot};
+    void* ptrs[] = {d.op,  d.vp,       d.psi,      d.pu,       d.gscal,   d.ftot,  d.gsel,    d.op_alt,
+                    d.dxb, d.pos_part, d.pos_ftot, d.pum_part, d.vp_part, d.vp_ftot, d.dxb_alt};
     for (void* p : ptrs)  // tmp and tmp2 live inside psi
       if (p) hipFree(p);
-    d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = d.dxb = d.pum_part = nullptr;
+    d.op = d.vp = d.psi = d.pu = d.tmp = d.tmp2 = d.dxb = d.pum_part = d.op_alt = d.dxb_alt = nullptr;
     d.gsel = nullptr;
     d.gscal = d.ftot = d.pos_part = d.pos_ftot = d.vp_part = d.vp_ftot = nullptr;
     d.gmax = 0;
     const long gm = (long)g * mm;
     const long tcap = std::max<long>(g, d.k);
     HIPCHK(c, dalloc(&d.op, gm));
+    HIPCHK(c, dalloc(&d.op_alt, gm));
     HIPCHK(c, dalloc(&d.vp, gm));
     // exit waves (K x g frames), then tmp (max(g, K) frames), then tmp2 (g frames): the position
     // step's transforms of tmp and tmp2 share the exit waves' FFT launches
@@ -929,6 +930,7 @@ static int mls_reserve(ptyk_ctx* c, int g) {
     HIPCHK(c, dalloc(&d.gscal, (size_t)MLS_S_COEF + g));
     HIPCHK(c, dalloc(&d.ftot, (size_t)8 * g));
     HIPCHK(c, dalloc(&d.dxb, gm));
+    HIPCHK(c, dalloc(&d.dxb_alt, gm));
     HIPCHK(c, dalloc(&d.pos_part, (size_t)4 * g * (d.m / 16)));
     HIPCHK(c, dalloc(&d.pos_ftot, (size_t)4 * g));
     HIPCHK(c, dalloc(&d.pum_part, (size_t)mls_pum_part_len(g, d.k, mm)));

@@ -807,6 +807,11 @@ __global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ ps
   const long mm = (long)M * M;
   const float2* pk = x.probe + (long)k * mm;
   const bool shifted = k == 0 && x.subpix;
+  // mode 0 in parts (the rest of the modes always whole): patches and dX only, or its exit waves
+  // only from the patches already in op
+  const bool exit_wave = !(k == 0 && x.mode0 == 1);
+  const bool patches = k == 0 && x.mode0 != 2;
+  const bool from_op = k == 0 && x.mode0 == 2;
   constexpr int HALF = PAIRS / 2;
 #pragma unroll
   for (int h0 = 0; h0 < PAIRS; h0 += HALF) {
@@ -815,41 +820,52 @@ __global__ __launch_bounds__(64 * W) void k_mls_pass_a_w(float2* __restrict__ ps
 #pragma unroll
     for (int u = 0; u < HALF; ++u) {
       const int r = WT::grow(t, lane, h0 + u), c = WT::lcol(lane, h0 + u);
-      const float2* orow = x.obj + (long)(o.x + r) * x.cols + (o.y + c);
-      o0[u] = orow[0];
-      o1[u] = orow[1];
-      pv[u] = shifted ? *reinterpret_cast<const float4*>(x.vp + (long)j * mm + r * M + c)
-                      : *reinterpret_cast<const float4*>(pk + r * M + c);
+      if (from_op) {
+        const float4 ov = *reinterpret_cast<const float4*>(x.op + (long)j * mm + r * M + c);
+        o0[u] = make_float2(ov.x, ov.y);
+        o1[u] = make_float2(ov.z, ov.w);
+      } else {
+        const float2* orow = x.obj + (long)(o.x + r) * x.cols + (o.y + c);
+        o0[u] = orow[0];
+        o1[u] = orow[1];
+      }
+      if (exit_wave)
+        pv[u] = shifted ? *reinterpret_cast<const float4*>(x.vp + (long)j * mm + r * M + c)
+                        : *reinterpret_cast<const float4*>(pk + r * M + c);
     }
 #pragma unroll
     for (int u = 0; u < HALF; ++u) {
       const int r = WT::grow(t, lane, h0 + u), c = WT::lcol(lane, h0 + u);
-      const float2 p0 = make_float2(pv[u].x, pv[u].y), p1 = make_float2(pv[u].z, pv[u].w);
-      // k_psi's operand order: O * vp for the shifted mode 0, P * O otherwise
-      const float2 a = shifted ? cmul(o0[u], p0) : cmul(p0, o0[u]);
-      const float2 b = shifted ? cmul(o1[u], p1) : cmul(p1, o1[u]);
-      *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, h0 + u), c)]) = make_float4(a.x, a.y, b.x, b.y);
-      if (k == 0) {
+      if (exit_wave) {
+        const float2 p0 = make_float2(pv[u].x, pv[u].y), p1 = make_float2(pv[u].z, pv[u].w);
+        // k_psi's operand order: O * vp for the shifted mode 0, P * O otherwise
+        const float2 a = shifted ? cmul(o0[u], p0) : cmul(p0, o0[u]);
+        const float2 b = shifted ? cmul(o1[u], p1) : cmul(p1, o1[u]);
+        *reinterpret_cast<float4*>(&tile[L::addr(WT::lrow(lane, h0 + u), c)]) = make_float4(a.x, a.y, b.x, b.y);
+      }
+      if (patches) {
         const float4 ov = make_float4(o0[u].x, o0[u].y, o1[u].x, o1[u].y);
         *reinterpret_cast<float4*>(x.op + (long)j * mm + r * M + c) = ov;
         if (x.op2) *reinterpret_cast<float4*>(x.op2 + (long)j * mm + r * M + c) = ov;
       }
     }
   }
-  lds_sync<true>();
-  fft_lds<M, false, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
-  {
-    float4 v[PAIRS];
+  if (exit_wave) {
+    lds_sync<true>();
+    fft_lds<M, false, WT::ROWS, 64, L, false, true, true>(tile, tw, lane);
+    {
+      float4 v[PAIRS];
 #pragma unroll
-    for (int i = 0; i < PAIRS; ++i)
-      v[i] = *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]);
-    WT::template cross<false>(v, t, tw);
-    float2* dst = psi + (long)plane * mm;
+      for (int i = 0; i < PAIRS; ++i)
+        v[i] = *reinterpret_cast<const float4*>(&tile[L::addr(WT::lrow(lane, i), WT::lcol(lane, i))]);
+      WT::template cross<false>(v, t, tw);
+      float2* dst = psi + (long)plane * mm;
 #pragma unroll
-    for (int i = 0; i < PAIRS; ++i)
-      *reinterpret_cast<float4*>(dst + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i)) = v[i];
+      for (int i = 0; i < PAIRS; ++i)
+        *reinterpret_cast<float4*>(dst + (long)WT::grow(t, lane, i) * M + WT::lcol(lane, i)) = v[i];
+    }
   }
-  if (k == 0 && x.dx) {
+  if (patches && x.dx) {
     // the column derivative of the patch rows this wave wrote (read back by the lanes that wrote
     // them): row transform, 2 pi i f_col / M, inverse row transform
     lds_sync<true>();
@@ -1120,7 +1136,7 @@ static hipError_t mls_modulus_k(float2* psi, int G, bool row_waves, const __half
   constexpr bool WAVE_OK = M <= 256;
   const bool waves = WAVE_OK && row_waves;
   const int wt = WaveTile<WAVE_OK ? M : 256>::T;
-  if (exit && (exit->p0 != 0 || exit->np != 0) && !waves) return hipErrorInvalidValue;
+  if (exit && (exit->p0 != 0 || exit->np != 0 || exit->mode0 != 0) && !waves) return hipErrorInvalidValue;
   if (tm) tm->mark(s, MLS_T_PASS_A);
   if (exit && waves) {
     const int np = exit->np ? exit->np : K * G;

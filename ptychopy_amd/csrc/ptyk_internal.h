@@ -132,6 +132,10 @@ struct MlsExitArgs {
   // the exit-wave row pass covers planes [p0, p0 + np) of the K G (np = 0: all of them; a
   // partial range needs the wave-tile pass, m in {128, 256} with row_waves)
   int p0 = 0, np = 0;
+  // mode 0's planes (wave-tile pass only): 0 its exit waves, patches and dX; 1 the patches and
+  // dX only (no exit wave: its illumination probes may not be formed yet); 2 its exit waves only,
+  // from the patches op already holds
+  int mode0 = 0;
 };
 int mls_modulus_blocks(int m, int k, int cb);
 // The exit-wave row pass alone over planes [p0, p0 + np) (wave tiles, m in {128, 256}): the
@@ -273,6 +277,10 @@ struct MlsDev {
   // position step on a second stream (it reads only the group's exit waves, illumination
   // probes and object patches, and writes only these and the positions)
   float2* dxb = nullptr;        // column-derivative images, gmax frames
+  // second object-patch and dX buffers: odd groups use them, so a group's patches can be formed
+  // while the previous group's side work still reads its own
+  float2* op_alt = nullptr;
+  float2* dxb_alt = nullptr;
   double* pos_part = nullptr;   // partials, gmax x (m / 16) x 4
   double* pos_ftot = nullptr;   // per-frame totals, gmax x 4
   float2* pum_part = nullptr;   // chunk partials of the probe-mode updates (mls_pum_part_len)

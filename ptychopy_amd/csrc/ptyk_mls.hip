@@ -1591,6 +1591,10 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
   auto group_body = [&](int g, const int* gf, const int* gfn) -> hipError_t {
     const int G = goffs[g + 1] - goffs[g];
     const bool vp_ready = prefetch && g > 0;  // formed by the previous group's second-stream work
+    // object patches and dX rows alternate between two buffers from group to group: this group's
+    // are formed while the previous group's side work still reads that group's
+    float2* const opg = (g & 1) ? d.op_alt : d.op;
+    float2* const dxg = (g & 1) ? d.dxb_alt : d.dxb;
     bool vp_on_side = false;                  // the variable-mode update runs on stream B this group
     // side pieces waiting for their fork point
     std::function<hipError_t(hipStream_t)> pos_work, vp_work, evo_work;
@@ -1607,12 +1611,16 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     // chi_0 planes 0 .. join_G - 1) when G >= join_G: they go ahead of the join, so the main
     // stream has work while the previous group's side streams finish
     bool modes_early = false;
-    const MlsExitArgs ex0{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, d.op, spec2d ? d.tmp : nullptr,
-                          pos_1d_step ? d.dxb : nullptr};
+    const MlsExitArgs ex0{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, opg, spec2d ? d.tmp : nullptr,
+                          pos_1d_step ? dxg : nullptr};
     if (join_lane >= 0) {
-      if (vp_ready && mod_blocks && d.row_waves && (m == 128 || m == 256) && K > 1 && G >= join_G) {
+      if (vp_ready && mod_blocks && d.row_waves && (m == 128 || m == 256) && K > 1 && G >= join_G && !spec2d) {
+        // with mode 0's object patches and dX rows (into this group's buffers; its exit waves wait
+        // for the illumination probes)
+        MlsExitArgs ex_early = ex0;
+        ex_early.mode0 = 1;
         MARK(MLS_T_PASS_A);
-        TRY(mls_exit_rows(d.psi, G, m, &ex0, G, (K - 1) * G, d.tw, s, tm));
+        TRY(mls_exit_rows(d.psi, G, m, &ex_early, 0, K * G, d.tw, s, tm));
         modes_early = true;
       }
       MARK(MLS_T_JOIN);  // the wait is charged to the join, not to the launch before it
@@ -1639,9 +1647,12 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       // from O, P and vp; the position step's patch copy takes its own forward transform; with
       // the one-dimensional position step, mode 0's exit-wave workgroups also form its dX rows
       MlsExitArgs ex = ex0;
-      if (modes_early) ex.np = G;  // mode 0's planes only
+      if (modes_early) {  // mode 0's exit waves only, from the patches formed ahead of the join
+        ex.np = G;
+        ex.mode0 = 2;
+      }
       // with mode 0 alone in the inverse row pass, that pass also forms the probe update's row shifts
-      const MlsShiftArgs shf{d.pu, d.op, d.sub, -1.f};
+      const MlsShiftArgs shf{d.pu, opg, d.sub, -1.f};
       TRY(mls_modulus_passes(d.psi, K, G, m, d.mod_cb, d.row_waves, d.amp, d.amp_scale, gf, d.part, d.tw, &ex,
                              defer_modes ? 1 : K, pu_fused ? &shf : nullptr, s, tm));
       if (spec2d) TRY(fft2_batch(d.tmp, G, m, false, d.tw, s));
@@ -1650,15 +1661,15 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         // and writes only its own buffers and the positions: it runs beside the rest of the
         // group (forked here rather than after the step application, which measured 0.8 ms per
         // 51x51 / 256^2 / 5-mode iteration slower)
-        pos_work = [&d, G, m, gf](hipStream_t st) -> hipError_t {
+        pos_work = [&d, G, m, gf, opg, dxg](hipStream_t st) -> hipError_t {
           // dX came with the exit waves (k_mls_pass_a)
-          const int pch = pos_1d(d.op, d.dxb, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
+          const int pch = pos_1d(opg, dxg, d.vp, d.psi, G, m, d.tw, d.pos_part, st);
           hipLaunchKernelGGL(k_pos_update, dim3(1), dim3(1024), 0, st, d.pos_part, pch, gf, G, d.pos, d.pos_ftot);
           return hipGetLastError();
         };
       }
     } else {
-      LAUNCH(k_psi, gpix, dim3(NT), d.psi, d.op, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
+      LAUNCH(k_psi, gpix, dim3(NT), d.psi, opg, pos_step ? d.tmp : nullptr, d.vp, d.obj, d.probe, d.roi, gf, G, K,
              m, d.cols, subpix);
       CHK();
       // with the position step, tmp (the object patch, right after the exit-wave buffer) rides
@@ -1697,9 +1708,9 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
             if (pu_fused)
               TRY(mls_shift_cols(d.pu, G, m, d.sub, gf, -1.f, d.tw, s));
             else
-              TRY(mls_shift_passes(d.pu, chi0, d.op, G, m, d.sub, gf, -1.f, d.tw, s));
+              TRY(mls_shift_passes(d.pu, chi0, opg, G, m, d.sub, gf, -1.f, d.tw, s));
           } else {
-            LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, d.op, gm);
+            LAUNCH(k_pu, g1(gm), dim3(NT), d.pu, chi0, opg, gm);
             CHK();
             if (subpix) {
               TRY(fft2_batch(d.pu, G, m, false, d.tw, s));
@@ -1708,7 +1719,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
               TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
             }
           }
-          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
+          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
           if (vp_side) {
             // the variable-mode update needs only pu, pum, the illumination and V, and writes
             // only V, its own scratch and the coefficients: it runs on the second stream beside
@@ -1731,7 +1742,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
             vp_on_side = true;
           }
         } else {
-          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, chi, d.op, d.illum, d.roi, gf, G, m, d.cols, fw, 1);
+          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, chi, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 1);
         }
         CHK();
       }
@@ -1757,7 +1768,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         const bool scatter = d.delta_p != 0;
         if (!scatter) LAUNCH(k_dir, gpix, dim3(NT), d.tmp2, chi, d.vp, pl, m);
         LAUNCH(k_lsq_partial, gch, dim3(NT), d.tmp2, scatter ? d.precond : nullptr, d.roi, gf, d.cols, d.vp, d.pum,
-               d.op, chi0, m, d.part);
+               opg, chi0, m, d.part);
         CHK();
       }
       const double lam_mm = (double)(float)(1.1920928955078125e-07 / mm) * mm;
@@ -1788,9 +1799,9 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
           // the evolution dots need the updated V (second stream, in order) and, with variable
           // intensity, the updated probe (this stream): they follow on the second stream, beside
           // the probe-mode update, which touches neither evo, V nor probe mode 1
-          evo_work = [&d, G, m, mm, ch, gch, gf, chi0](hipStream_t st) -> hipError_t {
+          evo_work = [&d, G, m, mm, ch, gch, gf, chi0, opg](hipStream_t st) -> hipError_t {
             const float2* page2 = d.variable_intensity ? d.probe : nullptr;
-            hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, st, d.op, d.vmode, page2, chi0, m, d.part);
+            hipLaunchKernelGGL(k_frame_dots, gch, dim3(NT), 0, st, opg, d.vmode, page2, chi0, m, d.part);
             hipLaunchKernelGGL(k_evo_update, dim3(1), dim3(1024), 0, st, d.part, ch, gf, G, mm, d.evo, d.n,
                                page2 ? 2 : 1, d.ftot);
             return hipGetLastError();
@@ -1810,12 +1821,12 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
           // with variable intensity too, both evolution updates take one pass (neither changes
           // op, chi, the variable mode or the probe, and they write disjoint halves of evo)
           const float2* page2 = d.variable_intensity ? d.probe : nullptr;
-          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.vmode, page2, chi0, m, d.part);
+          LAUNCH(k_frame_dots, gch, dim3(NT), opg, d.vmode, page2, chi0, m, d.part);
           LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, page2 ? 2 : 1, d.ftot);
           CHK();
         }
         if (d.variable_intensity && !d.variable_probe) {
-          LAUNCH(k_frame_dots, gch, dim3(NT), d.op, d.probe, nullptr, chi0, m, d.part);
+          LAUNCH(k_frame_dots, gch, dim3(NT), opg, d.probe, nullptr, chi0, m, d.part);
           LAUNCH(k_evo_update, dim3(1), dim3(1024), d.part, ch, gf, G, mm, d.evo, d.n, 0, d.ftot);
           CHK();
         }
@@ -1835,10 +1846,10 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     }
     MARK(MLS_T_MODES);
     if (defer_modes) {
-      TRY(mls_mode_updates(d.probe, d.psi, d.op, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
+      TRY(mls_mode_updates(d.probe, d.psi, opg, d.illum, d.roi, gf, K, G, m, d.cols, fw, d.gscal + MLS_S_BP,
                            d.pum_part, d.pum_fpc, d.row_waves, d.tw, s));
     } else if (fuse_modes) {
-      LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, d.op, d.illum, d.roi, gf, G, m,
+      LAUNCH(k_pum_modes, dim3((mm + NT - 1) / NT, K - 1), dim3(NT), d.probe, d.psi, opg, d.illum, d.roi, gf, G, m,
              d.cols, fw, d.gscal);
       CHK();
     }
@@ -1885,7 +1896,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     const uint64_t key = (uint64_t)G | (uint64_t)Gn << 12 | (uint64_t)(g > 0) << 24 |
                          (uint64_t)(iter >= d.pps) << 25 | (uint64_t)var_update << 26 | (uint64_t)probe_rec << 27 |
                          (uint64_t)obj_rec_iter << 28 | (uint64_t)(iter < 10) << 29 | (uint64_t)d.mod_cb << 30 |
-                         (uint64_t)d.pum_fpc << 36 | (uint64_t)d.row_waves << 44;
+                         (uint64_t)d.pum_fpc << 36 | (uint64_t)d.row_waves << 44 | (uint64_t)(g & 1) << 45;
     auto it = d.graphs.find(key);
     if (it == d.graphs.end()) {
       if (d.graphs.size() >= 64) mls_drop_graphs(d);
