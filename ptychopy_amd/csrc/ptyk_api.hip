@@ -900,7 +900,8 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
   if (!d.side) {
     HIPCHK(c, hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
-    for (hipEvent_t& ev : d.ev_side) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    // fork events: timing-enabled, since a launch's own dispatch may record one (hipExtLaunchKernelGGL)
+    for (hipEvent_t& ev : d.ev_side) HIPCHK(c, hipEventCreate(&ev));
     HIPCHK(c, hipStreamCreateWithFlags(&d.side2, hipStreamNonBlocking));
     for (hipEvent_t& ev : d.ev_join) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&d.ev_vpdone, hipEventDisableTiming));

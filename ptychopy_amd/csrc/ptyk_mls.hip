@@ -16,6 +16,7 @@
 //    forward and an inverse (Probe::shift_probe on the combined probe, Probe.cpp:1261-1337);
 //    equal up to FP32 rounding by linearity;
 //  * 1/M^2 normalisations (powers of two) are folded into the spectral multiplies.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1611,6 +1612,8 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     // chi_0 planes 0 .. join_G - 1) when G >= join_G: they go ahead of the join, so the main
     // stream has work while the previous group's side streams finish
     bool modes_early = false;
+    bool point_bound = false;   // the probe-update average's launch recorded the first fork event
+    bool point2_bound = false;  // the step application's launch recorded the second
     const MlsExitArgs ex0{d.obj, d.cols, d.probe, d.vp, d.roi, gf, subpix, opg, spec2d ? d.tmp : nullptr,
                           pos_1d_step ? dxg : nullptr};
     if (join_lane >= 0) {
@@ -1719,7 +1722,17 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
               TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
             }
           }
-          LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
+          if (defer_join && (static_cast<bool>(pos_work) || vp_side) && n_points < MLS_SIDE_FORKS) {
+            // the side pieces' fork event is the end of this launch itself (recorded by its
+            // dispatch, with no marker between it and the next launch on this stream)
+            hipExtLaunchKernelGGL(k_pum, g1(mm), dim3(NT), 0, s, nullptr, d.ev_side[n_points], 0, d.pum,
+                                  (const float2*)d.pu, (const float2*)opg, (const float*)d.illum,
+                                  (const int2*)d.roi, gf, G, m, d.cols, fw, 0);
+            ++n_points;
+            point_bound = true;
+          } else {
+            LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
+          }
           if (vp_side) {
             // the variable-mode update needs only pu, pum, the illumination and V, and writes
             // only V, its own scratch and the coefficients: it runs on the second stream beside
@@ -1750,7 +1763,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       if (ll == 1 && (pos_work || vp_work)) {
         // the position step (its inputs final since the inverse row pass) and the variable-mode
         // update (since the probe-update average) fork here, at one point
-        TRY(fork_point());
+        if (!point_bound) TRY(fork_point());
         if (pos_work) TRY(fork(0, false, false, std::move(pos_work)));
         if (vp_work) TRY(fork(1, false, true, std::move(vp_work)));
       }
@@ -1780,9 +1793,23 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       MARK(MLS_T_STEP);
       {  // object and probe steps in one launch (disjoint arrays, both after the step solve)
         const long no = obj_rec ? canvas : 0, np = probe_rec && ll <= K ? (long)mm : 0;
-        if (no + np > 0)
+        // the second fork point (evolution dots, next group's illumination probes) is the end of
+        // this launch when nothing else of the group runs on this stream before that point
+        const bool evo_side = var_update && d.variable_probe && vp_side;
+        const bool bind2 = defer_join && ll == 1 && no + np > 0 && (fuse_modes || ll_range == 1) &&
+                           (!pos_step || pos_1d_step) && (!var_update || evo_side) &&
+                           (evo_side || (prefetch && g + 1 < n_groups)) && n_points < MLS_SIDE_FORKS;
+        if (bind2) {
+          hipExtLaunchKernelGGL(k_axpy_beta, g1((no + 1) / 2 + (np + 1) / 2), dim3(NT), 0, s, nullptr,
+                                d.ev_side[n_points], 0, d.obj, (const float2*)d.precond, no,
+                                (const double*)(d.gscal + MLS_S_BO), d.probe + (long)(ll - 1) * mm,
+                                (const float2*)d.pum, np, (const double*)(d.gscal + MLS_S_BP));
+          ++n_points;
+          point2_bound = true;
+        } else if (no + np > 0) {
           LAUNCH(k_axpy_beta, g1((no + 1) / 2 + (np + 1) / 2), dim3(NT), d.obj, d.precond, no, d.gscal + MLS_S_BO,
                  d.probe + (long)(ll - 1) * mm, d.pum, np, d.gscal + MLS_S_BP);
+        }
       }
       CHK();
       if (pos_step && ll == 1 && !pos_1d_step) {
@@ -1835,7 +1862,8 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
     // the evolution dots (inputs final since the step application; nothing runs between it and
     // here on this stream) and the next group's illumination probes fork at one point
     const bool pre_side = prefetch && g + 1 < n_groups;
-    if (evo_work || pre_side) TRY(fork_point());
+    if ((evo_work || pre_side) && !point2_bound) TRY(fork_point());
+    if (point2_bound && !evo_work && !pre_side) return hipErrorInvalidValue;  // a bound point left unused
     if (evo_work) TRY(fork(1, false, false, std::move(evo_work)));
     if (pre_side) {
       const int Gn = goffs[g + 2] - goffs[g + 1];
