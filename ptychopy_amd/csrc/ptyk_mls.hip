@@ -538,11 +538,12 @@ __global__ void k_ramp(float2* __restrict__ x, const float2* __restrict__ sub, c
 
 // probe_update_m = mean_j (p / (p + 0.01 MAX_ILLUM))_roi_j probe_update_j  (MLS.cpp:251-279,
 // d_addFactorDivide utilitiesKernels.cu:171-179).  on_the_fly: probe_update_j = chi conj(op_j).
-__global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, const float2* __restrict__ op,
-                      const float* __restrict__ illum, const int2* __restrict__ roi, const int* __restrict__ gf, int G,
-                      int m, int cols, float fw, int on_the_fly) {
+__device__ __forceinline__ void pum_block(int block, float2* __restrict__ pum, const float2* __restrict__ src,
+                                          const float2* __restrict__ op, const float* __restrict__ illum,
+                                          const int2* __restrict__ roi, const int* __restrict__ gf, int G, int m,
+                                          int cols, float fw, int on_the_fly) {
   const int mm = m * m;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = block * NT + threadIdx.x;
   if (q >= mm) return;
   const int r = q / m, c = q % m;
   float2 acc = make_float2(0.f, 0.f);
@@ -580,6 +581,13 @@ __global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src, 
     add(p, u);
   }
   pum[q] = cscale(acc, (float)(1.0 / G));
+}
+
+__global__ void k_pum(float2* __restrict__ pum, const float2* __restrict__ src,
+                                            const float2* __restrict__ op, const float* __restrict__ illum,
+                                            const int2* __restrict__ roi, const int* __restrict__ gf, int G, int m,
+                                            int cols, float fw, int on_the_fly) {
+  pum_block(blockIdx.x, pum, src, op, illum, roi, gf, G, m, cols, fw, on_the_fly);
 }
 
 // Probe updates of modes 2..K in one launch (MLS.cpp:251-279 for ll >= 2, no sub-pixel shift):
@@ -630,15 +638,15 @@ __device__ __forceinline__ float2 obj_dir(const float2* __restrict__ chi, const 
 // in group order.
 constexpr int PRE_RPT = 4;
 template <int RPT>
-__global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const float* __restrict__ illum,
-                                                const float2* __restrict__ chi, const float2* __restrict__ vp,
-                                                const float2* __restrict__ pl, const int2* __restrict__ roi,
-                                                const int* __restrict__ gf, int G, int m, int rows, int cols, float fd,
-                                                int scatter) {
+__device__ __forceinline__ void precond_tile(int bx, int by, float2* __restrict__ pre, const float* __restrict__ illum,
+                                             const float2* __restrict__ chi, const float2* __restrict__ vp,
+                                             const float2* __restrict__ pl, const int2* __restrict__ roi,
+                                             const int* __restrict__ gf, int G, int m, int rows, int cols, float fd,
+                                             int scatter) {
   constexpr int TH = 4 * RPT;
   __shared__ int2 lr[NT];
   const int mm = m * m;
-  const int r0 = blockIdx.y * TH, c0 = blockIdx.x * 64;
+  const int r0 = by * TH, c0 = bx * 64;
   const int ty = threadIdx.x >> 6, c = c0 + (threadIdx.x & 63);
   float2 u[RPT];
   float il[RPT];
@@ -704,6 +712,35 @@ __global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const 
       }
       pre[(long)r * cols + c] = v;
     }
+  }
+}
+
+template <int RPT>
+__global__ __launch_bounds__(NT) void k_precond(float2* __restrict__ pre, const float* __restrict__ illum,
+                                                const float2* __restrict__ chi, const float2* __restrict__ vp,
+                                                const float2* __restrict__ pl, const int2* __restrict__ roi,
+                                                const int* __restrict__ gf, int G, int m, int rows, int cols, float fd,
+                                                int scatter) {
+  precond_tile<RPT>(blockIdx.x, blockIdx.y, pre, illum, chi, vp, pl, roi, gf, G, m, rows, cols, fd, scatter);
+}
+
+// The probe-update average (mode 1, from the shifted probe updates) and the object direction in
+// one launch: they read disjoint inputs and write disjoint outputs.  The first n_pum blocks
+// (one frame loop per probe pixel, the long ones) are dispatched first; the canvas tiles, short,
+// fill the rest of the chip behind them.  Each block's arithmetic is that of k_pum / k_precond.
+template <int RPT>
+__global__ __launch_bounds__(NT) void k_pum_precond(float2* __restrict__ pum, const float2* __restrict__ pu,
+                                                    float fw, int n_pum, float2* __restrict__ pre,
+                                                    const float* __restrict__ illum, const float2* __restrict__ chi,
+                                                    const float2* __restrict__ vp, const float2* __restrict__ pl,
+                                                    const int2* __restrict__ roi, const int* __restrict__ gf, int G,
+                                                    int m, int rows, int cols, float fd, int scatter, int tiles_x) {
+  const int b = blockIdx.x;
+  if (b < n_pum) {
+    pum_block(b, pum, pu, nullptr, illum, roi, gf, G, m, cols, fw, 0);
+  } else {
+    const int t = b - n_pum;
+    precond_tile<RPT>(t % tiles_x, t / tiles_x, pre, illum, chi, vp, pl, roi, gf, G, m, rows, cols, fd, scatter);
   }
 }
 
@@ -1703,6 +1740,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
       const bool obj_rec = obj_rec_iter && (d.apply_multimodal_update || ll <= d.n_objects);
       const int llp = std::min(K, ll);
       const float2* chi = d.psi + (long)(ll - 1) * gm;
+      bool pre_done = false;  // the object direction already ran with the probe-update average
       if (probe_rec) {
         if (ll == 1) {
           if (subpix && mod_blocks) {
@@ -1722,17 +1760,33 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
               TRY(fft2_batch(d.pu, G, m, true, d.tw, s));
             }
           }
+          // with the object update, mode 1's object direction (chi0 and vp, final since the inverse
+          // pass) runs in the same launch as the probe-update average
+          pre_done = obj_rec;
+          const int n_pum = (mm + NT - 1) / NT;
+          const dim3 g_pp((unsigned)n_pum + pre_tiles.x * pre_tiles.y);
+          const int scatter = d.delta_p != 0 ? 1 : 0;
+          hipEvent_t stop = nullptr;
           if (defer_join && (static_cast<bool>(pos_work) || vp_side) && n_points < MLS_SIDE_FORKS) {
             // the side pieces' fork event is the end of this launch itself (recorded by its
             // dispatch, with no marker between it and the next launch on this stream)
-            hipExtLaunchKernelGGL(k_pum, g1(mm), dim3(NT), 0, s, nullptr, d.ev_side[n_points], 0, d.pum,
-                                  (const float2*)d.pu, (const float2*)opg, (const float*)d.illum,
-                                  (const int2*)d.roi, gf, G, m, d.cols, fw, 0);
-            ++n_points;
+            stop = d.ev_side[n_points++];
             point_bound = true;
-          } else {
-            LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
           }
+          if (pre_done && stop)
+            hipExtLaunchKernelGGL(k_pum_precond<PRE_RPT>, g_pp, dim3(NT), 0, s, nullptr, stop, 0, d.pum,
+                                  (const float2*)d.pu, fw, n_pum, d.precond, (const float*)d.illum, chi0,
+                                  (const float2*)d.vp, (const float2*)nullptr, (const int2*)d.roi, gf, G, m, d.rows,
+                                  d.cols, fd, scatter, (int)pre_tiles.x);
+          else if (pre_done)
+            LAUNCH(k_pum_precond<PRE_RPT>, g_pp, dim3(NT), d.pum, d.pu, fw, n_pum, d.precond, d.illum, chi0, d.vp,
+                   (const float2*)nullptr, d.roi, gf, G, m, d.rows, d.cols, fd, scatter, (int)pre_tiles.x);
+          else if (stop)
+            hipExtLaunchKernelGGL(k_pum, g1(mm), dim3(NT), 0, s, nullptr, stop, 0, d.pum, (const float2*)d.pu,
+                                  (const float2*)opg, (const float*)d.illum, (const int2*)d.roi, gf, G, m, d.cols, fw,
+                                  0);
+          else
+            LAUNCH(k_pum, g1(mm), dim3(NT), d.pum, d.pu, opg, d.illum, d.roi, gf, G, m, d.cols, fw, 0);
           if (vp_side) {
             // the variable-mode update needs only pu, pum, the illumination and V, and writes
             // only V, its own scratch and the coefficients: it runs on the second stream beside
@@ -1768,7 +1822,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
         if (vp_work) TRY(fork(1, false, true, std::move(vp_work)));
       }
       MARK(MLS_T_PRECOND);
-      if (obj_rec) {
+      if (obj_rec && !pre_done) {
         const int scatter = d.delta_p != 0 ? 1 : 0;
         LAUNCH(k_precond<PRE_RPT>, pre_tiles, dim3(NT), d.precond, d.illum, chi, d.vp, pl, d.roi, gf, G, m, d.rows, d.cols, fd,
                scatter);
