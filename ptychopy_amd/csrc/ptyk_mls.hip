@@ -1790,7 +1790,7 @@ static hipError_t mls_iteration_body(MlsDev& d, int iter, const int* goffs, int 
           if (vp_side) {
             // the variable-mode update needs only pu, pum, the illumination and V, and writes
             // only V, its own scratch and the coefficients: it runs on the second stream beside
-            // the object direction, the LSQ steps and their application
+            // the LSQ steps and their application (and the object direction when that runs apart)
             CHK();
             vp_work = [&d, G, m, mm, ch, gch, gf](hipStream_t st) -> hipError_t {
               const int nb = (mm + NT - 1) / NT;
