@@ -900,11 +900,16 @@ static int mls_reserve(ptyk_ctx* c, int g) {
   const long nb = (mm + PTYK_NT - 1) / PTYK_NT;
   if (!d.side) {
     HIPCHK(c, hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
-    // fork events: timing-enabled, since a launch's own dispatch may record one (hipExtLaunchKernelGGL)
-    for (hipEvent_t& ev : d.ev_side) HIPCHK(c, hipEventCreate(&ev));
+    // fork events: timing-enabled, since a launch's own dispatch may record one (hipExtLaunchKernelGGL).
+    // The fork, join and variable-mode events order streams of this one device only: each kernel's
+    // own end-of-kernel release (device scope) makes its writes visible to the other streams, and
+    // the host reads results through copies queued after a synchronisation, so these events skip
+    // the system-scope fences (an L2 write-back and invalidate on every record and wait)
+    for (hipEvent_t& ev : d.ev_side) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
     HIPCHK(c, hipStreamCreateWithFlags(&d.side2, hipStreamNonBlocking));
-    for (hipEvent_t& ev : d.ev_join) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    HIPCHK(c, hipEventCreateWithFlags(&d.ev_vpdone, hipEventDisableTiming));
+    for (hipEvent_t& ev : d.ev_join)
+      HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIPCHK(c, hipEventCreateWithFlags(&d.ev_vpdone, hipEventDisableTiming | hipEventDisableSystemFence));
   }
   if (!d.enq) d.enq.reset(new SideEnqueuer(c->cfg.device));
   if (g > d.gmax) {
