@@ -636,7 +636,7 @@ __device__ __forceinline__ float2 obj_dir(const float2* __restrict__ chi, const 
 // 1 / (p + delta_p MAX_ILLUM) (d_object_sum_update_Gfun, utilitiesKernels.cu:181-189).
 // Gather by canvas tile of (4 RPT) x 64 pixels, RPT rows per thread, over the group's frames
 // in group order.
-constexpr int PRE_RPT = 4;
+constexpr int PRE_RPT = 2;
 template <int RPT>
 __device__ __forceinline__ void precond_tile(int bx, int by, float2* __restrict__ pre, const float* __restrict__ illum,
                                              const float2* __restrict__ chi, const float2* __restrict__ vp,
